@@ -1,0 +1,310 @@
+"""ctypes binding of libcroma_hip.so (the C ABI declared in include/croma_hip.h).
+
+This is the Python-side mirror of the reference's C entry points for the
+per-tracer Fokker-Planck update; the arithmetic runs in the HIP library.
+Importing this module loads the in-tree library and raises if it is missing:
+there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcroma_hip.so")
+
+OK, EINVAL, EPIVOT, ENOMEM, EDEVICE, ENONFINITE, EBUDGET, ECANCELED, EUNSUPPORTED, EBUSY = (
+    0, -1, -2, -3, -4, -5, -6, -7, -8, -9)
+PATH_FUSED, PATH_PLAIN = 0, 1
+SPECIES_ELECTRON, SPECIES_ELECTRON_PROTON = 1, 3
+DPP_ASA, DPP_TTD, DPP_OFF, DPP_DIRECT = 1, 2, 3, 4
+EMIS_FP32, EMIS_MFMA_XF32, EMIS_MFMA_BF16, EMIS_FP64 = 0, 1, 2, 3
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_up = C.POINTER(C.c_ubyte)
+
+
+class CromaHipError(RuntimeError):
+    def __init__(self, rc, what):
+        super().__init__(f"{what} failed (rc={rc}): {last_error()}")
+        self.rc = rc
+
+
+class GridSpec(C.Structure):
+    _fields_ = [("n", C.c_int), ("pmin", C.c_double), ("pmax", C.c_double),
+                ("pemin", C.c_double), ("pemax", C.c_double)]
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("dpp_model", C.c_int), ("ttd_tacc_model", C.c_int), ("reacc_window_mode", C.c_int),
+        ("tracer_nsub_max", C.c_int), ("coeff_interp_min_steps", C.c_int),
+        ("coeff_interp_max_segments", C.c_int), ("psi", C.c_double), ("mach_limit", C.c_double),
+        ("eta_dpp_cap", C.c_double), ("t_acc_direct_gyr", C.c_double), ("t_off_gyr", C.c_double),
+        ("t_acc_off_gyr", C.c_double), ("f_eddy", C.c_double), ("tracer_nsub_safety", C.c_double),
+        ("epmax", C.c_double),
+    ]
+
+
+class Tile(C.Structure):
+    _fields_ = [("rows_per_lane", C.c_int), ("waves_per_block", C.c_int), ("blocks_per_cu", C.c_int)]
+
+
+class Times(C.Structure):
+    _fields_ = [
+        ("nsub_estimate_ms", C.c_double), ("bucket_build_ms", C.c_double), ("ic_rows_ms", C.c_double),
+        ("h2d_ms", C.c_double), ("solve_ms", C.c_double), ("d2h_ms", C.c_double), ("total_ms", C.c_double),
+        ("cell_steps", C.c_longlong), ("species_steps", C.c_longlong),
+    ]
+
+
+class Chunk(C.Structure):
+    _fields_ = [
+        ("ntracer", C.c_int), ("n", C.c_int), ("species", C.c_int), ("dpp_model", C.c_int), ("nsnap", C.c_int),
+        ("z", _dp), ("dt_snap", _dp), ("bg", _dp), ("crp", _dp), ("cre", _dp), ("qpi", _dp), ("qepri", _dp),
+        ("mass", _dp),
+    ]
+
+
+class ChunkStats(C.Structure):
+    _fields_ = [("submit_s", C.c_double), ("admit_s", C.c_double), ("done_s", C.c_double),
+                ("cell_steps", C.c_longlong), ("status", C.c_int)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("submitted", C.c_longlong), ("completed", C.c_longlong), ("cancelled", C.c_longlong),
+                ("failed", C.c_longlong), ("cell_steps", C.c_longlong), ("budget_bytes", C.c_size_t),
+                ("in_use_bytes", C.c_size_t), ("peak_bytes", C.c_size_t), ("active", C.c_int),
+                ("queued", C.c_int), ("iterations", C.c_longlong)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        L.croma_hip_last_error.restype = C.c_char_p
+        L.croma_hip_context_create.restype = C.c_void_p
+        L.croma_hip_context_create.argtypes = [C.POINTER(GridSpec), C.POINTER(Params)]
+        for name in ("croma_hip_context_destroy", "croma_hip_batch_destroy"):
+            getattr(L, name).argtypes = [C.c_void_p]
+            getattr(L, name).restype = None
+        L.croma_hip_batch_create.restype = C.c_void_p
+        L.croma_hip_batch_create.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    try:
+        v = lib().croma_hip_last_error()
+        return v.decode() if v else ""
+    except Exception:  # pragma: no cover
+        return ""
+
+
+def _chk(rc, what):
+    if rc != 0:
+        raise CromaHipError(rc, what)
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.flags.c_contiguous, "arrays must be C-contiguous"
+    if a.dtype == np.float64:
+        return a.ctypes.data_as(_dp)
+    if a.dtype == np.int32:
+        return a.ctypes.data_as(_ip)
+    if a.dtype == np.uint8:
+        return a.ctypes.data_as(_up)
+    raise TypeError(a.dtype)
+
+
+def f64(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+def default_params(**kw) -> Params:
+    p = Params()
+    lib().croma_hip_default_params(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def is_available() -> bool:
+    return bool(lib().croma_hip_is_available())
+
+
+def device_name() -> str:
+    buf = C.create_string_buffer(256)
+    _chk(lib().croma_hip_get_current_device_id(buf, 256), "device id")
+    return buf.value.decode()
+
+
+class Context:
+    """Device, grid tables and streams for one grid size n."""
+
+    def __init__(self, n, params=None, pmin=-1.0, pmax=8.0, pemin=0.0, pemax=6.0):
+        gs = GridSpec(n, pmin, pmax, pemin, pemax)
+        self.n = n
+        self.params = params if params is not None else default_params()
+        self.h = lib().croma_hip_context_create(C.byref(gs), C.byref(self.params))
+        if not self.h:
+            raise CromaHipError(-1, "context create")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().croma_hip_context_destroy(C.c_void_p(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, params):
+        self.params = params
+        _chk(lib().croma_hip_context_set_params(C.c_void_p(self.h), C.byref(params)), "set_params")
+
+    def set_tile(self, rows_per_lane, waves_per_block, blocks_per_cu):
+        t = Tile(rows_per_lane, waves_per_block, blocks_per_cu)
+        _chk(lib().croma_hip_context_set_tile(C.c_void_p(self.h), C.byref(t)), "set_tile")
+
+    def get_tile(self, protons=False):
+        t = Tile()
+        _chk(lib().croma_hip_context_get_tile(C.c_void_p(self.h), int(protons), C.byref(t)), "get_tile")
+        return (t.rows_per_lane, t.waves_per_block, t.blocks_per_cu)
+
+    def autotune(self, protons=False, ntracer=16384):
+        t = Tile()
+        _chk(lib().croma_hip_autotune(C.c_void_p(self.h), int(protons), int(ntracer), C.byref(t)), "autotune")
+        return (t.rows_per_lane, t.waves_per_block, t.blocks_per_cu)
+
+    def grid(self, species):
+        n = self.n
+        p, dp, E = np.zeros(n), np.zeros(n), np.zeros(n)
+        m1, p1 = C.c_double(), C.c_double()
+        _chk(lib().croma_hip_context_grid(C.c_void_p(self.h), species, _p(p), _p(dp), _p(E), C.byref(m1),
+                                          C.byref(p1)), "grid")
+        return p, dp, E, (m1.value, p1.value)
+
+    def ic_cooling_row(self, z):
+        row = np.zeros(self.n)
+        m1, p1 = C.c_double(), C.c_double()
+        _chk(lib().croma_hip_ic_cooling_row(C.c_void_p(self.h), C.c_double(z), _p(row), C.byref(m1), C.byref(p1)),
+             "ic_cooling_row")
+        return row, m1.value, p1.value
+
+    def nsub(self, dt_snap, n_gas, kbt, b_field, divv, ric_lo, ric_hi, l_turb, dv):
+        nt = len(n_gas)
+        ns = np.zeros(nt, np.int32)
+        non = np.zeros(nt, np.int32)
+        _chk(lib().croma_hip_nsub_estimate(C.c_void_p(self.h), nt, C.c_double(dt_snap), _p(f64(n_gas)),
+                                           _p(f64(kbt)), _p(f64(b_field)), _p(f64(divv)), C.c_double(ric_lo),
+                                           C.c_double(ric_hi), _p(f64(l_turb)), _p(f64(dv)), _p(ns), _p(non)),
+             "nsub")
+        return ns, non
+
+    def secondary_weights(self):
+        fqe = np.zeros(self.n * self.n)
+        npmin = np.zeros(self.n, np.int32)
+        _chk(lib().croma_hip_secondary_weights(C.c_void_p(self.h), _p(fqe), _p(npmin)), "secondary_weights")
+        return fqe, npmin
+
+    def execute_snapshot(self, nsub, n_on, allow_interp, elapsed, dt_snap, bg_curr, rad_ic_row, ric_m1, ric_p1,
+                         bg_next, rad_ic_row_next, ric_m1n, ric_p1n, qpi, qepri, fqe, npmin, mass, dis, crp, cre,
+                         t_acc_direct=0.0):
+        """Drop-in for tracer_fp_batch_execute_parallel (host arrays; crp/cre updated in place)."""
+        nt = len(nsub)
+        n = self.n
+        ns = np.ascontiguousarray(nsub, np.int32)
+        no = np.ascontiguousarray(n_on, np.int32)
+        cad = np.ones(nt, np.int32)
+        ric_zero = np.ascontiguousarray(np.repeat(f64(rad_ic_row)[:, None], nt, axis=1))
+        m1z, p1z = np.full(nt, ric_m1), np.full(nt, ric_p1)
+        bgc = [f64(bg_curr[k]) for k in range(8)]
+        bgn = [f64(bg_next[k]) for k in range(8)]
+        times = Times()
+        bc, mn, mx = C.c_int(), C.c_int(), C.c_int()
+        assert crp.size == nt * n and cre.size == nt * n
+        rc = lib().croma_hip_execute_snapshot(
+            C.c_void_p(self.h), nt, _p(ns), _p(no), _p(ns), _p(cad), 0, int(allow_interp), C.c_double(elapsed),
+            C.c_double(dt_snap), C.c_double(0.0), C.c_double(0.0), C.c_double(t_acc_direct),
+            *[_p(a) for a in bgc], _p(ric_zero), _p(m1z), _p(p1z), *[_p(a) for a in bgn],
+            _p(f64(rad_ic_row_next)), C.c_double(ric_m1n), C.c_double(ric_p1n), _p(f64(qpi)), _p(f64(qepri)),
+            _p(f64(fqe)), _p(np.ascontiguousarray(npmin, np.int32)), _p(f64(mass)),
+            _p(np.ascontiguousarray(dis, np.uint8)) if dis is not None else None, _p(crp), _p(cre),
+            C.byref(times), C.byref(bc), C.byref(mn), C.byref(mx))
+        _chk(rc, "execute_snapshot")
+        return times
+
+
+class Batch:
+    """Device-resident tracers (spectra, sources, backgrounds of nsnap snapshots)."""
+
+    def __init__(self, ctx: Context, ntracer, species, nsnap):
+        self.ctx, self.ntracer, self.species, self.nsnap = ctx, ntracer, species, nsnap
+        self.h = lib().croma_hip_batch_create(C.c_void_p(ctx.h), ntracer, species, nsnap)
+        if not self.h:
+            raise CromaHipError(-1, "batch create")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().croma_hip_batch_destroy(C.c_void_p(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, crp, cre, qpi=None, qepri=None, mass=None, dis=None):
+        _chk(lib().croma_hip_batch_set_state(C.c_void_p(self.h), _p(f64(crp)), _p(f64(cre)), _p(f64(qpi)),
+                                             _p(f64(qepri)), _p(f64(mass)),
+                                             _p(np.ascontiguousarray(dis, np.uint8)) if dis is not None else None),
+             "set_state")
+
+    def get_state(self):
+        n = self.ctx.n
+        crp = np.zeros(self.ntracer * n)
+        cre = np.zeros(self.ntracer * n)
+        _chk(lib().croma_hip_batch_get_state(C.c_void_p(self.h), _p(crp), _p(cre)), "get_state")
+        return crp, cre
+
+    def set_background(self, isnap, fields):
+        arrs = [f64(fields[k]) for k in range(8)]
+        arr = (_dp * 8)(*[_p(a) for a in arrs])
+        _chk(lib().croma_hip_batch_set_background(C.c_void_p(self.h), int(isnap), arr), "set_background")
+
+    def set_sources(self, isnap, qpi, qepri):
+        _chk(lib().croma_hip_batch_set_sources(C.c_void_p(self.h), int(isnap), _p(f64(qpi)), _p(f64(qepri))),
+             "set_sources")
+
+    def run(self, s0, s1, nsnap_total, z, dt_snap, elapsed=0.0, path=PATH_FUSED):
+        t = Times()
+        _chk(lib().croma_hip_batch_run(C.c_void_p(self.h), int(s0), int(s1), int(nsnap_total), _p(f64(z)),
+                                       _p(f64(dt_snap)), C.c_double(elapsed), int(path), C.byref(t)), "batch_run")
+        return t
+
+    def sync(self):
+        _chk(lib().croma_hip_batch_sync(C.c_void_p(self.h)), "batch_sync")
+
+
+def solve_cc_batch(a, b, c, source, state, dt):
+    nsys, nrow = a.shape
+    out = np.zeros_like(f64(state))
+    ms = C.c_double()
+    _chk(lib().croma_hip_solve_cc_batch(nsys, nrow, C.c_double(dt), _p(f64(a)), _p(f64(b)), _p(f64(c)),
+                                        _p(f64(source)), _p(f64(state)), _p(out), C.byref(ms)), "solve_cc_batch")
+    return out, ms.value

@@ -1,0 +1,178 @@
+/*
+ * croma_kernels.h -- thin C ABI between the C host runtime (the host directory) and the
+ * HIP kernels (csrc/croma_kernels.hip).  Plain structs of device pointers and
+ * scalars; no HIP or torch types.  Internal to libcroma_hip.so.
+ */
+#ifndef CROMA_KERNELS_H
+#define CROMA_KERNELS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CK_MAX_N 512
+#define CK_NBG 8 /* n_gas, kbt, b_field, divv, l_turb, dv, cs, beta_pl */
+
+enum { CK_BG_NGAS = 0, CK_BG_KBT, CK_BG_B, CK_BG_DIVV, CK_BG_LTURB, CK_BG_DV, CK_BG_CS, CK_BG_BETA };
+
+/* Physics parameters (reference key names, params.c:67-88). */
+typedef struct {
+    int dpp_model;       /* 1 asa, 2 ttd, 3 off, 4 direct */
+    int ttd_tacc_model;  /* 0 spectrum, 1 brunetti16 */
+    int reacc_window_mode;
+    int tracer_nsub_max;
+    int coeff_interp_min_steps;
+    int coeff_interp_max_segments;
+    double psi;
+    double mach_limit;
+    double eta_dpp_cap;
+    double t_acc_direct_gyr;
+    double t_off_gyr;
+    double t_acc_off_gyr;
+    double f_eddy;
+    double tracer_nsub_safety;
+    double epmax;
+} CkParams;
+
+/* Device-resident grid tables for one grid size n (both species). */
+typedef struct {
+    int n;
+    int j_pp;
+    const double *pp, *pdp, *pE, *p_p2, *p_expcut, *p_sig; /* p_sig: sigma_pp(j or j_pp) * sigmoid */
+    const double *ep, *edp, *eE, *e_p2, *e_expcut;
+    const double *p_ecr_w, *e_ecr_w; /* eta-cap energy weights (E-m) GeV dp on bins 2..n-3, else 0 */
+    double ppm1, ppp1, epm1, epp1;
+} CkGrid;
+
+/* One request (or the whole batch) in a launch.  Arrays are device pointers. */
+typedef struct {
+    double *crp;                  /* [ntr * n] cell-major, updated in place */
+    double *cre;                  /* [ntr * n] */
+    const double *qpi;            /* [ntr * n] or NULL (all zero) */
+    const double *qepri;          /* [ntr * n] or NULL */
+    const double *mass;           /* [ntr] or NULL */
+    const unsigned char *dis;     /* [ntr] disable_adiabatic or NULL */
+    const double *bgc[CK_NBG];    /* current snapshot backgrounds [ntr] */
+    const double *bgn[CK_NBG];    /* next snapshot backgrounds [ntr] */
+    const double *ric_c;          /* IC cooling row at z_curr [n] */
+    const double *ric_n;          /* IC cooling row at z_next [n] */
+    const int *nsub;              /* [ntr] */
+    const int *n_on;              /* [ntr] */
+    const double *fqe_t;          /* [n * n] secondary weights (transposed) or NULL */
+    int jp0;                      /* first contributing proton bin */
+    int ntr;
+    int allow_interp;
+    int protons;                  /* proton species active (else crp == 0 exactly) */
+    double dt_snap;
+    double elapsed;
+    double ricm1_c, ricp1_c, ricm1_n, ricp1_n;
+    CkParams prm;
+} CkSlot;
+
+typedef struct {
+    int slot;
+    int tracer;
+} CkTask;
+
+/* Runtime helpers (thin wrappers over the HIP runtime). */
+int ck_device_count(void);
+int ck_set_device(int dev);
+int ck_get_device(void);
+int ck_device_name(int dev, char *buf, int len);
+int ck_device_cu_count(int dev);
+int ck_mem_info(size_t *free_b, size_t *total_b);
+void *ck_malloc(size_t bytes);
+void *ck_host_alloc(size_t bytes);
+int ck_free(void *p);
+int ck_host_free(void *p);
+void *ck_stream_create(void);
+int ck_stream_destroy(void *s);
+int ck_stream_sync(void *s);
+void *ck_event_create(void);
+int ck_event_destroy(void *e);
+int ck_event_record(void *e, void *s);
+int ck_event_query(void *e);          /* 1 done, 0 pending, <0 error */
+int ck_event_sync(void *e);
+float ck_event_elapsed_ms(void *a, void *b);
+int ck_stream_wait_event(void *s, void *e);
+int ck_h2d(void *dst, const void *src, size_t bytes, void *stream);
+int ck_d2h(void *dst, const void *src, size_t bytes, void *stream);
+int ck_d2d(void *dst, const void *src, size_t bytes, void *stream);
+int ck_device_sync(void);
+const char *ck_last_error(void);
+
+/* Graph capture of a stream's work (one snapshot's launch sequence). */
+int ck_graph_begin(void *stream);
+void *ck_graph_end(void *stream);          /* returns executable graph or NULL */
+int ck_graph_launch(void *graph_exec, void *stream);
+int ck_graph_destroy(void *graph_exec);
+
+/* Fill kernel (used instead of memset so it is safe under graph capture). */
+int ck_fill_i32(int *dst, int value, size_t count, void *stream);
+int ck_fill_f64(double *dst, double value, size_t count, void *stream);
+
+/* Fused path: persistent cost-ordered lane-group kernel.  rows_per_lane is
+ * 4, 8 or 16; waves_per_block 1, 2 or 4.  protons selects the two-species
+ * variant; every slot of one launch must agree.  The counter must be zero on
+ * entry. */
+int ck_launch_fused(int n, int rows_per_lane, int waves_per_block, int nblocks, int protons,
+                    CkGrid grid, const CkSlot *slots_dev, const CkTask *tasks_dev, int ntasks,
+                    int *counter_dev, int *err_dev, void *stream);
+int ck_fused_supported(int n, int rows_per_lane);
+
+/* Plain path (one tracer per thread, unfused coefficient kernels, cell-major
+ * a,b,c arrays).  idx[count] selects tracers of the slot (a bucket). */
+typedef struct {
+    double *radp_p, *tl_p, *radp_e, *dpp_on_p, *dpp_on_e, *dpp_off_p, *dpp_off_e; /* [count*n] */
+    double *a, *b, *c;         /* [4][count*n]: p-on, e-on, p-off, e-off */
+    double *inje;              /* [count*n] */
+    double *cprime;            /* [count*n] */
+    double *bg;                /* [CK_NBG][count] interpolated backgrounds */
+    double *ric;               /* [n] interpolated IC row */
+    double *xp, *xe;           /* [count*n] gathered state */
+    double *qp, *qe;           /* [count*n] gathered sources */
+    double *mass;              /* [count] */
+    unsigned char *dis;        /* [count] */
+} CkPlainWs;
+
+int ck_plain_gather(int n, int count, const int *idx_dev, const CkSlot *slot_host, CkPlainWs ws,
+                    void *stream);
+int ck_plain_lerp(int n, int count, const int *idx_dev, const CkSlot *slot_host, double alpha,
+                  CkPlainWs ws, void *stream);
+int ck_plain_coeff(int n, int count, CkGrid grid, CkParams prm, double dt, double t_acc_direct,
+                   int protons, CkPlainWs ws, int *err_dev, void *stream);
+int ck_plain_secondary(int n, int count, const double *fqe_t, int jp0, CkPlainWs ws, void *stream);
+int ck_plain_thomas(int n, int count, int nstep, double dt, int windowed, int on_start, int on_end,
+                    int protons, CkPlainWs ws, int *err_dev, void *stream);
+int ck_plain_scatter(int n, int count, const int *idx_dev, const CkSlot *slot_host, CkPlainWs ws,
+                     void *stream);
+
+/* Batched tridiagonal Chang-Cooper solve (solve_cc_cuda_batch semantics):
+ * state_out = Thomas(a,b,c; dt*Q + state_in), cell-major, one system per thread. */
+int ck_solve_cc_batch(int nsys, int nrow, double dt, const double *a, const double *b,
+                      const double *c, const double *q, const double *state_in, double *state_out,
+                      double *cprime_ws, int *err_dev, void *stream);
+
+/* Emission contractions.  spectra [ntr x K] FP64 cell-major; kernel matrix
+ * K_T [K x nbins] FP32 (weights folded in); row_scale [ntr] (may be NULL);
+ * out [ntr x nbins] FP32.  mode 0: FP32 VALU reference kernel, 1: FP32
+ * inputs on MFMA (xf32 emulation via split-BF16 x3), 2: split-BF16 x3 MFMA. */
+int ck_emission_contract(int ntr, int K, int nbins, const double *spectra, const float *kmat,
+                         const double *row_scale, float *out, int mode, void *stream);
+/* FP64 reference contraction used for checks (out FP64). */
+int ck_emission_contract_f64(int ntr, int K, int nbins, const double *spectra,
+                             const double *kmat, const double *row_scale, double *out,
+                             void *stream);
+/* Synchrotron: per-tracer log-interpolated table contraction (VALU). */
+int ck_emission_synch(int ntr, int n, int nfreq, int nlogb, double logb_min, double inv_dlogb,
+                      const double *table, const double *cre, const double *dp,
+                      const double *b_field, double *eps, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,516 @@
+// croma_kernels.hip -- CDNA4 (gfx950) kernels for CROMA's per-tracer
+// Fokker-Planck update, plus the thin runtime wrappers the C host code uses.
+//
+// Fused path (k_fused): one lane group of G = n/R lanes owns one tracer for a
+// whole snapshot.  Each lane keeps R consecutive momentum rows of the
+// spectrum in registers.  Per coefficient segment the group rebuilds the
+// Chang-Cooper matrix in FP64 (losses, D_pp, face weights fused into the
+// assembly), factors it once (Thomas LU; the pivots come from a 2x2
+// projective scan across lanes followed by the exact in-lane recurrence),
+// and then runs every substep as two affine scans (forward elimination and
+// back substitution) whose coefficients are all non-negative.  Tracers are
+// pulled from a cost-ordered task list through one atomic counter.
+//
+// Plain path (k_plain_*): the reference GPU semantics -- one tracer per
+// thread, cell-major a/b/c arrays in HBM, unfused coefficient kernels,
+// per-bucket and per-segment launches (fp_cuda_solver.cu:104-141,
+// fp_cuda_coeff.cu:379-803) -- written fresh as the baseline.
+//
+// Physics follows the reference CPU path operation for operation:
+// FP_Coef.c:438-849, fp_shared_core.c:729-1081, Chang_Cooper.c:22-184,
+// fp_shared_core.c:226-257, 564-615, 1591-1672, tracer_fp_cpu.c:88-196,
+// tracer_fp_batch.c:651-827.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "croma_device.h"
+
+
+static thread_local char g_errbuf[256];
+
+static int hip_ok(hipError_t e, const char *what)
+{
+    if (e == hipSuccess) return 0;
+    snprintf(g_errbuf, sizeof(g_errbuf), "%s: %s", what, hipGetErrorString(e));
+    return -1;
+}
+
+extern "C" int ck_launch_fused_n64(int, int, int, int, CkGrid, const CkSlot *, const CkTask *, int, int *, int *, void *);
+extern "C" int ck_launch_fused_n128(int, int, int, int, CkGrid, const CkSlot *, const CkTask *, int, int *, int *, void *);
+extern "C" int ck_launch_fused_n256(int, int, int, int, CkGrid, const CkSlot *, const CkTask *, int, int *, int *, void *);
+extern "C" int ck_launch_fused_n512(int, int, int, int, CkGrid, const CkSlot *, const CkTask *, int, int *, int *, void *);
+
+extern "C" int ck_fused_supported(int n, int r)
+{
+    if (!(n == 64 || n == 128 || n == 256 || n == 512)) return 0;
+    if (!(r == 4 || r == 8 || r == 16)) return 0;
+    if (n / r > 64) return 0;
+    return 1;
+}
+
+extern "C" int ck_launch_fused(int n, int r, int wpb, int nblocks, int protons, CkGrid grid, const CkSlot *slots,
+                               const CkTask *tasks, int ntasks, int *counter, int *err, void *stream)
+{
+    int rc = -1;
+    if (!ck_fused_supported(n, r) || ntasks < 0 || nblocks <= 0) return -1;
+    if (!(wpb == 1 || wpb == 2 || wpb == 4)) return -1;
+    if (ntasks == 0) return 0;
+    switch (n) {
+    case 64: rc = ck_launch_fused_n64(r, wpb, nblocks, protons, grid, slots, tasks, ntasks, counter, err, stream); break;
+    case 128: rc = ck_launch_fused_n128(r, wpb, nblocks, protons, grid, slots, tasks, ntasks, counter, err, stream); break;
+    case 256: rc = ck_launch_fused_n256(r, wpb, nblocks, protons, grid, slots, tasks, ntasks, counter, err, stream); break;
+    case 512: rc = ck_launch_fused_n512(r, wpb, nblocks, protons, grid, slots, tasks, ntasks, counter, err, stream); break;
+    }
+    if (rc) hip_ok(hipGetLastError(), "k_fused launch");
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Plain path kernels: one tracer per thread, cell-major arrays in HBM.
+// ---------------------------------------------------------------------------
+__global__ void k_plain_gather(int n, int count, const int *__restrict__ idx, const double *__restrict__ crp,
+                               const double *__restrict__ cre, const double *__restrict__ qpi,
+                               const double *__restrict__ qepri, const double *__restrict__ mass,
+                               const unsigned char *__restrict__ dis, CkPlainWs ws)
+{
+    const size_t tot = (size_t)count * n;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
+        const int c = (int)(i / n), j = (int)(i % n);
+        const size_t src = (size_t)idx[c] * n + j;
+        ws.xp[i] = crp ? crp[src] : 0.0;
+        ws.xe[i] = cre[src];
+        ws.qp[i] = qpi ? qpi[src] : 0.0;
+        ws.qe[i] = qepri ? qepri[src] : 0.0;
+        if (j == 0) {
+            ws.mass[c] = mass ? mass[idx[c]] : 0.0;
+            ws.dis[c] = dis ? dis[idx[c]] : 0;
+        }
+    }
+}
+
+__global__ void k_plain_scatter(int n, int count, const int *__restrict__ idx, double *__restrict__ crp,
+                                double *__restrict__ cre, CkPlainWs ws)
+{
+    const size_t tot = (size_t)count * n;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
+        const int c = (int)(i / n), j = (int)(i % n);
+        const size_t dst = (size_t)idx[c] * n + j;
+        if (crp) crp[dst] = ws.xp[i];
+        cre[dst] = ws.xe[i];
+    }
+}
+
+struct BgPtrs {
+    const double *c[CK_NBG];
+    const double *x[CK_NBG];
+};
+
+__global__ void k_plain_lerp(int n, int count, const int *__restrict__ idx, BgPtrs bp, const double *ric_c,
+                             const double *ric_n, double alpha, int interp, CkPlainWs ws)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) {
+        const int t = idx[i];
+#pragma unroll
+        for (int f = 0; f < CK_NBG; ++f)
+            ws.bg[(size_t)f * count + i] = interp ? d_lerp(bp.c[f][t], bp.x[f][t], alpha) : bp.c[f][t];
+    }
+    if (i < n) ws.ric[i] = interp ? d_lerp(ric_c[i], ric_n[i], alpha) : ric_c[i];
+}
+
+// losses + D_pp (on / off) + Chang-Cooper assembly, one tracer per thread
+__global__ void k_plain_coeff(int n, int count, CkGrid g, CkParams prm, double dt, double t_acc_direct,
+                              int protons, CkPlainWs ws, int *err)
+{
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const double n_gas = ws.bg[0 * count + i], kbt = ws.bg[1 * count + i], bfield = ws.bg[2 * count + i];
+    const double divv = ws.dis[i] ? 0.0 : ws.bg[3 * count + i];
+    const double lturb = ws.bg[4 * count + i], dv = ws.bg[5 * count + i], cs = ws.bg[6 * count + i];
+    const double beta = ws.bg[7 * count + i];
+    const size_t off = (size_t)i * n;
+    const double b_micro = bfield * 1.0e+6;
+    const double b_ratio = b_micro / 3.2;
+    const double synch_coeff = 4.8e-4 * b_ratio * b_ratio * 9.11e-28 * 3.0e+10 * 3.1536e+16;
+    const double coul_coeff = 3.05e-29 * n_gas * K_C * K_GYR / (K_ME * K_GEV);
+    const double adia = 1.0 / 3.0 * divv;
+    const double tl_pref = 1.0 / (0.5 * K_C * n_gas) / K_GYR;
+    for (int j = 0; j < n; ++j) {
+        const double pe = g.ep[j], pe2 = g.e_p2[j];
+        const double beta2 = pe2 / (1.0 + pe2);
+        ws.radp_e[off + j] = synch_coeff * pe2 + ws.ric[j] + coul_coeff * (1.0 + log(pe / n_gas) / 74.8) / beta2 +
+                             adia * pe;
+        if (protons) {
+            ws.radp_p[off + j] = d_b_coulomb_p(n_gas, g.pp[j], kbt) + adia * g.pp[j];
+            ws.tl_p[off + j] = tl_pref / g.p_sig[j];
+        }
+    }
+    double inv_on = 0.0;
+    const int mode_on = d_dpp_model(prm, prm.dpp_model, lturb, dv, cs, beta, t_acc_direct, &inv_on);
+    double scale = 1.0;
+    if (prm.dpp_model != 3 && prm.eta_dpp_cap > 0.0) {
+        double ecr = 0.0;
+        if (protons)
+            for (int j = 2; j < n - 2; ++j) {
+                const double w = g.p_ecr_w[j], v = ws.xp[off + j];
+                if (w > 0.0 && isfinite(v)) ecr += w * v * g.pdp[j];
+            }
+        for (int j = 2; j < n - 2; ++j) {
+            const double w = g.e_ecr_w[j], v = ws.xe[off + j];
+            if (w > 0.0 && isfinite(v)) ecr += w * v * g.edp[j];
+        }
+        scale = d_eta_scale(prm, prm.dpp_model, lturb, dv, cs, beta, ws.mass[i], t_acc_direct, ecr);
+    }
+    for (int j = 0; j < n; ++j) {
+        const double off_e = g.e_p2[j] * (1.0 / (4.0 * 1.0e+5));
+        const double on_e = mode_on ? g.e_p2[j] * inv_on * g.e_expcut[j] : off_e;
+        ws.dpp_off_e[off + j] = off_e;
+        ws.dpp_on_e[off + j] = on_e * scale;
+        if (protons) {
+            const double off_p = g.p_p2[j] * (1.0 / (4.0 * 1.0e+5));
+            const double on_p = mode_on ? g.p_p2[j] * inv_on * exp(-g.pE[j] / prm.epmax) * g.p_expcut[j] : off_p;
+            ws.dpp_off_p[off + j] = off_p;
+            ws.dpp_on_p[off + j] = on_p * scale;
+        }
+    }
+    // assembly: sets 0 p-on, 1 e-on, 2 p-off, 3 e-off
+    const size_t plane = (size_t)count * n;
+    for (int set = 0; set < 4; ++set) {
+        const int sp = set & 1; // 0 proton, 1 electron
+        if (sp == 0 && !protons) continue;
+        const double *xg = sp ? g.ep : g.pp;
+        const double *dxg = sp ? g.edp : g.pdp;
+        const double *radp = sp ? ws.radp_e + off : ws.radp_p + off;
+        const double *D = (set < 2) ? (sp ? ws.dpp_on_e + off : ws.dpp_on_p + off)
+                                    : (sp ? ws.dpp_off_e + off : ws.dpp_off_p + off);
+        double *a = ws.a + set * plane + off, *b = ws.b + set * plane + off, *c = ws.c + set * plane + off;
+        double Wp_prev = 0.0, Wm_prev = 0.0, w_prev = 0.0;
+        for (int m = 0; m < n; ++m) {
+            const double Bm = radp[m] - 2.0 / xg[m] * D[m];
+            double Bl = 0.0, Br = 0.0, xl = 0.0, xr = 0.0, Cl = 0.0, Cr = 0.0;
+            if (m > 0) { Bl = radp[m - 1] - 2.0 / xg[m - 1] * D[m - 1]; Cl = D[m - 1]; xl = xg[m - 1]; }
+            if (m < n - 1) { Br = radp[m + 1] - 2.0 / xg[m + 1] * D[m + 1]; Cr = D[m + 1]; xr = xg[m + 1]; }
+            double Wp = 0.0, Wm = 0.0, w = 0.0;
+            if (m < n - 1) d_face(Bm, Br, D[m], Cr, xg[m], xr, Wp, Wm, w);
+            const int kind = (m == 0) ? 1 : ((m == n - 1) ? 2 : 0);
+            const double dtT = sp ? dt / 1.0e+100 : dt / ws.tl_p[off + m];
+            double delta_l = 0.0;
+            if (kind == 2 && !(fabs(D[m]) > 1.0e-200) && fabs(Cl) > 1.0e-200) delta_l = d_face_delta(w_prev);
+            d_cc_row(kind, dt, xg[m], dxg[m], Bm, D[m], dtT, xl, Bl, Cl, Wp_prev, Wm_prev, delta_l, xr, Br, Cr, Wp,
+                     Wm, a[m], b[m], c[m]);
+            Wp_prev = Wp;
+            Wm_prev = Wm;
+            w_prev = w;
+        }
+    }
+    (void)err;
+}
+
+__global__ void k_plain_secondary(int n, int count, const double *__restrict__ fqe_t, int jp0, CkPlainWs ws)
+{
+    const size_t tot = (size_t)count * n;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
+        const int c = (int)(i / n), je = (int)(i % n);
+        double q = 0.0;
+        if (fqe_t)
+            for (int jp = jp0; jp < n; ++jp) q += ws.xp[(size_t)c * n + jp] * fqe_t[(size_t)jp * n + je];
+        ws.inje[i] = q * K_GYR * ws.bg[(size_t)c] + ws.qe[i];
+    }
+}
+
+// Thomas with the rhs update (solve_cc_batch_cpu_timed semantics) for one
+// system, state in place; cprime scratch in HBM.
+__device__ __forceinline__ int d_thomas_cc(int n, double dt, const double *a, const double *b, const double *c,
+                                           const double *q, double *x, double *cp)
+{
+    for (int j = 0; j < n; ++j) x[j] = dt * q[j] + x[j];
+    if (fabs(b[0]) <= 1.0e-300) return ERR_PIVOT;
+    cp[0] = -c[0] / b[0];
+    x[0] /= b[0];
+    for (int j = 1; j < n; ++j) {
+        const double den = b[j] + a[j] * cp[j - 1];
+        if (fabs(den) <= 1.0e-300) return ERR_PIVOT;
+        cp[j] = (j == n - 1) ? 0.0 : -c[j] / den;
+        x[j] = (x[j] + a[j] * x[j - 1]) / den;
+    }
+    for (int j = n - 2; j >= 0; --j) x[j] -= cp[j] * x[j + 1];
+    return 0;
+}
+
+__global__ void k_plain_thomas(int n, int count, int nstep, double dt, int windowed, int on_start, int on_end,
+                               int protons, CkPlainWs ws, int *err)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const size_t off = (size_t)i * n, plane = (size_t)count * n;
+    int bad = 0;
+    for (int s = 0; s < nstep && !bad; ++s) {
+        const int on = !windowed || (s >= on_start && s < on_end);
+        const size_t ps = (on ? 0 : 2) * plane, es = (on ? 1 : 3) * plane;
+        if (protons)
+            bad |= d_thomas_cc(n, dt, ws.a + ps + off, ws.b + ps + off, ws.c + ps + off, ws.qp + off, ws.xp + off,
+                               ws.cprime + off);
+        bad |= d_thomas_cc(n, dt, ws.a + es + off, ws.b + es + off, ws.c + es + off, ws.inje + off, ws.xe + off,
+                           ws.cprime + off);
+    }
+    if (bad) atomicOr(err, bad);
+}
+
+__global__ void k_solve_cc_batch(int nsys, int nrow, double dt, const double *a, const double *b, const double *c,
+                                 const double *q, const double *xin, double *xout, double *cp, int *err)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nsys) return;
+    const size_t off = (size_t)i * nrow;
+    for (int j = 0; j < nrow; ++j) xout[off + j] = xin[off + j];
+    const int bad = d_thomas_cc(nrow, dt, a + off, b + off, c + off, q + off, xout + off, cp + off);
+    if (bad) atomicOr(err, bad);
+}
+
+static int grid1d(size_t work, int block)
+{
+    size_t nb = (work + block - 1) / block;
+    if (nb > 65535u * 32u) nb = 65535u * 32u;
+    if (nb < 1) nb = 1;
+    return (int)nb;
+}
+
+extern "C" int ck_plain_gather(int n, int count, const int *idx, const CkSlot *S, CkPlainWs ws, void *stream)
+{
+    hipLaunchKernelGGL(k_plain_gather, dim3(grid1d((size_t)count * n, 256)), dim3(256), 0, (hipStream_t)stream, n,
+                       count, idx, S->crp, S->cre, S->qpi, S->qepri, S->mass, S->dis, ws);
+    return hip_ok(hipGetLastError(), "k_plain_gather");
+}
+
+extern "C" int ck_plain_scatter(int n, int count, const int *idx, const CkSlot *S, CkPlainWs ws, void *stream)
+{
+    hipLaunchKernelGGL(k_plain_scatter, dim3(grid1d((size_t)count * n, 256)), dim3(256), 0, (hipStream_t)stream,
+                       n, count, idx, S->crp, S->cre, ws);
+    return hip_ok(hipGetLastError(), "k_plain_scatter");
+}
+
+extern "C" int ck_plain_lerp(int n, int count, const int *idx, const CkSlot *S, double alpha, CkPlainWs ws,
+                             void *stream)
+{
+    BgPtrs bp;
+    for (int f = 0; f < CK_NBG; ++f) {
+        bp.c[f] = S->bgc[f];
+        bp.x[f] = S->bgn[f];
+    }
+    const int work = count > n ? count : n;
+    hipLaunchKernelGGL(k_plain_lerp, dim3(grid1d((size_t)work, 128)), dim3(128), 0, (hipStream_t)stream, n, count,
+                       idx, bp, S->ric_c, S->ric_n, alpha, alpha > 0.0 ? 1 : 0, ws);
+    return hip_ok(hipGetLastError(), "k_plain_lerp");
+}
+
+extern "C" int ck_plain_coeff(int n, int count, CkGrid grid, CkParams prm, double dt, double t_acc_direct,
+                              int protons, CkPlainWs ws, int *err, void *stream)
+{
+    hipLaunchKernelGGL(k_plain_coeff, dim3(grid1d((size_t)count, 128)), dim3(128), 0, (hipStream_t)stream, n,
+                       count, grid, prm, dt, t_acc_direct, protons, ws, err);
+    return hip_ok(hipGetLastError(), "k_plain_coeff");
+}
+
+extern "C" int ck_plain_secondary(int n, int count, const double *fqe_t, int jp0, CkPlainWs ws, void *stream)
+{
+    hipLaunchKernelGGL(k_plain_secondary, dim3(grid1d((size_t)count * n, 256)), dim3(256), 0,
+                       (hipStream_t)stream, n, count, fqe_t, jp0, ws);
+    return hip_ok(hipGetLastError(), "k_plain_secondary");
+}
+
+extern "C" int ck_plain_thomas(int n, int count, int nstep, double dt, int windowed, int on_start, int on_end,
+                               int protons, CkPlainWs ws, int *err, void *stream)
+{
+    hipLaunchKernelGGL(k_plain_thomas, dim3(grid1d((size_t)count, 32)), dim3(32), 0, (hipStream_t)stream, n, count,
+                       nstep, dt, windowed, on_start, on_end, protons, ws, err);
+    return hip_ok(hipGetLastError(), "k_plain_thomas");
+}
+
+extern "C" int ck_solve_cc_batch(int nsys, int nrow, double dt, const double *a, const double *b, const double *c,
+                                 const double *q, const double *xin, double *xout, double *cp, int *err,
+                                 void *stream)
+{
+    hipLaunchKernelGGL(k_solve_cc_batch, dim3(grid1d((size_t)nsys, 64)), dim3(64), 0, (hipStream_t)stream, nsys,
+                       nrow, dt, a, b, c, q, xin, xout, cp, err);
+    return hip_ok(hipGetLastError(), "k_solve_cc_batch");
+}
+
+// ---------------------------------------------------------------------------
+// Fill kernels
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_fill(T *dst, T v, size_t count)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = v;
+}
+
+extern "C" int ck_fill_i32(int *dst, int v, size_t count, void *stream)
+{
+    if (count == 0) return 0;
+    hipLaunchKernelGGL(k_fill<int>, dim3(grid1d(count, 256)), dim3(256), 0, (hipStream_t)stream, dst, v, count);
+    return hip_ok(hipGetLastError(), "k_fill_i32");
+}
+
+extern "C" int ck_fill_f64(double *dst, double v, size_t count, void *stream)
+{
+    if (count == 0) return 0;
+    hipLaunchKernelGGL(k_fill<double>, dim3(grid1d(count, 256)), dim3(256), 0, (hipStream_t)stream, dst, v, count);
+    return hip_ok(hipGetLastError(), "k_fill_f64");
+}
+
+// ---------------------------------------------------------------------------
+// Runtime wrappers
+// ---------------------------------------------------------------------------
+extern "C" const char *ck_last_error(void) { return g_errbuf; }
+
+extern "C" int ck_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+extern "C" int ck_set_device(int d) { return hip_ok(hipSetDevice(d), "hipSetDevice"); }
+
+extern "C" int ck_get_device(void)
+{
+    int d = -1;
+    if (hipGetDevice(&d) != hipSuccess) return -1;
+    return d;
+}
+
+extern "C" int ck_device_name(int dev, char *buf, int len)
+{
+    hipDeviceProp_t p;
+    if (hip_ok(hipGetDeviceProperties(&p, dev), "hipGetDeviceProperties")) return -1;
+    snprintf(buf, (size_t)len, "%s (%s)", p.name, p.gcnArchName);
+    return 0;
+}
+
+extern "C" int ck_device_cu_count(int dev)
+{
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, dev) != hipSuccess) return 0;
+    return p.multiProcessorCount;
+}
+
+extern "C" int ck_mem_info(size_t *f, size_t *t) { return hip_ok(hipMemGetInfo(f, t), "hipMemGetInfo"); }
+
+extern "C" void *ck_malloc(size_t bytes)
+{
+    void *p = nullptr;
+    if (bytes == 0) bytes = 8;
+    if (hip_ok(hipMalloc(&p, bytes), "hipMalloc")) return nullptr;
+    return p;
+}
+
+extern "C" void *ck_host_alloc(size_t bytes)
+{
+    void *p = nullptr;
+    if (bytes == 0) bytes = 8;
+    if (hip_ok(hipHostMalloc(&p, bytes, hipHostMallocDefault), "hipHostMalloc")) return nullptr;
+    return p;
+}
+
+extern "C" int ck_free(void *p) { return p ? hip_ok(hipFree(p), "hipFree") : 0; }
+extern "C" int ck_host_free(void *p) { return p ? hip_ok(hipHostFree(p), "hipHostFree") : 0; }
+
+extern "C" void *ck_stream_create(void)
+{
+    hipStream_t s = nullptr;
+    if (hip_ok(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate")) return nullptr;
+    return (void *)s;
+}
+
+extern "C" int ck_stream_destroy(void *s) { return s ? hip_ok(hipStreamDestroy((hipStream_t)s), "hipStreamDestroy") : 0; }
+extern "C" int ck_stream_sync(void *s) { return hip_ok(hipStreamSynchronize((hipStream_t)s), "hipStreamSynchronize"); }
+
+extern "C" void *ck_event_create(void)
+{
+    hipEvent_t e = nullptr;
+    if (hip_ok(hipEventCreate(&e), "hipEventCreate")) return nullptr;
+    return (void *)e;
+}
+
+extern "C" int ck_event_destroy(void *e) { return e ? hip_ok(hipEventDestroy((hipEvent_t)e), "hipEventDestroy") : 0; }
+extern "C" int ck_event_record(void *e, void *s)
+{
+    return hip_ok(hipEventRecord((hipEvent_t)e, (hipStream_t)s), "hipEventRecord");
+}
+
+extern "C" int ck_event_query(void *e)
+{
+    const hipError_t r = hipEventQuery((hipEvent_t)e);
+    if (r == hipSuccess) return 1;
+    if (r == hipErrorNotReady) return 0;
+    hip_ok(r, "hipEventQuery");
+    return -1;
+}
+
+extern "C" int ck_event_sync(void *e) { return hip_ok(hipEventSynchronize((hipEvent_t)e), "hipEventSynchronize"); }
+
+extern "C" float ck_event_elapsed_ms(void *a, void *b)
+{
+    float ms = -1.0f;
+    if (hipEventElapsedTime(&ms, (hipEvent_t)a, (hipEvent_t)b) != hipSuccess) return -1.0f;
+    return ms;
+}
+
+extern "C" int ck_stream_wait_event(void *s, void *e)
+{
+    return hip_ok(hipStreamWaitEvent((hipStream_t)s, (hipEvent_t)e, 0), "hipStreamWaitEvent");
+}
+
+extern "C" int ck_h2d(void *dst, const void *src, size_t bytes, void *s)
+{
+    if (bytes == 0) return 0;
+    return hip_ok(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)s), "hipMemcpyAsync H2D");
+}
+
+extern "C" int ck_d2h(void *dst, const void *src, size_t bytes, void *s)
+{
+    if (bytes == 0) return 0;
+    return hip_ok(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)s), "hipMemcpyAsync D2H");
+}
+
+extern "C" int ck_d2d(void *dst, const void *src, size_t bytes, void *s)
+{
+    if (bytes == 0) return 0;
+    return hip_ok(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)s), "hipMemcpyAsync D2D");
+}
+
+extern "C" int ck_device_sync(void) { return hip_ok(hipDeviceSynchronize(), "hipDeviceSynchronize"); }
+
+extern "C" int ck_graph_begin(void *s)
+{
+    return hip_ok(hipStreamBeginCapture((hipStream_t)s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+}
+
+extern "C" void *ck_graph_end(void *s)
+{
+    hipGraph_t gph = nullptr;
+    hipGraphExec_t ex = nullptr;
+    if (hip_ok(hipStreamEndCapture((hipStream_t)s, &gph), "hipStreamEndCapture")) return nullptr;
+    if (hip_ok(hipGraphInstantiate(&ex, gph, nullptr, nullptr, 0), "hipGraphInstantiate")) {
+        (void)hipGraphDestroy(gph);
+        return nullptr;
+    }
+    (void)hipGraphDestroy(gph);
+    return (void *)ex;
+}
+
+extern "C" int ck_graph_launch(void *ex, void *s)
+{
+    return hip_ok(hipGraphLaunch((hipGraphExec_t)ex, (hipStream_t)s), "hipGraphLaunch");
+}
+
+extern "C" int ck_graph_destroy(void *ex)
+{
+    return ex ? hip_ok(hipGraphExecDestroy((hipGraphExec_t)ex), "hipGraphExecDestroy") : 0;
+}

@@ -1,0 +1,1219 @@
+/*
+ * croma_host.c -- C host runtime of libcroma_hip.so: device binding,
+ * contexts (grid tables on the device), device-resident tracer batches, the
+ * per-snapshot scheduler (nsub -> cost-ordered task list -> persistent fused
+ * kernel, or the plain per-bucket path), and the snapshot-level drop-in for
+ * tracer_fp_batch_execute_parallel.
+ *
+ * Per snapshot the host work (IC cooling rows, nsub, task ordering) of
+ * snapshot s+1 is prepared while the GPU runs snapshot s.
+ */
+#include "croma_host.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+static __thread char g_err[512];
+
+void ch_set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+const char *croma_hip_last_error(void) { return g_err; }
+
+double ch_now_s(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1.0e-9 * (double)ts.tv_nsec;
+}
+
+static int dev_fail(const char *what)
+{
+    ch_set_error("%s: %s", what, ck_last_error());
+    return CROMA_HIP_EDEVICE;
+}
+
+int croma_hip_backend_parse(const char *spec)
+{
+    if (!spec) return 0;
+    return strcmp(spec, "hip") == 0 || strcmp(spec, "HIP") == 0 || strcmp(spec, "rocm") == 0;
+}
+
+/* ------------------------------------------------------------------ */
+int croma_hip_is_available(void) { return ck_device_count() > 0 ? 1 : 0; }
+
+int croma_hip_bind_local_rank(int local_rank, int *device_out)
+{
+    const int n = ck_device_count();
+    int dev;
+    if (n <= 0) {
+        ch_set_error("no HIP device visible");
+        return CROMA_HIP_EDEVICE;
+    }
+    if (local_rank < 0) local_rank = 0;
+    dev = local_rank % n;
+    if (ck_set_device(dev) != 0) return dev_fail("bind");
+    if (device_out) *device_out = dev;
+    return 0;
+}
+
+int croma_hip_get_binding_state(int *count, int *current)
+{
+    if (count) *count = ck_device_count();
+    if (current) *current = ck_get_device();
+    return 0;
+}
+
+int croma_hip_get_current_device_id(char *buf, int len)
+{
+    const int d = ck_get_device();
+    if (!buf || len <= 0) return CROMA_HIP_EINVAL;
+    if (d < 0) return dev_fail("device id");
+    return ck_device_name(d, buf, len) == 0 ? 0 : CROMA_HIP_EDEVICE;
+}
+
+void croma_hip_default_params(CromaHipParams *p)
+{
+    if (!p) return;
+    memset(p, 0, sizeof(*p));
+    p->dpp_model = 3;
+    p->ttd_tacc_model = 1;
+    p->reacc_window_mode = 0;
+    p->tracer_nsub_max = 0;
+    p->coeff_interp_min_steps = 4;
+    p->coeff_interp_max_segments = 16;
+    p->psi = 0.5;
+    p->mach_limit = 0.5;
+    p->eta_dpp_cap = 0.0;
+    p->t_acc_direct_gyr = 0.0;
+    p->t_off_gyr = 0.0;
+    p->t_acc_off_gyr = 3.0;
+    p->f_eddy = 2.0;
+    p->tracer_nsub_safety = 0.25;
+    p->epmax = 1.0e7;
+}
+
+void ch_params_to_ck(const CromaHipParams *in, CkParams *o)
+{
+    o->dpp_model = in->dpp_model;
+    o->ttd_tacc_model = in->ttd_tacc_model;
+    o->reacc_window_mode = in->reacc_window_mode;
+    o->tracer_nsub_max = in->tracer_nsub_max;
+    o->coeff_interp_min_steps = in->coeff_interp_min_steps;
+    o->coeff_interp_max_segments = in->coeff_interp_max_segments;
+    o->psi = in->psi;
+    o->mach_limit = in->mach_limit;
+    o->eta_dpp_cap = in->eta_dpp_cap;
+    o->t_acc_direct_gyr = in->t_acc_direct_gyr;
+    o->t_off_gyr = in->t_off_gyr;
+    o->t_acc_off_gyr = in->t_acc_off_gyr;
+    o->f_eddy = in->f_eddy;
+    o->tracer_nsub_safety = in->tracer_nsub_safety;
+    o->epmax = in->epmax;
+}
+
+static int valid_params(const CromaHipParams *p)
+{
+    if (!p) return 0;
+    if (p->dpp_model < 1 || p->dpp_model > 4) return 0;
+    if (!(p->tracer_nsub_safety > 0.0)) return 0;
+    return 1;
+}
+
+/* ------------------------------------------------------------------ */
+#define CH_IC_CACHE 8
+
+struct CromaHipContext {
+    int device;
+    int cu_count;
+    ChGrid g;
+    CromaHipParams uprm;
+    CkParams prm;
+    CkGrid dgrid;
+    double *d_grid_mem;
+    void *stream;
+    int *d_counter;
+    int *d_err;
+    CromaHipTile tile[2]; /* [protons] */
+    pthread_mutex_t ic_lock;
+    struct {
+        int valid;
+        double z, m1, p1;
+        double row[CH_NMAX];
+    } ic[CH_IC_CACHE];
+    int ic_next;
+    /* secondary weights, built on first use */
+    int fqe_ready;
+    int jp0;
+    double *h_fqe;
+    int *h_npmin;
+    double *d_fqe;
+    /* plain-path workspace */
+    CkPlainWs pws;
+    void *pws_mem;
+    int pws_cap;
+};
+
+static int ctx_upload_grid(CromaHipContext *c)
+{
+    const int n = c->g.n;
+    const size_t nb = (size_t)n * sizeof(double);
+    const double *src[13] = {c->g.pp, c->g.pdp, c->g.pE, c->g.p_p2, c->g.p_expcut, c->g.p_sig, c->g.ep,
+                             c->g.edp, c->g.eE, c->g.e_p2, c->g.e_expcut, c->g.p_ecr_w, c->g.e_ecr_w};
+    int k;
+    c->d_grid_mem = (double *)ck_malloc(13 * nb);
+    if (!c->d_grid_mem) return dev_fail("grid alloc");
+    for (k = 0; k < 13; k++)
+        if (ck_h2d(c->d_grid_mem + (size_t)k * n, src[k], nb, c->stream)) return dev_fail("grid upload");
+    c->dgrid.n = n;
+    c->dgrid.j_pp = c->g.j_pp;
+    c->dgrid.pp = c->d_grid_mem + 0 * (size_t)n;
+    c->dgrid.pdp = c->d_grid_mem + 1 * (size_t)n;
+    c->dgrid.pE = c->d_grid_mem + 2 * (size_t)n;
+    c->dgrid.p_p2 = c->d_grid_mem + 3 * (size_t)n;
+    c->dgrid.p_expcut = c->d_grid_mem + 4 * (size_t)n;
+    c->dgrid.p_sig = c->d_grid_mem + 5 * (size_t)n;
+    c->dgrid.ep = c->d_grid_mem + 6 * (size_t)n;
+    c->dgrid.edp = c->d_grid_mem + 7 * (size_t)n;
+    c->dgrid.eE = c->d_grid_mem + 8 * (size_t)n;
+    c->dgrid.e_p2 = c->d_grid_mem + 9 * (size_t)n;
+    c->dgrid.e_expcut = c->d_grid_mem + 10 * (size_t)n;
+    c->dgrid.p_ecr_w = c->d_grid_mem + 11 * (size_t)n;
+    c->dgrid.e_ecr_w = c->d_grid_mem + 12 * (size_t)n;
+    c->dgrid.ppm1 = c->g.ppm1;
+    c->dgrid.ppp1 = c->g.ppp1;
+    c->dgrid.epm1 = c->g.epm1;
+    c->dgrid.epp1 = c->g.epp1;
+    return ck_stream_sync(c->stream) ? dev_fail("grid sync") : 0;
+}
+
+static void default_tiles(CromaHipContext *c)
+{
+    const int n = c->g.n;
+    /* rows per lane: 8 keeps each tracer's spectrum and its factors in
+     * registers; for two species a smaller R halves the register set. */
+    c->tile[0].rows_per_lane = (n >= 128) ? 8 : 4;
+    c->tile[0].waves_per_block = 4;
+    c->tile[0].blocks_per_cu = 2;
+    c->tile[1].rows_per_lane = (n == 512) ? 8 : ((n >= 128) ? 8 : 4);
+    c->tile[1].waves_per_block = 4;
+    c->tile[1].blocks_per_cu = 1;
+}
+
+CromaHipContext *croma_hip_context_create(const CromaHipGridSpec *gs, const CromaHipParams *prm)
+{
+    CromaHipContext *c;
+    CromaHipParams def;
+    if (!gs || !(gs->n == 64 || gs->n == 128 || gs->n == 256 || gs->n == 512)) {
+        ch_set_error("grid size must be 64, 128, 256 or 512");
+        return 0;
+    }
+    if (!prm) {
+        croma_hip_default_params(&def);
+        prm = &def;
+    }
+    if (!valid_params(prm)) {
+        ch_set_error("invalid parameters");
+        return 0;
+    }
+    if (ck_device_count() <= 0) {
+        ch_set_error("no HIP device visible");
+        return 0;
+    }
+    c = (CromaHipContext *)calloc(1, sizeof(*c));
+    if (!c) return 0;
+    pthread_mutex_init(&c->ic_lock, 0);
+    c->device = ck_get_device();
+    if (c->device < 0) c->device = 0;
+    c->cu_count = ck_device_cu_count(c->device);
+    if (c->cu_count <= 0) c->cu_count = 256;
+    if (ch_grid_build(&c->g, gs->n, gs->pmin, gs->pmax, gs->pemin, gs->pemax) != 0) {
+        ch_set_error("grid build failed");
+        free(c);
+        return 0;
+    }
+    c->uprm = *prm;
+    ch_params_to_ck(prm, &c->prm);
+    c->stream = ck_stream_create();
+    c->d_counter = (int *)ck_malloc(sizeof(int) * 64);
+    c->d_err = (int *)ck_malloc(sizeof(int) * 64);
+    if (!c->stream || !c->d_counter || !c->d_err || ctx_upload_grid(c) != 0) {
+        croma_hip_context_destroy(c);
+        return 0;
+    }
+    if (ck_fill_i32(c->d_err, 0, 64, c->stream) || ck_stream_sync(c->stream)) {
+        dev_fail("context init");
+        croma_hip_context_destroy(c);
+        return 0;
+    }
+    default_tiles(c);
+    return c;
+}
+
+void croma_hip_context_destroy(CromaHipContext *c)
+{
+    if (!c) return;
+    if (c->stream) ck_stream_sync(c->stream);
+    ck_free(c->d_grid_mem);
+    ck_free(c->d_counter);
+    ck_free(c->d_err);
+    ck_free(c->d_fqe);
+    ck_free(c->pws_mem);
+    free(c->h_fqe);
+    free(c->h_npmin);
+    if (c->stream) ck_stream_destroy(c->stream);
+    pthread_mutex_destroy(&c->ic_lock);
+    free(c);
+}
+
+int croma_hip_context_set_params(CromaHipContext *c, const CromaHipParams *prm)
+{
+    if (!c || !valid_params(prm)) return CROMA_HIP_EINVAL;
+    c->uprm = *prm;
+    ch_params_to_ck(prm, &c->prm);
+    return 0;
+}
+
+int croma_hip_context_set_tile(CromaHipContext *c, const CromaHipTile *t)
+{
+    int k;
+    if (!c || !t) return CROMA_HIP_EINVAL;
+    if (!ck_fused_supported(c->g.n, t->rows_per_lane)) return CROMA_HIP_EUNSUPPORTED;
+    if (!(t->waves_per_block == 1 || t->waves_per_block == 2 || t->waves_per_block == 4)) return CROMA_HIP_EINVAL;
+    if (t->blocks_per_cu < 1 || t->blocks_per_cu > 64) return CROMA_HIP_EINVAL;
+    for (k = 0; k < 2; k++) c->tile[k] = *t;
+    return 0;
+}
+
+int croma_hip_context_get_tile(const CromaHipContext *c, int protons, CromaHipTile *t)
+{
+    if (!c || !t) return CROMA_HIP_EINVAL;
+    *t = c->tile[protons ? 1 : 0];
+    return 0;
+}
+
+int croma_hip_context_grid(const CromaHipContext *c, int species, double *p, double *dp, double *E, double *pm1,
+                           double *pp1)
+{
+    const size_t nb = c ? (size_t)c->g.n * sizeof(double) : 0;
+    if (!c) return CROMA_HIP_EINVAL;
+    if (species == 0) {
+        if (p) memcpy(p, c->g.pp, nb);
+        if (dp) memcpy(dp, c->g.pdp, nb);
+        if (E) memcpy(E, c->g.pE, nb);
+        if (pm1) *pm1 = c->g.ppm1;
+        if (pp1) *pp1 = c->g.ppp1;
+    } else {
+        if (p) memcpy(p, c->g.ep, nb);
+        if (dp) memcpy(dp, c->g.edp, nb);
+        if (E) memcpy(E, c->g.eE, nb);
+        if (pm1) *pm1 = c->g.epm1;
+        if (pp1) *pp1 = c->g.epp1;
+    }
+    return 0;
+}
+
+int croma_hip_ic_cooling_row(const CromaHipContext *cc, double z, double *row, double *m1, double *p1)
+{
+    CromaHipContext *c = (CromaHipContext *)cc;
+    int k, slot;
+    if (!c || !row) return CROMA_HIP_EINVAL;
+    pthread_mutex_lock(&c->ic_lock);
+    for (k = 0; k < CH_IC_CACHE; k++)
+        if (c->ic[k].valid && c->ic[k].z == z) {
+            memcpy(row, c->ic[k].row, (size_t)c->g.n * sizeof(double));
+            if (m1) *m1 = c->ic[k].m1;
+            if (p1) *p1 = c->ic[k].p1;
+            pthread_mutex_unlock(&c->ic_lock);
+            return 0;
+        }
+    pthread_mutex_unlock(&c->ic_lock);
+    {
+        double r[CH_NMAX], a, b;
+        ch_ic_cooling_row(&c->g, z, r, &a, &b);
+        pthread_mutex_lock(&c->ic_lock);
+        slot = c->ic_next;
+        c->ic_next = (c->ic_next + 1) % CH_IC_CACHE;
+        memcpy(c->ic[slot].row, r, (size_t)c->g.n * sizeof(double));
+        c->ic[slot].z = z;
+        c->ic[slot].m1 = a;
+        c->ic[slot].p1 = b;
+        c->ic[slot].valid = 1;
+        pthread_mutex_unlock(&c->ic_lock);
+        memcpy(row, r, (size_t)c->g.n * sizeof(double));
+        if (m1) *m1 = a;
+        if (p1) *p1 = b;
+    }
+    return 0;
+}
+
+int croma_hip_nsub_estimate(const CromaHipContext *c, int ntr, double dt_snap, const double *n_gas,
+                            const double *kbt, const double *b_field, const double *divv, double ric_lo,
+                            double ric_hi, const double *l_turb, const double *dv, int *nsub, int *n_on)
+{
+    if (!c || ntr < 0 || !n_gas || !kbt || !b_field || !divv || !l_turb || !dv || !nsub || !n_on)
+        return CROMA_HIP_EINVAL;
+    return ch_nsub_estimate(&c->g, &c->prm, ntr, dt_snap, ric_lo, ric_hi, n_gas, kbt, b_field, divv, l_turb, dv,
+                            nsub, n_on) == 0
+               ? 0
+               : CROMA_HIP_EINVAL;
+}
+
+static int ctx_fqe(CromaHipContext *c)
+{
+    const int n = c->g.n;
+    if (c->fqe_ready) return 0;
+    c->h_fqe = (double *)malloc((size_t)n * n * sizeof(double));
+    c->h_npmin = (int *)malloc((size_t)n * sizeof(int));
+    if (!c->h_fqe || !c->h_npmin) return CROMA_HIP_ENOMEM;
+    ch_secondary_weights(&c->g, c->h_fqe, c->h_npmin, &c->jp0);
+    c->d_fqe = (double *)ck_malloc((size_t)n * n * sizeof(double));
+    if (!c->d_fqe) return dev_fail("fqe alloc");
+    if (ck_h2d(c->d_fqe, c->h_fqe, (size_t)n * n * sizeof(double), c->stream) || ck_stream_sync(c->stream))
+        return dev_fail("fqe upload");
+    c->fqe_ready = 1;
+    return 0;
+}
+
+int croma_hip_secondary_weights(const CromaHipContext *cc, double *fqe_t, int *np_min_qe)
+{
+    CromaHipContext *c = (CromaHipContext *)cc;
+    int rc;
+    if (!c || !fqe_t || !np_min_qe) return CROMA_HIP_EINVAL;
+    rc = ctx_fqe(c);
+    if (rc) return rc;
+    memcpy(fqe_t, c->h_fqe, (size_t)c->g.n * c->g.n * sizeof(double));
+    memcpy(np_min_qe, c->h_npmin, (size_t)c->g.n * sizeof(int));
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* task ordering: counting sort of tracers by descending cost          */
+static void order_tasks(const CkParams *prm, int allow, int slot, int ntr, const int *nsub, int protons,
+                        CkTask *out, int *hist_buf, int hist_n)
+{
+    int i;
+    long pos = 0;
+    memset(hist_buf, 0, (size_t)hist_n * sizeof(int));
+    for (i = 0; i < ntr; i++) {
+        long cost = (long)nsub[i] * (protons ? 2 : 1) + 3L * ch_choose_segments(prm, nsub[i], allow);
+        if (cost >= hist_n) cost = hist_n - 1;
+        hist_buf[cost]++;
+    }
+    for (i = hist_n - 1; i >= 0; i--) {
+        const int cnt = hist_buf[i];
+        hist_buf[i] = (int)pos;
+        pos += cnt;
+    }
+    for (i = 0; i < ntr; i++) {
+        long cost = (long)nsub[i] * (protons ? 2 : 1) + 3L * ch_choose_segments(prm, nsub[i], allow);
+        if (cost >= hist_n) cost = hist_n - 1;
+        out[hist_buf[cost]].slot = slot;
+        out[hist_buf[cost]].tracer = i;
+        hist_buf[cost]++;
+    }
+}
+
+#define CH_HIST 65536
+
+/* ------------------------------------------------------------------ */
+struct CromaHipBatch {
+    CromaHipContext *ctx;
+    int ntr, species, nsnap_res, protons;
+    double *d_crp, *d_cre, *d_qpi, *d_qepri, *d_mass;
+    unsigned char *d_dis;
+    int has_qpi, has_qepri, has_mass, has_dis;
+    double *d_bg;  /* [nsnap_res][8][ntr] */
+    double *h_bg;  /* host copy for nsub */
+    double *d_src; /* optional [nsnap_res][2][ntr*n] per-snapshot sources */
+    unsigned char *src_set;
+    /* double-buffered per-snapshot scheduling state */
+    int *d_nsub[2], *d_non[2];
+    CkTask *d_tasks[2];
+    CkSlot *d_slot[2];
+    double *d_ric[2]; /* [2][n]: z_curr, z_next rows */
+    int *h_nsub[2], *h_non[2]; /* pinned */
+    CkTask *h_tasks[2];        /* pinned */
+    CkSlot *h_slot[2];         /* pinned */
+    double *h_ric[2];          /* pinned [2][n] */
+    int *hist;
+    void *ev_up[2];
+    void *ev_a, *ev_b;
+    /* plain path */
+    int *d_idx;
+    int *h_idx;
+};
+
+void croma_hip_batch_destroy(CromaHipBatch *b)
+{
+    int k;
+    if (!b) return;
+    if (b->ctx && b->ctx->stream) ck_stream_sync(b->ctx->stream);
+    ck_free(b->d_crp);
+    ck_free(b->d_cre);
+    ck_free(b->d_qpi);
+    ck_free(b->d_qepri);
+    ck_free(b->d_mass);
+    ck_free(b->d_dis);
+    ck_free(b->d_bg);
+    ck_free(b->d_src);
+    ck_free(b->d_idx);
+    free(b->h_bg);
+    free(b->src_set);
+    free(b->hist);
+    ck_host_free(b->h_idx);
+    for (k = 0; k < 2; k++) {
+        ck_free(b->d_nsub[k]);
+        ck_free(b->d_non[k]);
+        ck_free(b->d_tasks[k]);
+        ck_free(b->d_slot[k]);
+        ck_free(b->d_ric[k]);
+        ck_host_free(b->h_nsub[k]);
+        ck_host_free(b->h_non[k]);
+        ck_host_free(b->h_tasks[k]);
+        ck_host_free(b->h_slot[k]);
+        ck_host_free(b->h_ric[k]);
+        ck_event_destroy(b->ev_up[k]);
+    }
+    ck_event_destroy(b->ev_a);
+    ck_event_destroy(b->ev_b);
+    free(b);
+}
+
+CromaHipBatch *croma_hip_batch_create(CromaHipContext *c, int ntr, int species, int nsnap_res)
+{
+    CromaHipBatch *b;
+    const int n = c ? c->g.n : 0;
+    const size_t sn = (size_t)ntr * n * sizeof(double);
+    int k;
+    if (!c || ntr <= 0 || nsnap_res <= 0 ||
+        !(species == CROMA_HIP_SPECIES_ELECTRON || species == CROMA_HIP_SPECIES_ELECTRON_PROTON)) {
+        ch_set_error("invalid batch arguments");
+        return 0;
+    }
+    b = (CromaHipBatch *)calloc(1, sizeof(*b));
+    if (!b) return 0;
+    b->ctx = c;
+    b->ntr = ntr;
+    b->species = species;
+    b->protons = (species == CROMA_HIP_SPECIES_ELECTRON_PROTON);
+    b->nsnap_res = nsnap_res;
+    b->d_cre = (double *)ck_malloc(sn);
+    b->d_crp = b->protons ? (double *)ck_malloc(sn) : 0;
+    b->d_bg = (double *)ck_malloc((size_t)nsnap_res * CK_NBG * ntr * sizeof(double));
+    b->h_bg = (double *)malloc((size_t)nsnap_res * CK_NBG * ntr * sizeof(double));
+    b->hist = (int *)malloc(CH_HIST * sizeof(int));
+    b->src_set = (unsigned char *)calloc((size_t)nsnap_res, 1);
+    b->ev_a = ck_event_create();
+    b->ev_b = ck_event_create();
+    if (!b->d_cre || (b->protons && !b->d_crp) || !b->d_bg || !b->h_bg || !b->hist || !b->src_set || !b->ev_a ||
+        !b->ev_b)
+        goto fail;
+    for (k = 0; k < 2; k++) {
+        b->d_nsub[k] = (int *)ck_malloc((size_t)ntr * sizeof(int));
+        b->d_non[k] = (int *)ck_malloc((size_t)ntr * sizeof(int));
+        b->d_tasks[k] = (CkTask *)ck_malloc((size_t)ntr * sizeof(CkTask));
+        b->d_slot[k] = (CkSlot *)ck_malloc(sizeof(CkSlot));
+        b->d_ric[k] = (double *)ck_malloc(2 * (size_t)n * sizeof(double));
+        b->h_nsub[k] = (int *)ck_host_alloc((size_t)ntr * sizeof(int));
+        b->h_non[k] = (int *)ck_host_alloc((size_t)ntr * sizeof(int));
+        b->h_tasks[k] = (CkTask *)ck_host_alloc((size_t)ntr * sizeof(CkTask));
+        b->h_slot[k] = (CkSlot *)ck_host_alloc(sizeof(CkSlot));
+        b->h_ric[k] = (double *)ck_host_alloc(2 * (size_t)n * sizeof(double));
+        b->ev_up[k] = ck_event_create();
+        if (!b->d_nsub[k] || !b->d_non[k] || !b->d_tasks[k] || !b->d_slot[k] || !b->d_ric[k] || !b->h_nsub[k] ||
+            !b->h_non[k] || !b->h_tasks[k] || !b->h_slot[k] || !b->h_ric[k] || !b->ev_up[k])
+            goto fail;
+    }
+    if (b->protons && ctx_fqe(c) != 0) goto fail;
+    return b;
+fail:
+    if (!g_err[0]) ch_set_error("batch allocation failed: %s", ck_last_error());
+    croma_hip_batch_destroy(b);
+    return 0;
+}
+
+static int upload_opt(double **dst, const double *src, size_t bytes, void *stream)
+{
+    if (!src) return 0;
+    if (!*dst) *dst = (double *)ck_malloc(bytes);
+    if (!*dst) return -1;
+    return ck_h2d(*dst, src, bytes, stream);
+}
+
+int croma_hip_batch_set_state(CromaHipBatch *b, const double *crp, const double *cre, const double *qpi,
+                              const double *qepri, const double *mass, const unsigned char *dis)
+{
+    CromaHipContext *c;
+    size_t sn;
+    if (!b || !cre) return CROMA_HIP_EINVAL;
+    c = b->ctx;
+    sn = (size_t)b->ntr * c->g.n * sizeof(double);
+    if (!b->protons && (crp || qpi)) {
+        /* electron_only requires crp == 0 and qpi == 0 exactly */
+        size_t i, m = (size_t)b->ntr * c->g.n;
+        for (i = 0; i < m; i++)
+            if ((crp && crp[i] != 0.0) || (qpi && qpi[i] != 0.0)) {
+                ch_set_error("electron_only batch given non-zero proton state or source");
+                return CROMA_HIP_EINVAL;
+            }
+    }
+    if (ck_h2d(b->d_cre, cre, sn, c->stream)) return dev_fail("state upload");
+    if (b->protons) {
+        if (crp) {
+            if (ck_h2d(b->d_crp, crp, sn, c->stream)) return dev_fail("state upload");
+        } else if (ck_fill_f64(b->d_crp, 0.0, (size_t)b->ntr * c->g.n, c->stream)) {
+            return dev_fail("state clear");
+        }
+        if (upload_opt(&b->d_qpi, qpi, sn, c->stream)) return dev_fail("qpi upload");
+        b->has_qpi = qpi != 0;
+    }
+    if (upload_opt(&b->d_qepri, qepri, sn, c->stream)) return dev_fail("qepri upload");
+    b->has_qepri = qepri != 0;
+    if (upload_opt(&b->d_mass, mass, (size_t)b->ntr * sizeof(double), c->stream)) return dev_fail("mass upload");
+    b->has_mass = mass != 0;
+    if (dis) {
+        if (!b->d_dis) b->d_dis = (unsigned char *)ck_malloc((size_t)b->ntr);
+        if (!b->d_dis || ck_h2d(b->d_dis, dis, (size_t)b->ntr, c->stream)) return dev_fail("dis upload");
+    }
+    b->has_dis = dis != 0;
+    return ck_stream_sync(c->stream) ? dev_fail("state sync") : 0;
+}
+
+int croma_hip_batch_get_state(CromaHipBatch *b, double *crp, double *cre)
+{
+    CromaHipContext *c;
+    size_t sn;
+    if (!b) return CROMA_HIP_EINVAL;
+    c = b->ctx;
+    sn = (size_t)b->ntr * c->g.n * sizeof(double);
+    if (cre && ck_d2h(cre, b->d_cre, sn, c->stream)) return dev_fail("state download");
+    if (crp) {
+        if (b->protons) {
+            if (ck_d2h(crp, b->d_crp, sn, c->stream)) return dev_fail("state download");
+        } else {
+            memset(crp, 0, sn);
+        }
+    }
+    return ck_stream_sync(c->stream) ? dev_fail("state sync") : 0;
+}
+
+int croma_hip_batch_set_background(CromaHipBatch *b, int isnap, const double *const fields[8])
+{
+    int f;
+    if (!b || !fields || isnap < 0 || isnap >= b->nsnap_res) return CROMA_HIP_EINVAL;
+    for (f = 0; f < CK_NBG; f++) {
+        const size_t off = ((size_t)isnap * CK_NBG + f) * b->ntr;
+        if (!fields[f]) return CROMA_HIP_EINVAL;
+        memcpy(b->h_bg + off, fields[f], (size_t)b->ntr * sizeof(double));
+        if (ck_h2d(b->d_bg + off, b->h_bg + off, (size_t)b->ntr * sizeof(double), b->ctx->stream))
+            return dev_fail("background upload");
+    }
+    return ck_stream_sync(b->ctx->stream) ? dev_fail("background sync") : 0;
+}
+
+int croma_hip_batch_set_sources(CromaHipBatch *b, int isnap, const double *qpi, const double *qepri)
+{
+    const size_t m = b ? (size_t)b->ntr * b->ctx->g.n : 0;
+    if (!b || isnap < 0 || isnap >= b->nsnap_res) return CROMA_HIP_EINVAL;
+    if (!qpi && !qepri) {
+        b->src_set[isnap] = 0;
+        return 0;
+    }
+    if (!b->d_src) {
+        b->d_src = (double *)ck_malloc((size_t)b->nsnap_res * 2 * m * sizeof(double));
+        if (!b->d_src) return dev_fail("source alloc");
+    }
+    {
+        double *dp = b->d_src + (size_t)isnap * 2 * m;
+        if (qpi) {
+            if (!b->protons) {
+                size_t i;
+                for (i = 0; i < m; i++)
+                    if (qpi[i] != 0.0) {
+                        ch_set_error("electron_only batch given a proton source");
+                        return CROMA_HIP_EINVAL;
+                    }
+            }
+            if (ck_h2d(dp, qpi, m * sizeof(double), b->ctx->stream)) return dev_fail("source upload");
+        } else if (ck_fill_f64(dp, 0.0, m, b->ctx->stream)) {
+            return dev_fail("source clear");
+        }
+        if (qepri) {
+            if (ck_h2d(dp + m, qepri, m * sizeof(double), b->ctx->stream)) return dev_fail("source upload");
+        } else if (ck_fill_f64(dp + m, 0.0, m, b->ctx->stream)) {
+            return dev_fail("source clear");
+        }
+    }
+    b->src_set[isnap] = 1;
+    return ck_stream_sync(b->ctx->stream) ? dev_fail("source sync") : 0;
+}
+
+/* Host-side preparation of snapshot s into buffer set k. */
+typedef struct {
+    double ic_ms, nsub_ms, order_ms;
+    long long cell_steps, species_steps;
+} PrepStats;
+
+static int prepare_snapshot(CromaHipBatch *b, int k, int s, int nsnap_total, const double *z, const double *dt_snap,
+                            double elapsed, PrepStats *ps)
+{
+    CromaHipContext *c = b->ctx;
+    const int n = c->g.n, ntr = b->ntr;
+    const int allow = (s + 1 < nsnap_total) ? 1 : 0;
+    const int sn = (s + 1 < b->nsnap_res && allow) ? s + 1 : s;
+    const double *hb = b->h_bg + (size_t)s * CK_NBG * ntr;
+    CkSlot *S = b->h_slot[k];
+    double m1, p1, m1n, p1n, t0;
+    int f, i;
+    long long cs = 0;
+
+    if (allow && s + 1 >= b->nsnap_res) {
+        ch_set_error("snapshot %d needs background slot %d (resident %d)", s, s + 1, b->nsnap_res);
+        return CROMA_HIP_EINVAL;
+    }
+    t0 = ch_now_s();
+    croma_hip_ic_cooling_row(c, z[s], b->h_ric[k], &m1, &p1);
+    if (allow) croma_hip_ic_cooling_row(c, z[s + 1], b->h_ric[k] + n, &m1n, &p1n);
+    else {
+        memcpy(b->h_ric[k] + n, b->h_ric[k], (size_t)n * sizeof(double));
+        m1n = m1;
+        p1n = p1;
+    }
+    ps->ic_ms += 1e3 * (ch_now_s() - t0);
+    t0 = ch_now_s();
+    ch_nsub_estimate(&c->g, &c->prm, ntr, dt_snap[s], b->h_ric[k][0], b->h_ric[k][n - 1], hb + 0 * (size_t)ntr,
+                     hb + 1 * (size_t)ntr, hb + 2 * (size_t)ntr, hb + 3 * (size_t)ntr, hb + 4 * (size_t)ntr,
+                     hb + 5 * (size_t)ntr, b->h_nsub[k], b->h_non[k]);
+    ps->nsub_ms += 1e3 * (ch_now_s() - t0);
+    t0 = ch_now_s();
+    order_tasks(&c->prm, allow, 0, ntr, b->h_nsub[k], b->protons, b->h_tasks[k], b->hist, CH_HIST);
+    for (i = 0; i < ntr; i++) cs += b->h_nsub[k][i];
+    ps->cell_steps += cs;
+    ps->species_steps += cs * (b->protons ? 2 : 1);
+    ps->order_ms += 1e3 * (ch_now_s() - t0);
+
+    memset(S, 0, sizeof(*S));
+    S->crp = b->d_crp;
+    S->cre = b->d_cre;
+    if (b->d_src && b->src_set[s]) {
+        const size_t m = (size_t)ntr * n;
+        S->qpi = b->protons ? b->d_src + (size_t)s * 2 * m : 0;
+        S->qepri = b->d_src + (size_t)s * 2 * m + m;
+    } else {
+        S->qpi = (b->protons && b->has_qpi) ? b->d_qpi : 0;
+        S->qepri = b->has_qepri ? b->d_qepri : 0;
+    }
+    S->mass = b->has_mass ? b->d_mass : 0;
+    S->dis = b->has_dis ? b->d_dis : 0;
+    for (f = 0; f < CK_NBG; f++) {
+        S->bgc[f] = b->d_bg + ((size_t)s * CK_NBG + f) * ntr;
+        S->bgn[f] = b->d_bg + ((size_t)sn * CK_NBG + f) * ntr;
+    }
+    S->ric_c = b->d_ric[k];
+    S->ric_n = b->d_ric[k] + n;
+    S->nsub = b->d_nsub[k];
+    S->n_on = b->d_non[k];
+    S->fqe_t = b->protons ? c->d_fqe : 0;
+    S->jp0 = c->jp0;
+    S->ntr = ntr;
+    S->allow_interp = allow;
+    S->protons = b->protons;
+    S->dt_snap = dt_snap[s];
+    S->elapsed = elapsed;
+    S->ricm1_c = m1;
+    S->ricp1_c = p1;
+    S->ricm1_n = m1n;
+    S->ricp1_n = p1n;
+    S->prm = c->prm;
+    return 0;
+}
+
+static int upload_snapshot(CromaHipBatch *b, int k)
+{
+    CromaHipContext *c = b->ctx;
+    const int ntr = b->ntr, n = c->g.n;
+    if (ck_h2d(b->d_nsub[k], b->h_nsub[k], (size_t)ntr * sizeof(int), c->stream) ||
+        ck_h2d(b->d_non[k], b->h_non[k], (size_t)ntr * sizeof(int), c->stream) ||
+        ck_h2d(b->d_tasks[k], b->h_tasks[k], (size_t)ntr * sizeof(CkTask), c->stream) ||
+        ck_h2d(b->d_slot[k], b->h_slot[k], sizeof(CkSlot), c->stream) ||
+        ck_h2d(b->d_ric[k], b->h_ric[k], 2 * (size_t)n * sizeof(double), c->stream) ||
+        ck_event_record(b->ev_up[k], c->stream))
+        return dev_fail("snapshot upload");
+    return 0;
+}
+
+static int launch_fused(CromaHipBatch *b, int k)
+{
+    CromaHipContext *c = b->ctx;
+    const CromaHipTile *t = &c->tile[b->protons ? 1 : 0];
+    const int nblocks = c->cu_count * t->blocks_per_cu;
+    if (ck_fill_i32(c->d_counter, 0, 1, c->stream)) return dev_fail("counter reset");
+    if (ck_launch_fused(c->g.n, t->rows_per_lane, t->waves_per_block, nblocks, b->protons, c->dgrid, b->d_slot[k],
+                        b->d_tasks[k], b->ntr, c->d_counter, c->d_err, c->stream))
+        return dev_fail("fused launch");
+    return 0;
+}
+
+/* Plain path: reference GPU semantics (per bucket of equal (nsub, n_on),
+ * per coefficient segment: lerp, coefficient kernels, secondary, Thomas). */
+static int ensure_plain_ws(CromaHipContext *c, int count)
+{
+    const int n = c->g.n;
+    size_t per, tot;
+    char *m;
+    if (c->pws_cap >= count) return 0;
+    ck_stream_sync(c->stream);
+    ck_free(c->pws_mem);
+    c->pws_mem = 0;
+    c->pws_cap = 0;
+    per = (size_t)count * n * sizeof(double);
+    tot = per * (7 + 12 + 1 + 1 + 4) + (size_t)CK_NBG * count * sizeof(double) + (size_t)n * sizeof(double) +
+          (size_t)count * sizeof(double) + (size_t)count + 8192;
+    c->pws_mem = ck_malloc(tot);
+    if (!c->pws_mem) return dev_fail("plain workspace");
+    m = (char *)c->pws_mem;
+#define TAKE(field, bytes)                                                                                             \
+    do {                                                                                                               \
+        c->pws.field = (void *)m;                                                                                      \
+        m += ((bytes) + 255) & ~(size_t)255;                                                                           \
+    } while (0)
+    TAKE(radp_p, per);
+    TAKE(tl_p, per);
+    TAKE(radp_e, per);
+    TAKE(dpp_on_p, per);
+    TAKE(dpp_on_e, per);
+    TAKE(dpp_off_p, per);
+    TAKE(dpp_off_e, per);
+    TAKE(a, 4 * per);
+    TAKE(b, 4 * per);
+    TAKE(c, 4 * per);
+    TAKE(inje, per);
+    TAKE(cprime, per);
+    TAKE(xp, per);
+    TAKE(xe, per);
+    TAKE(qp, per);
+    TAKE(qe, per);
+    TAKE(bg, (size_t)CK_NBG * count * sizeof(double));
+    TAKE(ric, (size_t)n * sizeof(double));
+    TAKE(mass, (size_t)count * sizeof(double));
+    TAKE(dis, (size_t)count);
+#undef TAKE
+    c->pws_cap = count;
+    return 0;
+}
+
+/* LSD radix sort of (key, index) pairs on 64-bit keys, 16 bits per pass. */
+static int radix_sort_pairs(unsigned long long *key, int *idx, long n)
+{
+    unsigned long long *k2 = (unsigned long long *)malloc((size_t)n * sizeof(*k2));
+    int *i2 = (int *)malloc((size_t)n * sizeof(int));
+    long *cnt = (long *)malloc(65536 * sizeof(long));
+    int pass;
+    if (!k2 || !i2 || !cnt) {
+        free(k2);
+        free(i2);
+        free(cnt);
+        return -1;
+    }
+    for (pass = 0; pass < 4; pass++) {
+        const int sh = 16 * pass;
+        long i, pos = 0;
+        memset(cnt, 0, 65536 * sizeof(long));
+        for (i = 0; i < n; i++) cnt[(key[i] >> sh) & 0xffff]++;
+        for (i = 0; i < 65536; i++) {
+            const long c = cnt[i];
+            cnt[i] = pos;
+            pos += c;
+        }
+        for (i = 0; i < n; i++) {
+            const long d = cnt[(key[i] >> sh) & 0xffff]++;
+            k2[d] = key[i];
+            i2[d] = idx[i];
+        }
+        memcpy(key, k2, (size_t)n * sizeof(*key));
+        memcpy(idx, i2, (size_t)n * sizeof(int));
+    }
+    free(k2);
+    free(i2);
+    free(cnt);
+    return 0;
+}
+
+/* Plain path: the reference GPU semantics.  Tracers are bucketed by
+ * (nsub, n_on) (tracer_fp_bucket.c:208-262); every bucket runs its
+ * coefficient segments with separate launches: background lerp, the
+ * coefficient kernel, the secondary source and the thread-per-tracer Thomas
+ * substeps (fp_cuda_tracer_backend.cu:335-830). */
+static int run_plain_snapshot(CromaHipBatch *b, int k)
+{
+    CromaHipContext *c = b->ctx;
+    const CkSlot *S = b->h_slot[k];
+    const int n = c->g.n, ntr = b->ntr;
+    const int *nsub = b->h_nsub[k], *non = b->h_non[k];
+    unsigned long long *key;
+    long i0;
+    int maxb = 0;
+    if (!b->h_idx) {
+        b->h_idx = (int *)ck_host_alloc((size_t)ntr * sizeof(int));
+        b->d_idx = (int *)ck_malloc((size_t)ntr * sizeof(int));
+        if (!b->h_idx || !b->d_idx) return dev_fail("plain index");
+    }
+    /* the previous snapshot's kernels may still read d_idx / h_idx */
+    if (ck_stream_sync(c->stream)) return dev_fail("plain sync");
+    key = (unsigned long long *)malloc((size_t)ntr * sizeof(*key));
+    if (!key) return CROMA_HIP_ENOMEM;
+    for (i0 = 0; i0 < ntr; i0++) {
+        key[i0] = ((unsigned long long)(unsigned)nsub[i0] << 32) | (unsigned)non[i0];
+        b->h_idx[i0] = (int)i0;
+    }
+    if (radix_sort_pairs(key, b->h_idx, ntr)) {
+        free(key);
+        return CROMA_HIP_ENOMEM;
+    }
+    for (i0 = 0; i0 < ntr;) {
+        long i1 = i0 + 1;
+        while (i1 < ntr && key[i1] == key[i0]) i1++;
+        if (i1 - i0 > maxb) maxb = (int)(i1 - i0);
+        i0 = i1;
+    }
+    if (ensure_plain_ws(c, maxb) || ck_h2d(b->d_idx, b->h_idx, (size_t)ntr * sizeof(int), c->stream)) {
+        free(key);
+        return dev_fail("plain setup");
+    }
+    for (i0 = 0; i0 < ntr;) {
+        long i1 = i0 + 1;
+        int count, ns, on, nseg, iseg, on_start, on_end;
+        const int *didx;
+        while (i1 < ntr && key[i1] == key[i0]) i1++;
+        count = (int)(i1 - i0);
+        ns = (int)(key[i0] >> 32);
+        on = (int)(key[i0] & 0xffffffffu);
+        didx = b->d_idx + i0;
+        nseg = ch_choose_segments(&c->prm, ns, S->allow_interp);
+        on_start = (ns - on) / 2;
+        on_end = on_start + on;
+        if (ck_plain_gather(n, count, didx, S, c->pws, c->stream)) {
+            free(key);
+            return dev_fail("plain gather");
+        }
+        for (iseg = 0; iseg < nseg; iseg++) {
+            const int fb = (int)(((long long)iseg * ns) / nseg);
+            const int fe = (int)(((long long)(iseg + 1) * ns) / nseg);
+            const int steps = fe - fb;
+            const double alpha = (nseg > 1) ? ((double)iseg + 0.5) / (double)nseg : 0.0;
+            const double dt = S->dt_snap / (double)ns;
+            int sob = on_start > fb ? on_start : fb, soe = on_end < fe ? on_end : fe, windowed;
+            double tacc_d = c->prm.t_acc_direct_gyr;
+            if (steps <= 0) continue;
+            if (soe < sob) soe = sob;
+            windowed = (soe - sob) < steps;
+            {
+                const double tmid = S->elapsed + ((double)fb + 0.5 * (double)steps) * dt;
+                const CkParams *p = &c->prm;
+                const int direct_on = p->dpp_model == 4 && isfinite(p->t_acc_direct_gyr) &&
+                                      p->t_acc_direct_gyr > 0.0 && isfinite(p->t_off_gyr) && p->t_off_gyr > 0.0 &&
+                                      isfinite(p->t_acc_off_gyr) && p->t_acc_off_gyr > 0.0;
+                if (direct_on && tmid < p->t_off_gyr) tacc_d = p->t_acc_off_gyr;
+            }
+            if (ck_plain_lerp(n, count, didx, S, alpha, c->pws, c->stream) ||
+                ck_plain_coeff(n, count, c->dgrid, c->prm, dt, tacc_d, b->protons, c->pws, c->d_err, c->stream) ||
+                ck_plain_secondary(n, count, b->protons ? c->d_fqe : 0, c->jp0, c->pws, c->stream) ||
+                ck_plain_thomas(n, count, steps, dt, windowed, sob - fb, sob - fb + (soe - sob), b->protons, c->pws,
+                                c->d_err, c->stream)) {
+                free(key);
+                return dev_fail("plain segment");
+            }
+        }
+        if (ck_plain_scatter(n, count, didx, S, c->pws, c->stream)) {
+            free(key);
+            return dev_fail("plain scatter");
+        }
+        i0 = i1;
+    }
+    free(key);
+    return 0;
+}
+
+int croma_hip_batch_sync(CromaHipBatch *b)
+{
+    int err = 0;
+    if (!b) return CROMA_HIP_EINVAL;
+    if (ck_d2h(&err, b->ctx->d_err, sizeof(int), b->ctx->stream) || ck_stream_sync(b->ctx->stream))
+        return dev_fail("batch sync");
+    if (err) {
+        ck_fill_i32(b->ctx->d_err, 0, 1, b->ctx->stream);
+        ck_stream_sync(b->ctx->stream);
+        if (err & 1) {
+            ch_set_error("Thomas pivot below 1e-300");
+            return CROMA_HIP_EPIVOT;
+        }
+        ch_set_error("non-finite spectrum");
+        return CROMA_HIP_ENONFINITE;
+    }
+    return 0;
+}
+
+int croma_hip_batch_run(CromaHipBatch *b, int s0, int s1, int nsnap_total, const double *z, const double *dt_snap,
+                        double elapsed, int path, CromaHipTimes *times)
+{
+    CromaHipContext *c;
+    PrepStats ps;
+    double t_start, el = elapsed;
+    int s, rc = 0, k = 0;
+    float dev_ms = 0.0f;
+    if (!b || !z || !dt_snap || s0 < 0 || s1 < s0 || s1 > b->nsnap_res || nsnap_total < s1 ||
+        !(path == CROMA_HIP_PATH_FUSED || path == CROMA_HIP_PATH_PLAIN))
+        return CROMA_HIP_EINVAL;
+    if (s1 == s0) return 0;
+    c = b->ctx;
+    memset(&ps, 0, sizeof(ps));
+    t_start = ch_now_s();
+    if (ck_event_record(b->ev_a, c->stream)) return dev_fail("event");
+    rc = prepare_snapshot(b, 0, s0, nsnap_total, z, dt_snap, el, &ps);
+    if (rc) return rc;
+    for (s = s0; s < s1; s++) {
+        k = (s - s0) & 1;
+        if ((rc = upload_snapshot(b, k)) != 0) return rc;
+        if (path == CROMA_HIP_PATH_FUSED) rc = launch_fused(b, k);
+        else rc = run_plain_snapshot(b, k);
+        if (rc) return rc;
+        el += dt_snap[s];
+        if (s + 1 < s1) {
+            /* host buffers of set k^1 were consumed by the upload two snapshots ago */
+            if (ck_event_sync(b->ev_up[k ^ 1])) return dev_fail("upload event");
+            rc = prepare_snapshot(b, k ^ 1, s + 1, nsnap_total, z, dt_snap, el, &ps);
+            if (rc) return rc;
+        }
+    }
+    if (ck_event_record(b->ev_b, c->stream)) return dev_fail("event");
+    rc = croma_hip_batch_sync(b);
+    dev_ms = ck_event_elapsed_ms(b->ev_a, b->ev_b);
+    if (times) {
+        times->ic_rows_ms += ps.ic_ms;
+        times->nsub_estimate_ms += ps.nsub_ms;
+        times->bucket_build_ms += ps.order_ms;
+        times->solve_ms += dev_ms;
+        times->total_ms += 1e3 * (ch_now_s() - t_start);
+        times->cell_steps += ps.cell_steps;
+        times->species_steps += ps.species_steps;
+    }
+    return rc;
+}
+
+/* ------------------------------------------------------------------ */
+/* Snapshot-level drop-in for tracer_fp_batch_execute_parallel          */
+int croma_hip_execute_snapshot(
+    CromaHipContext *c, int ntr, const int *nsubsteps, const int *n_onsteps, const int *base_nsteps,
+    const int *fp_cadence_steps, int integration_mode, int allow, double elapsed, double dt_snap, double z_curr,
+    double z_next, double t_acc_direct, const double *n_gas, const double *kbt, const double *bf,
+    const double *divv, const double *lturb, const double *dv, const double *cs, const double *beta,
+    const double *rad_ic_zero, const double *rad_ic_m1_zero, const double *rad_ic_p1_zero, const double *n_gas_n,
+    const double *kbt_n, const double *bf_n, const double *divv_n, const double *lturb_n, const double *dv_n,
+    const double *cs_n, const double *beta_n, const double *row_next, double m1n, double p1n, const double *qpi,
+    const double *qepri, const double *fqe_flat, const int *np_min_qe, const double *mass,
+    const unsigned char *dis, double *crp, double *cre, CromaHipTimes *times, int *bucket_calls_out,
+    int *min_bucket_out, int *max_bucket_out)
+{
+    CromaHipBatch *b = 0;
+    const double *cur[8] = {n_gas, kbt, bf, divv, lturb, dv, cs, beta};
+    const double *nxt[8] = {n_gas_n, kbt_n, bf_n, divv_n, lturb_n, dv_n, cs_n, beta_n};
+    int n, i, j, rc = 0, protons = 0, minb = 0, maxb = 0;
+    double t0 = ch_now_s();
+    size_t m;
+    CkSlot *S;
+    void *d_ws = 0;
+    int *d_nsub = 0, *d_non = 0;
+    CkTask *d_tasks = 0;
+    CkSlot *d_slot = 0;
+    double *d_ric = 0, *d_fqe = 0;
+    CkParams prm;
+    (void)z_curr;
+    (void)z_next;
+    if (!c || ntr <= 0 || !nsubsteps || !n_onsteps || !base_nsteps || !fp_cadence_steps || !rad_ic_zero ||
+        !rad_ic_m1_zero || !rad_ic_p1_zero || !row_next || !qpi || !qepri || !fqe_flat || !np_min_qe || !mass ||
+        !crp || !cre)
+        return CROMA_HIP_EINVAL;
+    for (i = 0; i < 8; i++)
+        if (!cur[i] || !nxt[i]) return CROMA_HIP_EINVAL;
+    if (integration_mode != 0) {
+        ch_set_error("only the quantized integration mode exists");
+        return CROMA_HIP_EUNSUPPORTED;
+    }
+    n = c->g.n;
+    m = (size_t)ntr * n;
+    for (i = 0; i < ntr; i++) {
+        if (base_nsteps[i] != nsubsteps[i] || fp_cadence_steps[i] != 1) {
+            ch_set_error("base_nsteps must equal nsubsteps and cadence must be 1");
+            return CROMA_HIP_EUNSUPPORTED;
+        }
+        if (nsubsteps[i] <= 0) return CROMA_HIP_EINVAL;
+        if (rad_ic_m1_zero[i] != rad_ic_m1_zero[0] || rad_ic_p1_zero[i] != rad_ic_p1_zero[0]) {
+            ch_set_error("rad_ic edges must be the broadcast snapshot row");
+            return CROMA_HIP_EUNSUPPORTED;
+        }
+    }
+    for (j = 0; j < n; j++)
+        for (i = 1; i < ntr; i++)
+            if (rad_ic_zero[(size_t)j * ntr + i] != rad_ic_zero[(size_t)j * ntr]) {
+                ch_set_error("rad_ic_zero must be the broadcast snapshot row");
+                return CROMA_HIP_EUNSUPPORTED;
+            }
+    for (i = 0; i < (int)m && !protons; i++)
+        if (crp[i] != 0.0 || qpi[i] != 0.0) protons = 1;
+
+    b = croma_hip_batch_create(c, ntr, protons ? CROMA_HIP_SPECIES_ELECTRON_PROTON : CROMA_HIP_SPECIES_ELECTRON, 2);
+    if (!b) return CROMA_HIP_ENOMEM;
+    rc = croma_hip_batch_set_state(b, protons ? crp : 0, cre, protons ? qpi : 0, qepri, mass, dis);
+    if (!rc) rc = croma_hip_batch_set_background(b, 0, cur);
+    if (!rc) rc = croma_hip_batch_set_background(b, 1, nxt);
+    if (rc) goto done;
+
+    /* caller-provided schedule, IC rows and secondary weights */
+    d_ws = ck_malloc((size_t)ntr * (2 * sizeof(int) + sizeof(CkTask)) + sizeof(CkSlot) + 2 * (size_t)n * sizeof(double) +
+                     (size_t)n * n * sizeof(double) + 8192);
+    if (!d_ws) {
+        rc = dev_fail("execute workspace");
+        goto done;
+    }
+    {
+        char *p = (char *)d_ws;
+        d_nsub = (int *)p;
+        p += (((size_t)ntr * sizeof(int)) + 255) & ~(size_t)255;
+        d_non = (int *)p;
+        p += (((size_t)ntr * sizeof(int)) + 255) & ~(size_t)255;
+        d_tasks = (CkTask *)p;
+        p += (((size_t)ntr * sizeof(CkTask)) + 255) & ~(size_t)255;
+        d_slot = (CkSlot *)p;
+        p += 512;
+        d_ric = (double *)p;
+        p += ((2 * (size_t)n * sizeof(double)) + 255) & ~(size_t)255;
+        d_fqe = (double *)p;
+    }
+    {
+        double *ric2 = b->h_ric[0];
+        int jp0 = n;
+        for (j = 0; j < n; j++) ric2[j] = rad_ic_zero[(size_t)j * ntr];
+        memcpy(ric2 + n, row_next, (size_t)n * sizeof(double));
+        for (j = 0; j < n; j++) {
+            int v = np_min_qe[j] < 0 ? 0 : np_min_qe[j];
+            if (v < jp0) jp0 = v;
+        }
+        memcpy(b->h_nsub[0], nsubsteps, (size_t)ntr * sizeof(int));
+        memcpy(b->h_non[0], n_onsteps, (size_t)ntr * sizeof(int));
+        ch_params_to_ck(&c->uprm, &prm);
+        prm.t_acc_direct_gyr = t_acc_direct;
+        order_tasks(&prm, allow, 0, ntr, nsubsteps, protons, b->h_tasks[0], b->hist, CH_HIST);
+        S = b->h_slot[0];
+        memset(S, 0, sizeof(*S));
+        S->crp = b->d_crp;
+        S->cre = b->d_cre;
+        S->qpi = protons ? b->d_qpi : 0;
+        S->qepri = b->d_qepri;
+        S->mass = b->d_mass;
+        S->dis = b->has_dis ? b->d_dis : 0;
+        for (i = 0; i < CK_NBG; i++) {
+            S->bgc[i] = b->d_bg + (size_t)i * ntr;
+            S->bgn[i] = b->d_bg + ((size_t)CK_NBG + i) * ntr;
+        }
+        S->ric_c = d_ric;
+        S->ric_n = d_ric + n;
+        S->nsub = d_nsub;
+        S->n_on = d_non;
+        S->fqe_t = protons ? d_fqe : 0;
+        S->jp0 = jp0;
+        S->ntr = ntr;
+        S->allow_interp = allow;
+        S->protons = protons;
+        S->dt_snap = dt_snap;
+        S->elapsed = elapsed;
+        S->ricm1_c = rad_ic_m1_zero[0];
+        S->ricp1_c = rad_ic_p1_zero[0];
+        S->ricm1_n = m1n;
+        S->ricp1_n = p1n;
+        S->prm = prm;
+        if (ck_h2d(d_nsub, b->h_nsub[0], (size_t)ntr * sizeof(int), c->stream) ||
+            ck_h2d(d_non, b->h_non[0], (size_t)ntr * sizeof(int), c->stream) ||
+            ck_h2d(d_tasks, b->h_tasks[0], (size_t)ntr * sizeof(CkTask), c->stream) ||
+            ck_h2d(d_slot, S, sizeof(CkSlot), c->stream) ||
+            ck_h2d(d_ric, ric2, 2 * (size_t)n * sizeof(double), c->stream) ||
+            (protons && ck_h2d(d_fqe, fqe_flat, (size_t)n * n * sizeof(double), c->stream))) {
+            rc = dev_fail("execute upload");
+            goto done;
+        }
+        {
+            const CromaHipTile *t = &c->tile[protons ? 1 : 0];
+            if (ck_fill_i32(c->d_counter, 0, 1, c->stream) ||
+                ck_launch_fused(n, t->rows_per_lane, t->waves_per_block, c->cu_count * t->blocks_per_cu, protons,
+                                c->dgrid, d_slot, d_tasks, ntr, c->d_counter, c->d_err, c->stream)) {
+                rc = dev_fail("execute launch");
+                goto done;
+            }
+        }
+    }
+    rc = croma_hip_batch_sync(b);
+    if (!rc) rc = croma_hip_batch_get_state(b, protons ? crp : 0, cre);
+    if (!rc && !protons) memset(crp, 0, m * sizeof(double));
+    for (i = 0; i < ntr; i++) {
+        if (minb == 0 || nsubsteps[i] < minb) minb = nsubsteps[i];
+        if (nsubsteps[i] > maxb) maxb = nsubsteps[i];
+        if (times) times->cell_steps += nsubsteps[i];
+    }
+    if (bucket_calls_out) *bucket_calls_out = 1;
+    if (min_bucket_out) *min_bucket_out = ntr;
+    if (max_bucket_out) *max_bucket_out = ntr;
+    if (times) times->total_ms += 1e3 * (ch_now_s() - t0);
+done:
+    ck_stream_sync(c->stream);
+    ck_free(d_ws);
+    croma_hip_batch_destroy(b);
+    return rc;
+}
+
+/* ------------------------------------------------------------------ */
+int croma_hip_solve_cc_batch(int nsys, int nrow, double dt, const double *a, const double *bb, const double *cc,
+                             const double *src, const double *st_in, double *st_out, double *elapsed_ms)
+{
+    const size_t m = (size_t)nsys * nrow;
+    void *s = 0, *e0 = 0, *e1 = 0;
+    double *d = 0;
+    int *derr = 0, err = 0, rc = 0;
+    if (nsys <= 0 || nrow <= 0 || !a || !bb || !cc || !src || !st_in || !st_out) return CROMA_HIP_EINVAL;
+    s = ck_stream_create();
+    e0 = ck_event_create();
+    e1 = ck_event_create();
+    d = (double *)ck_malloc(m * 7 * sizeof(double));
+    derr = (int *)ck_malloc(sizeof(int));
+    if (!s || !e0 || !e1 || !d || !derr) {
+        rc = dev_fail("solve alloc");
+        goto out;
+    }
+    if (ck_fill_i32(derr, 0, 1, s) || ck_h2d(d, a, m * 8, s) || ck_h2d(d + m, bb, m * 8, s) ||
+        ck_h2d(d + 2 * m, cc, m * 8, s) || ck_h2d(d + 3 * m, src, m * 8, s) || ck_h2d(d + 4 * m, st_in, m * 8, s) ||
+        ck_event_record(e0, s) ||
+        ck_solve_cc_batch(nsys, nrow, dt, d, d + m, d + 2 * m, d + 3 * m, d + 4 * m, d + 5 * m, d + 6 * m, derr, s) ||
+        ck_event_record(e1, s) || ck_d2h(st_out, d + 5 * m, m * 8, s) || ck_d2h(&err, derr, sizeof(int), s) ||
+        ck_stream_sync(s)) {
+        rc = dev_fail("solve");
+        goto out;
+    }
+    if (elapsed_ms) *elapsed_ms = ck_event_elapsed_ms(e0, e1);
+    if (err) rc = CROMA_HIP_EPIVOT;
+out:
+    ck_free(d);
+    ck_free(derr);
+    ck_event_destroy(e0);
+    ck_event_destroy(e1);
+    ck_stream_destroy(s);
+    return rc;
+}

@@ -1,0 +1,50 @@
+/*
+ * croma_host.h -- internal declarations of the C host runtime of
+ * libcroma_hip.so (not installed; the public ABI is include/croma_hip.h).
+ */
+#ifndef CROMA_HOST_H
+#define CROMA_HOST_H
+
+#include <stddef.h>
+
+#include "../csrc/croma_kernels.h"
+#include "../../include/croma_hip.h"
+
+#define CH_NMAX 512
+
+/* Host copy of one grid and its per-grid caches. */
+typedef struct {
+    int n;
+    int j_pp;
+    double pmin, pmax, pemin, pemax;
+    double pp[CH_NMAX], pdp[CH_NMAX], pE[CH_NMAX], beta_p[CH_NMAX];
+    double p_p2[CH_NMAX], p_expcut[CH_NMAX], sigma_pp[CH_NMAX], sigmoid[CH_NMAX], p_sig[CH_NMAX];
+    double p_ecr_w[CH_NMAX];
+    double ep[CH_NMAX], edp[CH_NMAX], eE[CH_NMAX], e_p2[CH_NMAX], e_expcut[CH_NMAX], e_ecr_w[CH_NMAX];
+    double ppm1, ppp1, epm1, epp1;
+} ChGrid;
+
+/* croma_physics.c */
+int ch_host_threads(void);
+void ch_parallel_for(long n, long min_chunk, void (*fn)(void *, long, long), void *arg);
+double ch_pp_pion_cross_section(int kind, double p);
+int ch_grid_build(ChGrid *g, int n, double pmin, double pmax, double pemin, double pemax);
+double ch_b_coulomb_p(double nicm, double p, double kBT);
+int ch_nsub_estimate(const ChGrid *g, const CkParams *prm, long ntr, double dt_snap, double ric_lo,
+                     double ric_hi, const double *n_gas, const double *kbt, const double *b_field,
+                     const double *divv, const double *l_turb, const double *dv, int *nsub, int *n_on);
+int ch_choose_segments(const CkParams *prm, int nsub, int allow);
+int ch_ic_cooling_row(const ChGrid *g, double z, double *row, double *m1, double *p1);
+int ch_secondary_weights(const ChGrid *g, double *fqe_t, int *np_min_qe, int *jp0_out);
+double ch_gamma_kernel(double eg, double p);
+double ch_nu_kernel(double e, double p);
+double ch_ic_emissivity_kernel(double nu, double gamma, double z);
+/* kind 0 gamma (E in log10 GeV), 1 neutrino, 2 IC (nu in log10 Hz, at z) */
+int ch_emission_table(const ChGrid *g, int kind, int nbins, double lo, double hi, double z, double *out);
+
+/* croma_host.c */
+double ch_now_s(void);
+void ch_set_error(const char *fmt, ...);
+void ch_params_to_ck(const CromaHipParams *in, CkParams *out);
+
+#endif

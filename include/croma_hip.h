@@ -1,0 +1,277 @@
+/*
+ * croma_hip.h -- C ABI of libcroma_hip.so, the MI355X (gfx950) backend for
+ * CROMA's per-tracer Fokker-Planck update.
+ *
+ * Conventions (kept from the reference backend): every function returns 0 on
+ * success; caller-owned host arrays are read (or written) only during the
+ * call; handles are opaque.  Changed from the reference: errors are negative
+ * return codes (the library never calls exit()), physics parameters live in
+ * a per-handle struct instead of process globals, and the grid size n is a
+ * runtime value (64, 128, 256 or 512; both species use the same n as the
+ * reference's CRspectrum layout requires, params.h:52-61).
+ *
+ * Array layouts use the reference names: "cell-major" = [tracer][bin];
+ * "momentum-major" = [bin][tracer].
+ */
+#ifndef CROMA_HIP_H
+#define CROMA_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ---------------------------------------------------- */
+#define CROMA_HIP_OK 0
+#define CROMA_HIP_EINVAL (-1)      /* invalid argument (reference: -1)      */
+#define CROMA_HIP_EPIVOT (-2)      /* Thomas pivot |denom| <= 1e-300 (-2)   */
+#define CROMA_HIP_ENOMEM (-3)      /* host or device allocation failed      */
+#define CROMA_HIP_EDEVICE (-4)     /* HIP runtime / kernel launch failure   */
+#define CROMA_HIP_ENONFINITE (-5)  /* non-finite spectrum after a snapshot  */
+#define CROMA_HIP_EBUDGET (-6)     /* request exceeds the server HBM budget */
+#define CROMA_HIP_ECANCELED (-7)   /* request was cancelled                 */
+#define CROMA_HIP_EUNSUPPORTED (-8)/* configuration this build does not run */
+#define CROMA_HIP_EBUSY (-9)       /* result not ready yet                   */
+
+/* Backend selector value for the reference's `backend = auto|cpu|cuda`
+ * switch (tracer_fp_setup.c:401-416, include/tracer_fp_types.h:173-177). */
+#define CROMA_HIP_BACKEND_NAME "hip"
+int croma_hip_backend_parse(const char *spec); /* 1 if spec names this backend */
+
+/* ---- device binding (fp_cuda_backend.h:92-95) ------------------------ */
+/* Replaces cuda_pipeline_is_available. */
+int croma_hip_is_available(void);
+/* Replaces cuda_bind_local_rank: binds device local_rank % device_count. */
+int croma_hip_bind_local_rank(int local_rank, int *device_out);
+/* Replaces cuda_get_binding_state. */
+int croma_hip_get_binding_state(int *device_count_out, int *current_device_out);
+/* Replaces cuda_get_current_device_id. */
+int croma_hip_get_current_device_id(char *device_id_out, int device_id_out_len);
+const char *croma_hip_last_error(void);
+
+/* ---- configuration -------------------------------------------------- */
+typedef struct {
+    int n;          /* bins per species: 64, 128, 256 or 512 */
+    double pmin;    /* log10 proton p/(m_p c) bounds (params.c:51) */
+    double pmax;
+    double pemin;   /* log10 electron p/(m_e c) bounds (params.c:52) */
+    double pemax;
+} CromaHipGridSpec;
+
+/* Physics and scheduling parameters; names follow the reference keys
+ * (params.c:67-88).  dpp_model: 1 asa, 2 ttd, 3 off, 4 direct_tacc. */
+typedef struct {
+    int dpp_model;
+    int ttd_tacc_model;          /* 0 spectrum, 1 brunetti16 */
+    int reacc_window_mode;       /* 0 always on, 1 centred window */
+    int tracer_nsub_max;         /* 0 = no cap */
+    int coeff_interp_min_steps;  /* 4 */
+    int coeff_interp_max_segments; /* 16 */
+    double psi;                  /* 0.5 */
+    double mach_limit;           /* 0.5 */
+    double eta_dpp_cap;          /* <= 0 disables */
+    double t_acc_direct_gyr;
+    double t_off_gyr;
+    double t_acc_off_gyr;        /* 3.0 */
+    double f_eddy;               /* 2.0 */
+    double tracer_nsub_safety;   /* 0.25 */
+    double epmax;                /* 1e7 GeV */
+} CromaHipParams;
+
+void croma_hip_default_params(CromaHipParams *prm);
+
+/* Execution path. */
+#define CROMA_HIP_PATH_FUSED 0   /* persistent lane-group kernel (product) */
+#define CROMA_HIP_PATH_PLAIN 1   /* one tracer per thread, per-bucket launches */
+
+/* Tile shape of the fused kernel; 0 fields mean "autotuned / default". */
+typedef struct {
+    int rows_per_lane;    /* 4, 8 or 16: rows of the spectrum per lane */
+    int waves_per_block;  /* 1, 2 or 4 */
+    int blocks_per_cu;    /* persistent blocks per compute unit */
+} CromaHipTile;
+
+/* Stage timers, the subset of TracerFpGpuTimes (include/tracer_fp_types.h:10-57)
+ * this backend fills. */
+typedef struct {
+    double nsub_estimate_ms;
+    double bucket_build_ms;   /* task ordering / bucketing */
+    double ic_rows_ms;        /* IC cooling rows (host) */
+    double h2d_ms;
+    double solve_ms;          /* device time of the snapshot kernels */
+    double d2h_ms;
+    double total_ms;
+    long long cell_steps;     /* tracer-substeps: sum of nsub over tracers */
+    long long species_steps;  /* sum of nsub x active species */
+} CromaHipTimes;
+
+/* ---- context: device, grid tables, streams (cuda_pipeline_workspace_*,
+ * fp_cuda_backend.h:121-122) ----------------------------------------- */
+typedef struct CromaHipContext CromaHipContext;
+
+CromaHipContext *croma_hip_context_create(const CromaHipGridSpec *grid, const CromaHipParams *prm);
+void croma_hip_context_destroy(CromaHipContext *ctx);
+int croma_hip_context_set_params(CromaHipContext *ctx, const CromaHipParams *prm);
+int croma_hip_context_set_tile(CromaHipContext *ctx, const CromaHipTile *tile);
+int croma_hip_context_get_tile(const CromaHipContext *ctx, int protons, CromaHipTile *tile);
+/* Grid arrays (momentum_bin, FP_Coef.c:291-330); species 0 proton, 1 electron. */
+int croma_hip_context_grid(const CromaHipContext *ctx, int species, double *p, double *dp, double *E,
+                           double *pm1, double *pp1);
+/* Autotune the fused kernel's tile for (protons) on a synthetic probe of
+ * ntracer tracers; stores the best tile in the context and returns it. */
+int croma_hip_autotune(CromaHipContext *ctx, int protons, int ntracer, CromaHipTile *best);
+
+/* ---- snapshot-level drop-in ----------------------------------------- */
+/* Same parameters and meaning as tracer_fp_batch_execute_parallel
+ * (include/tracer_fp_batch.h:152-197, tracer_fp_batch.c:829-876), with the
+ * grids held by ctx.  rad_ic_zero is momentum-major [n * ntracer] and must be
+ * the broadcast row the reference builds (tracer_fp_synch.c:284-291).
+ * fqe_flat is the transposed secondary weight [n_p * n_e] with np_min_qe.
+ * base_nsteps / fp_cadence_steps / integration_mode are accepted for ABI
+ * compatibility; only the quantized mode (base == nsub, cadence 1) exists.
+ * Host arrays in, host arrays out; runs the fused path. */
+int croma_hip_execute_snapshot(
+    CromaHipContext *ctx, int ntracer, const int *nsubsteps, const int *n_onsteps,
+    const int *base_nsteps, const int *fp_cadence_steps, int integration_mode,
+    int allow_snapshot_interp, double elapsed_gyr, double dt_snap, double z_curr, double z_next,
+    double requested_t_acc_direct_gyr,
+    const double *n_gas_snap, const double *kbt_snap, const double *b_field_snap,
+    const double *divv_snap, const double *lturb_snap, const double *dv_snap,
+    const double *cs_snap, const double *beta_snap,
+    const double *rad_ic_zero, const double *rad_ic_m1_zero, const double *rad_ic_p1_zero,
+    const double *n_gas_next_snap, const double *kbt_next_snap, const double *b_field_next_snap,
+    const double *divv_next_snap, const double *lturb_next_snap, const double *dv_next_snap,
+    const double *cs_next_snap, const double *beta_next_snap,
+    const double *rad_ic_row_next, double rad_ic_m1_next_val, double rad_ic_p1_next_val,
+    const double *qpi_batch, const double *qepri_batch, const double *fqe_flat,
+    const int *np_min_qe, const double *tracer_mass, const unsigned char *disable_adiabatic,
+    double *crp_state, double *cre_state, CromaHipTimes *times,
+    int *bucket_calls_out, int *min_bucket_size_out, int *max_bucket_size_out);
+
+/* nsub estimate from the current snapshot (tracer_fp_prepare_nsub_loss_endpoints
+ * + tracer_fp_compute_nsubsteps_from_endpoints, tracer_fp_nsub.c:94-282);
+ * rad_ic_lo / rad_ic_hi are the IC cooling row at bins 0 and n-1. */
+int croma_hip_nsub_estimate(const CromaHipContext *ctx, int ntracer, double dt_snap,
+                            const double *n_gas, const double *kbt, const double *b_field,
+                            const double *divv, double rad_ic_lo, double rad_ic_hi,
+                            const double *l_turb, const double *dv, int *nsubsteps, int *n_onsteps);
+/* IC cooling row at redshift z (prepare_ic_cooling_row, fp_shared_core.c:844-864). */
+int croma_hip_ic_cooling_row(const CromaHipContext *ctx, double z, double *row, double *m1, double *p1);
+/* Secondary-electron weights, transposed layout (prepare_secondary_kernel_flat,
+ * fp_shared_core.c:1453-1506). */
+int croma_hip_secondary_weights(const CromaHipContext *ctx, double *fqe_t, int *np_min_qe);
+
+/* ---- device-resident batch (cuda_rank_state_upload/download +
+ * run_tracer_pipeline, fp_cuda_backend.h:124-150) --------------------- */
+typedef struct CromaHipBatch CromaHipBatch;
+
+#define CROMA_HIP_SPECIES_ELECTRON 1          /* electron_only: crp == 0 */
+#define CROMA_HIP_SPECIES_ELECTRON_PROTON 3   /* both species evolve */
+
+/* nsnap_resident: number of snapshot background sets kept in HBM. */
+CromaHipBatch *croma_hip_batch_create(CromaHipContext *ctx, int ntracer, int species, int nsnap_resident);
+void croma_hip_batch_destroy(CromaHipBatch *b);
+/* Upload state and per-tracer constants (cuda_rank_state_upload). crp may be
+ * NULL for electron_only; qpi/qepri may be NULL (zero). */
+int croma_hip_batch_set_state(CromaHipBatch *b, const double *crp, const double *cre, const double *qpi,
+                              const double *qepri, const double *tracer_mass,
+                              const unsigned char *disable_adiabatic);
+/* Download state (cuda_rank_state_download). */
+int croma_hip_batch_get_state(CromaHipBatch *b, double *crp, double *cre);
+/* Backgrounds of snapshot slot isnap (8 arrays [ntracer]: n_gas [cm^-3],
+ * kbt [GeV], b_field [G], divv [1/Gyr], l_turb [Mpc], dv [cm/s], cs [cm/s], beta_pl). */
+int croma_hip_batch_set_background(CromaHipBatch *b, int isnap, const double *const fields[8]);
+/* Per-snapshot sources (DSA source mode adds S/dt into qpi/qepri,
+ * tracer_fp_dsa.c:470-505); NULL clears the snapshot's sources. */
+int croma_hip_batch_set_sources(CromaHipBatch *b, int isnap, const double *qpi, const double *qepri);
+/* Run snapshots [isnap_begin, isnap_end): z[] and dt_snap[] indexed by
+ * snapshot (z needs isnap_end + 1 entries).  Snapshot s interpolates towards
+ * s+1 when s + 1 < nsnap_total (allow_snapshot_interp, solve.inc:10-12). */
+int croma_hip_batch_run(CromaHipBatch *b, int isnap_begin, int isnap_end, int nsnap_total, const double *z,
+                        const double *dt_snap, double elapsed_gyr, int path, CromaHipTimes *times);
+/* Wait for the batch's device work. */
+int croma_hip_batch_sync(CromaHipBatch *b);
+
+/* Batched Chang-Cooper solve (solve_cc_cuda_batch, fp_cuda_backend.h:112-116):
+ * state_out = Thomas(a, b, c; dt*source + state_init) per system, cell-major
+ * [nsys * nrow] host arrays. */
+int croma_hip_solve_cc_batch(int nsys, int nrow, double dt, const double *a, const double *b,
+                             const double *c, const double *source, const double *state_init,
+                             double *state_out, double *elapsed_ms);
+
+/* ---- emission (prepare_{synch,gamma,neutrino}_emission_cuda_batch,
+ * fp_cuda_backend.h:97-110, plus IC which the reference runs on the CPU) */
+#define CROMA_HIP_EMIS_FP32 0        /* FP32 VALU contraction */
+#define CROMA_HIP_EMIS_MFMA_XF32 1   /* split-BF16 x3 on MFMA, FP32 accumulate */
+#define CROMA_HIP_EMIS_MFMA_BF16 2   /* single BF16 pass on MFMA (fastest, ~1e-3) */
+#define CROMA_HIP_EMIS_FP64 3        /* FP64 reference on the GPU */
+
+/* IC: eps[i][k] = sum_j fic[k][j] cre[i][j] dp[j]  (fp_shared_core.c:2131-2157). */
+int croma_hip_emission_ic(const CromaHipContext *ctx, int ntracer, int nbins, const double *cre,
+                          const double *fic_flat, double *eps, int mode, double *elapsed_ms);
+/* gamma / neutrino: eps[i][k] = E_k * n_gas[i] * 0.5 * sum trapezoid(beta N f) dp
+ * (fp_shared_core.c:2166-2243, EMISSION.c:297-308).  emin/emax log10 GeV. */
+int croma_hip_emission_hadronic(const CromaHipContext *ctx, int ntracer, int nbins, double emin,
+                                double emax, const double *n_gas, const double *crp,
+                                const double *f_flat, double *eps, int mode, double *elapsed_ms);
+/* Synchrotron from a pitch-kernel table [nlogb][nfreq][n] with log
+ * interpolation in log B (fp_shared_core.c:2070-2129). */
+int croma_hip_emission_synch(const CromaHipContext *ctx, int ntracer, int nfreq, int nlogb,
+                             double logb_min, double inv_dlogb, const double *table, const double *cre,
+                             const double *b_field, double *eps, double *elapsed_ms);
+
+/* ---- serving: continuous batching of tracer chunks ------------------ */
+typedef struct CromaHipServer CromaHipServer;
+
+typedef struct {
+    int ntracer;
+    int n;                 /* grid size of this chunk */
+    int species;           /* CROMA_HIP_SPECIES_* */
+    int dpp_model;         /* overrides params.dpp_model for this chunk */
+    int nsnap;             /* snapshots to advance */
+    const double *z;       /* [nsnap + 1] */
+    const double *dt_snap; /* [nsnap] */
+    const double *bg;      /* [(nsnap + 1)][8][ntracer] backgrounds */
+    const double *crp;     /* [ntracer * n] or NULL */
+    const double *cre;     /* [ntracer * n] */
+    const double *qpi;     /* [ntracer * n] or NULL */
+    const double *qepri;   /* [ntracer * n] or NULL */
+    const double *mass;    /* [ntracer] or NULL */
+} CromaHipChunk;
+
+typedef struct {
+    double submit_s;       /* host submit time (monotonic clock) */
+    double admit_s;        /* admitted into HBM */
+    double done_s;         /* result visible on the host */
+    long long cell_steps;  /* tracer-substeps of the request */
+    int status;            /* 0 ok, else CROMA_HIP_E* */
+} CromaHipChunkStats;
+
+typedef struct {
+    long long submitted, completed, cancelled, failed;
+    long long cell_steps;
+    size_t budget_bytes, in_use_bytes, peak_bytes;
+    int active, queued;
+    long long iterations;  /* scheduler iterations (batched launches) */
+} CromaHipServeStats;
+
+CromaHipServer *croma_hip_serve_create(const CromaHipParams *prm, size_t hbm_budget_bytes,
+                                       double pmin, double pmax, double pemin, double pemax);
+void croma_hip_serve_destroy(CromaHipServer *s);
+/* Copies the chunk's inputs; returns the request id in *id_out. */
+int croma_hip_serve_submit(CromaHipServer *s, const CromaHipChunk *chunk, long long *id_out);
+/* 1 done, 0 pending, < 0 error code of the request. */
+int croma_hip_serve_poll(CromaHipServer *s, long long id);
+/* Blocks until done; copies the final spectra (crp may be NULL). */
+int croma_hip_serve_wait(CromaHipServer *s, long long id, double *crp, double *cre, CromaHipChunkStats *st);
+int croma_hip_serve_cancel(CromaHipServer *s, long long id);
+int croma_hip_serve_stats(CromaHipServer *s, CromaHipServeStats *out);
+/* Memory model used for admission (bytes of HBM a chunk holds). */
+size_t croma_hip_chunk_bytes(int ntracer, int n, int species, int nsnap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

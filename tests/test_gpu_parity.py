@@ -1,0 +1,57 @@
+"""HIP fused and plain paths vs the CPU oracle on the same seeded inputs.
+
+Tolerance (north star): spectra within 1e-10 relative above a 1e-30 floor,
+i.e. the reference comparator |hip - cpu| <= 1e-30 + 1e-10 |cpu|
+(analysis/compare_tracer_outputs.py:126-127, 356-366)."""
+import numpy as np
+import pytest
+
+import croma_hip as ch
+import run_oracle
+from workloads import hip_params, rel_err, run_hip, small_workload
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    # n, ntracer, nsnap, species, dpp, extra params, dsa
+    (128, 96, 3, 1, 3, {}, 0.0),
+    (64, 96, 3, 1, 1, {}, 0.0),
+    (256, 64, 2, 1, 2, {}, 0.0),
+    (512, 48, 2, 1, 2, {}, 0.0),
+    (256, 48, 2, 3, 1, {}, 0.3),
+    (128, 64, 2, 3, 3, {}, 0.0),
+    (128, 64, 3, 1, 1, {"reacc_window_mode": 1, "f_eddy": 0.02}, 0.0),
+    (128, 64, 2, 1, 1, {"eta_dpp_cap": 1e-6}, 0.0),
+]
+
+
+@pytest.fixture(scope="module")
+def ctxs():
+    return {}
+
+
+def _ctx(ctxs, n, prm):
+    if n not in ctxs:
+        ctxs[n] = ch.Context(n, prm)
+    ctxs[n].set_params(prm)
+    return ctxs[n]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[f"n{c[0]}-sp{c[3]}-dpp{c[4]}-{i}" for i, c in enumerate(CASES)])
+@pytest.mark.parametrize("path", [ch.PATH_FUSED, ch.PATH_PLAIN], ids=["fused", "plain"])
+def test_spectra_match_oracle(ctxs, case, path):
+    n, nt, ns, sp, dpp, extra, dsa = case
+    print(f"[parity] n={n} ntracer={nt} nsnap={ns} species={sp} dpp={dpp} extra={extra} path={path}", flush=True)
+    w = small_workload(n, nt, ns, sp, dpp, dsa_frac=dsa)
+    crp_ref, cre_ref = run_oracle.run_workload(w, run_oracle.oracle_params(w, **extra))
+    ctx = _ctx(ctxs, n, hip_params(w, **extra))
+    crp, cre, t = run_hip(w, ctx, path=path)
+    e_err, e_ok = rel_err(cre, cre_ref)
+    print(f"  electron max rel err {e_err:.3e}; substeps {t.cell_steps}", flush=True)
+    assert e_ok, f"electron spectra differ: max rel err {e_err:.3e}"
+    if sp == 3:
+        p_err, p_ok = rel_err(crp, crp_ref)
+        print(f"  proton max rel err {p_err:.3e}", flush=True)
+        assert p_ok, f"proton spectra differ: max rel err {p_err:.3e}"
+    else:
+        assert np.all(crp == 0.0)
