@@ -1,0 +1,211 @@
+"""Benchmark of the per-tracer Fokker-Planck update on one MI355X.
+
+A step is one snapshot update of every tracer of the workload (coefficient
+segments, Chang-Cooper assembly and all nsub implicit substeps of both active
+species).  The default workload is the batched uneven-work configuration:
+10,000 electron_proton tracers, 256 momentum bins, ASA D_pp, DSA-like sources
+on 5% of (tracer, snapshot) pairs, 100 snapshots (z = 1 -> 0).  W warmup
+snapshots run untimed, then K snapshots are timed with the device
+synchronized on both sides.  Inputs (spectra, all snapshot backgrounds,
+sources) are resident in HBM before the timed region starts.
+
+The JSON line reports the fused (persistent, cost-ordered) path's
+tracer-substeps/s; the plain one-tracer-per-thread path on the same
+snapshots; a served-stream latency summary; the roofline of the fused
+kernel; and the CPU oracle timed on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+for _p in (os.path.join(REPO, "croma-fp_amd"), os.path.join(REPO, "oracle"), os.path.join(REPO, "tests")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+# FP64 flops per row and species-substep of the solve (reference Thomas: rhs
+# add 1, forward elimination 3 + 2, back substitution 2 + 2 in the scan form).
+FP64_FLOP_PER_ROW = 10
+
+
+def load_traffic(name):
+    """HBM bytes per launch measured with rocprofv3 FETCH_SIZE/WRITE_SIZE, from
+    the committed summary profiles/<name>.txt ("bytes_per_launch = ...")."""
+    path = os.path.join(REPO, "profiles", name + ".txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.strip().startswith("bytes_per_launch"):
+            try:
+                return float(line.split("=")[1].split()[0])
+            except (IndexError, ValueError):
+                return None
+    return None
+
+
+def make_batch(ch, ctx, w):
+    b = ch.Batch(ctx, w.ntracer, w.species, w.nsnap)
+    b.set_state(w.crp0 if w.protons else None, w.cre0, mass=w.mass)
+    for s in range(w.nsnap):
+        b.set_background(s, w.bg[s])
+        if s in w.qpi or s in w.qepri:
+            b.set_sources(s, w.qpi.get(s), w.qepri.get(s))
+    return b
+
+
+def run_path(ch, ctx, w, path, warmup, steps):
+    b = make_batch(ch, ctx, w)
+    try:
+        if warmup:
+            b.run(0, warmup, w.nsnap, w.z, w.dt_snap, path=path)
+        b.sync()
+        t0 = time.perf_counter()
+        t = b.run(warmup, warmup + steps, w.nsnap, w.z, w.dt_snap, elapsed=float(np.sum(w.dt_snap[:warmup])),
+                  path=path)
+        b.sync()
+        wall = time.perf_counter() - t0
+    finally:
+        b.close()
+    return wall, t
+
+
+def cpu_baseline(w, warmup, seconds_target=15.0):
+    """Oracle (C + OpenMP restatement of the reference CPU path) on a bounded
+    tracer sample of the same workload and snapshots.  The per-grid secondary
+    weights and the per-snapshot IC cooling rows (identical for every tracer)
+    are computed before timing, so the sample measures the per-tracer rate."""
+    import pyoracle as po
+    import run_oracle
+
+    nthreads = os.cpu_count() or 1
+    rng = np.random.default_rng(5)
+    nsample = max(nthreads * 8, 64)
+    idx = np.sort(rng.choice(w.ntracer, size=min(nsample, w.ntracer), replace=False))
+    sub = w.subset(idx)
+    prm = run_oracle.oracle_params(sub)
+    o = po.Oracle(sub.n, sub.pmin, sub.pmax, sub.pemin, sub.pemax)
+    secondary = o.secondary_kernel() if sub.protons else (np.zeros(sub.n * sub.n), np.full(sub.n, sub.n, np.int32))
+    rows = {}
+    crp, cre = sub.crp0.copy(), sub.cre0.copy()
+    steps_done, t_total, s = 0, 0.0, warmup
+    while s < w.nsnap and (t_total < seconds_target or s == warmup):
+        for zz in (sub.z[s], sub.z[min(s + 1, sub.nsnap)]):
+            if zz not in rows:
+                rows[zz] = o.ic_cooling_row(zz)
+        t0 = time.perf_counter()
+        crp, cre, ns = run_oracle.run_workload(sub, prm, s0=s, s1=s + 1, nthreads=nthreads, oracle=o, crp=crp,
+                                               cre=cre, return_nsub=True, secondary=secondary, ic_rows=rows)
+        t_total += time.perf_counter() - t0
+        steps_done += int(ns[0].sum())
+        s += 1
+    return {"value": steps_done / t_total, "unit": "tracer-substeps/s", "cores": nthreads, "kind": "port",
+            "sample": f"{len(idx)} of {w.ntracer} tracers, snapshots {warmup}..{s - 1} ({steps_done} "
+                      f"tracer-substeps in {t_total:.1f} s, OpenMP {nthreads} threads)"}
+
+
+def served_latency(ch, seconds=20.0, seed=3):
+    """Short Poisson stream of mixed chunks through the serving loop."""
+    import serve_stream
+
+    return serve_stream.run_stream(ch, seconds=seconds, seed=seed)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="c1", choices=["c0", "c1", "c2"])
+    ap.add_argument("--ntracer", type=int, default=0, help="override the configuration's tracer count")
+    ap.add_argument("--plain-steps", type=int, default=3, help="snapshots timed on the plain path (0 skips)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=20.0)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on exactly one GPU; the tracer update shards across GPUs "
+                                   "without communication (see DESIGN.md), but --gpus > 1 is not run here"}))
+        sys.exit(2)
+
+    import croma_hip as ch
+    import synth
+    from workloads import hip_params
+
+    if not ch.is_available():
+        raise SystemExit("no HIP device visible")
+    cfg = dict(synth.CONFIGS[args.config])
+    if args.ntracer:
+        cfg["ntracer"] = args.ntracer
+    w = synth.make_workload(seed=2026, **cfg)
+    steps = min(args.steps, w.nsnap - args.warmup)
+    ctx = ch.Context(w.n, hip_params(w))
+
+    wall, t = run_path(ch, ctx, w, ch.PATH_FUSED, args.warmup, steps)
+    value = t.cell_steps / wall
+    species = 2 if w.protons else 1
+
+    out = {
+        "metric": "tracer-substeps/s (fused batched path) + p50/p99 chunk latency, served vs plain, 1 MI355X",
+        "value": value,
+        "unit": "tracer-substeps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded log-uniform backgrounds, power-law spectra, DSA-like sources)",
+        "config": {"workload": f"{args.config}: {w.ntracer} tracers x {w.n} bins, "
+                               f"{'electron_proton' if w.protons else 'electron_only'}, "
+                               f"dpp={['', 'asa', 'ttd', 'off'][w.dpp_model]}, {w.nsnap} snapshots z=1->0",
+                   "ntracer": w.ntracer, "n": w.n, "species": species, "snapshots_timed": steps,
+                   "tile": ctx.get_tile(w.protons)},
+        "tracer_substeps": int(t.cell_steps),
+        "species_substeps": int(t.species_steps),
+        "host_ms": {"ic_rows": t.ic_rows_ms, "nsub": t.nsub_estimate_ms, "order": t.bucket_build_ms},
+    }
+
+    # roofline of the dominant kernel (k_fused), from per-launch HIP events on
+    # the library's stream: algorithmic bytes 48 n per species-substep
+    kern_ms = t.kernel_ms / max(t.kernel_launches, 1)
+    alg_bytes = 48.0 * w.n * t.species_steps / max(t.kernel_launches, 1)
+    achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
+    flops = FP64_FLOP_PER_ROW * w.n * t.species_steps / max(t.kernel_launches, 1)
+    out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(f"k_fused_{args.config}_traffic"),
+                       "kernel": "k_fused", "avg_launch_ms": kern_ms, "alg_bytes_per_launch": alg_bytes,
+                       "fp64_valu": {"achieved_tflops": flops / (kern_ms * 1e-3) / 1e12, "peak_tflops": FP64_PEAK_TFLOPS,
+                                     "frac": flops / (kern_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}}
+
+    if args.plain_steps > 0:
+        pw, pt = run_path(ch, ctx, w, ch.PATH_PLAIN, args.warmup, min(args.plain_steps, steps))
+        fw, ft = run_path(ch, ctx, w, ch.PATH_FUSED, args.warmup, min(args.plain_steps, steps))
+        out["plain"] = {"value": pt.cell_steps / pw, "snapshots": min(args.plain_steps, steps),
+                        "fused_same_snapshots": ft.cell_steps / fw,
+                        "speedup_fused_vs_plain": (ft.cell_steps / fw) / (pt.cell_steps / pw)}
+    ctx.close()
+
+    if not args.no_serve:
+        try:
+            out["served"] = served_latency(ch, seconds=args.serve_seconds)
+        except (ImportError, AttributeError, OSError) as exc:  # serving loop not built
+            out["served"] = {"error": str(exc)}
+
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(w, args.warmup)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

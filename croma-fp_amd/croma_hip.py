@@ -57,7 +57,8 @@ class Times(C.Structure):
     _fields_ = [
         ("nsub_estimate_ms", C.c_double), ("bucket_build_ms", C.c_double), ("ic_rows_ms", C.c_double),
         ("h2d_ms", C.c_double), ("solve_ms", C.c_double), ("d2h_ms", C.c_double), ("total_ms", C.c_double),
-        ("cell_steps", C.c_longlong), ("species_steps", C.c_longlong),
+        ("cell_steps", C.c_longlong), ("species_steps", C.c_longlong), ("kernel_ms", C.c_double),
+        ("kernel_launches", C.c_longlong),
     ]
 
 

@@ -449,6 +449,8 @@ struct CromaHipBatch {
     int *hist;
     void *ev_up[2];
     void *ev_a, *ev_b;
+    void **ev_k; /* per-launch timing events, 2 per snapshot */
+    int ev_k_cap;
     /* plain path */
     int *d_idx;
     int *h_idx;
@@ -487,6 +489,8 @@ void croma_hip_batch_destroy(CromaHipBatch *b)
     }
     ck_event_destroy(b->ev_a);
     ck_event_destroy(b->ev_b);
+    for (k = 0; k < b->ev_k_cap; k++) ck_event_destroy(b->ev_k[k]);
+    free(b->ev_k);
     free(b);
 }
 
@@ -972,11 +976,26 @@ int croma_hip_batch_run(CromaHipBatch *b, int s0, int s1, int nsnap_total, const
     double t_start, el = elapsed;
     int s, rc = 0, k = 0;
     float dev_ms = 0.0f;
+    double kern_ms = 0.0;
     if (!b || !z || !dt_snap || s0 < 0 || s1 < s0 || s1 > b->nsnap_res || nsnap_total < s1 ||
         !(path == CROMA_HIP_PATH_FUSED || path == CROMA_HIP_PATH_PLAIN))
         return CROMA_HIP_EINVAL;
     if (s1 == s0) return 0;
     c = b->ctx;
+    if (path == CROMA_HIP_PATH_FUSED && b->ev_k_cap < 2 * (s1 - s0)) {
+        int need = 2 * (s1 - s0), j;
+        void **ne = (void **)realloc(b->ev_k, (size_t)need * sizeof(void *));
+        if (!ne) return CROMA_HIP_ENOMEM;
+        b->ev_k = ne;
+        for (j = b->ev_k_cap; j < need; j++) {
+            b->ev_k[j] = ck_event_create();
+            if (!b->ev_k[j]) {
+                b->ev_k_cap = j;
+                return dev_fail("event");
+            }
+        }
+        b->ev_k_cap = need;
+    }
     memset(&ps, 0, sizeof(ps));
     t_start = ch_now_s();
     if (ck_event_record(b->ev_a, c->stream)) return dev_fail("event");
@@ -985,8 +1004,13 @@ int croma_hip_batch_run(CromaHipBatch *b, int s0, int s1, int nsnap_total, const
     for (s = s0; s < s1; s++) {
         k = (s - s0) & 1;
         if ((rc = upload_snapshot(b, k)) != 0) return rc;
-        if (path == CROMA_HIP_PATH_FUSED) rc = launch_fused(b, k);
-        else rc = run_plain_snapshot(b, k);
+        if (path == CROMA_HIP_PATH_FUSED) {
+            if (ck_event_record(b->ev_k[2 * (s - s0)], c->stream)) return dev_fail("event");
+            rc = launch_fused(b, k);
+            if (!rc && ck_event_record(b->ev_k[2 * (s - s0) + 1], c->stream)) return dev_fail("event");
+        } else {
+            rc = run_plain_snapshot(b, k);
+        }
         if (rc) return rc;
         el += dt_snap[s];
         if (s + 1 < s1) {
@@ -999,7 +1023,11 @@ int croma_hip_batch_run(CromaHipBatch *b, int s0, int s1, int nsnap_total, const
     if (ck_event_record(b->ev_b, c->stream)) return dev_fail("event");
     rc = croma_hip_batch_sync(b);
     dev_ms = ck_event_elapsed_ms(b->ev_a, b->ev_b);
+    if (path == CROMA_HIP_PATH_FUSED)
+        for (s = s0; s < s1; s++) kern_ms += ck_event_elapsed_ms(b->ev_k[2 * (s - s0)], b->ev_k[2 * (s - s0) + 1]);
     if (times) {
+        times->kernel_ms += kern_ms;
+        if (path == CROMA_HIP_PATH_FUSED) times->kernel_launches += s1 - s0;
         times->ic_rows_ms += ps.ic_ms;
         times->nsub_estimate_ms += ps.nsub_ms;
         times->bucket_build_ms += ps.order_ms;

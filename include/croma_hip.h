@@ -104,6 +104,8 @@ typedef struct {
     double total_ms;
     long long cell_steps;     /* tracer-substeps: sum of nsub over tracers */
     long long species_steps;  /* sum of nsub x active species */
+    double kernel_ms;         /* device time inside the fused snapshot kernels */
+    long long kernel_launches;/* fused snapshot kernel launches timed in kernel_ms */
 } CromaHipTimes;
 
 /* ---- context: device, grid tables, streams (cuda_pipeline_workspace_*,

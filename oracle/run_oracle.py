@@ -18,22 +18,36 @@ def oracle_params(w, **kw):
     return po.default_params(**base)
 
 
-def run_workload(w, prm=None, s0=0, s1=None, nthreads=0, oracle=None, crp=None, cre=None, return_nsub=False):
-    """Advance the workload's spectra over snapshots [s0, s1); returns (crp, cre[, nsub list])."""
+def run_workload(w, prm=None, s0=0, s1=None, nthreads=0, oracle=None, crp=None, cre=None, return_nsub=False,
+                 secondary=None, ic_rows=None):
+    """Advance the workload's spectra over snapshots [s0, s1); returns (crp, cre[, nsub list]).
+
+    secondary: precomputed (fqe_t, np_min_qe) of the grid; ic_rows: dict
+    z -> (row, m1, p1) of precomputed IC cooling rows (both are per-grid or
+    per-snapshot constants shared by every tracer)."""
     o = oracle or po.Oracle(w.n, w.pmin, w.pmax, w.pemin, w.pemax)
     prm = prm or oracle_params(w)
     s1 = w.nsnap if s1 is None else s1
     crp = (w.crp0 if crp is None else crp).copy()
     cre = (w.cre0 if cre is None else cre).copy()
-    fqe, npmin = (o.secondary_kernel() if w.protons else (np.zeros(w.n * w.n), np.full(w.n, w.n, np.int32)))
+    if secondary is not None:
+        fqe, npmin = secondary
+    else:
+        fqe, npmin = (o.secondary_kernel() if w.protons else (np.zeros(w.n * w.n), np.full(w.n, w.n, np.int32)))
+    rows = ic_rows if ic_rows is not None else {}
+
+    def ic(z):
+        if z not in rows:
+            rows[z] = o.ic_cooling_row(z)
+        return rows[z]
     zeros = np.zeros(w.ntracer * w.n)
     elapsed = float(np.sum(w.dt_snap[:s0]))
     nsubs = []
     for s in range(s0, s1):
         allow = s + 1 < w.nsnap
-        row, m1, p1 = o.ic_cooling_row(w.z[s])
+        row, m1, p1 = ic(w.z[s])
         if allow:
-            rown, m1n, p1n = o.ic_cooling_row(w.z[s + 1])
+            rown, m1n, p1n = ic(w.z[s + 1])
         else:
             rown, m1n, p1n = row, m1, p1
         bgc = w.bg[s]
