@@ -50,7 +50,7 @@ class Params(C.Structure):
 
 
 class Tile(C.Structure):
-    _fields_ = [("rows_per_lane", C.c_int), ("waves_per_block", C.c_int), ("blocks_per_cu", C.c_int)]
+    _fields_ = [("rows_per_lane", C.c_int), ("waves_per_simd", C.c_int), ("chain", C.c_int)]
 
 
 class Times(C.Structure):
@@ -177,19 +177,20 @@ class Context:
         self.params = params
         _chk(lib().croma_hip_context_set_params(C.c_void_p(self.h), C.byref(params)), "set_params")
 
-    def set_tile(self, rows_per_lane, waves_per_block, blocks_per_cu):
-        t = Tile(rows_per_lane, waves_per_block, blocks_per_cu)
-        _chk(lib().croma_hip_context_set_tile(C.c_void_p(self.h), C.byref(t)), "set_tile")
+    def set_tile(self, rows_per_lane, waves_per_simd, chain=1, protons=-1):
+        """protons: 0 electron_only tile, 1 electron_proton tile, -1 both."""
+        t = Tile(rows_per_lane, waves_per_simd, chain)
+        _chk(lib().croma_hip_context_set_tile(C.c_void_p(self.h), int(protons), C.byref(t)), "set_tile")
 
     def get_tile(self, protons=False):
         t = Tile()
         _chk(lib().croma_hip_context_get_tile(C.c_void_p(self.h), int(protons), C.byref(t)), "get_tile")
-        return (t.rows_per_lane, t.waves_per_block, t.blocks_per_cu)
+        return (t.rows_per_lane, t.waves_per_simd, t.chain)
 
     def autotune(self, protons=False, ntracer=16384):
         t = Tile()
         _chk(lib().croma_hip_autotune(C.c_void_p(self.h), int(protons), int(ntracer), C.byref(t)), "autotune")
-        return (t.rows_per_lane, t.waves_per_block, t.blocks_per_cu)
+        return (t.rows_per_lane, t.waves_per_simd, t.chain)
 
     def grid(self, species):
         n = self.n
@@ -205,6 +206,14 @@ class Context:
         _chk(lib().croma_hip_ic_cooling_row(C.c_void_p(self.h), C.c_double(z), _p(row), C.byref(m1), C.byref(p1)),
              "ic_cooling_row")
         return row, m1.value, p1.value
+
+    def ic_cooling_rows_device(self, zs):
+        zs = f64(zs)
+        rows = np.zeros((len(zs), self.n))
+        gh = np.zeros((len(zs), 2))
+        _chk(lib().croma_hip_ic_cooling_rows_device(C.c_void_p(self.h), len(zs), _p(zs), _p(rows), _p(gh)),
+             "ic_cooling_rows_device")
+        return rows, gh
 
     def nsub(self, dt_snap, n_gas, kbt, b_field, divv, ric_lo, ric_hi, l_turb, dv):
         nt = len(n_gas)

@@ -272,10 +272,50 @@ __device__ __forceinline__ double d_lerp(double cur, double nxt, double alpha)
 }
 
 // ---------------------------------------------------------------------------
-// Lane-group cross-lane helpers.  G lanes per group, groups aligned in the wave.
-// Out-of-group sources are masked by the callers (zero coefficients), so the
-// raw shuffles below may read a neighbouring group's lane.
+// Lane-group cross-lane helpers.  G lanes per group, groups aligned in the
+// wave.  Groups that fit a 16-lane row shift with DPP (row_shr / row_shl, no
+// LDS round trip); wider groups use ds_bpermute.  A lane whose
+// source lies outside its group reads a neighbouring group's (finite) value;
+// every caller multiplies such values by a zero coefficient.
 // ---------------------------------------------------------------------------
+template <int CTRL>
+__device__ __forceinline__ double dpp64(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, true);
+    return __hiloint2double(hi, lo);
+}
+
+// value of lane (l - S) inside the group segment (S a power of two < G)
+template <int G, int S>
+__device__ __forceinline__ double sh_up(double v)
+{
+    if constexpr (G <= 16) return dpp64<0x110 + S>(v);
+    else return __shfl_up(v, (unsigned)S, G);
+}
+
+// value of lane (l + S)
+template <int G, int S>
+__device__ __forceinline__ double sh_down(double v)
+{
+    if constexpr (G <= 16) return dpp64<0x100 + S>(v);
+    else return __shfl_down(v, (unsigned)S, G);
+}
+
+// lane (l - 1) / (l + 1) across row boundaries of the group
+template <int G>
+__device__ __forceinline__ double sh_up1(double v)
+{
+    if constexpr (G <= 16) return dpp64<0x111>(v);
+    else return __shfl_up(v, 1u, G);
+}
+template <int G>
+__device__ __forceinline__ double sh_down1(double v)
+{
+    if constexpr (G <= 16) return dpp64<0x101>(v);
+    else return __shfl_down(v, 1u, G);
+}
+
 template <int G>
 __device__ __forceinline__ double g_up(double v, int s)
 {
@@ -298,39 +338,45 @@ struct LogG {
     static constexpr int v = (N <= 1) ? 0 : 1 + LogG<N / 2>::v;
 };
 
-// Per-species factorised operator held in registers for one segment.
+// Factorised Chang-Cooper operator of one species for one coefficient run,
+// held in registers.  One implicit substep is x <- M^{-1} (x + src):
+//   forward   d_i = al_i d_{i-1} + iv_i x_i + vs_i
+//   backward  x_i = d_i + g_i x_{i+1}
+// with the lane-level affine scans over the group (Af, Qb: products of al
+// resp. g over lane ranges, zero where the source lane is outside the group).
 template <int R, int G>
 struct Factor {
-    double al[R];   // a_i / denom_i            (forward multiplier)
+    double al[R];   // a_i / denom_i
     double iv[R];   // 1 / denom_i
-    double P[R];    // prefix product of al inside the lane
-    double g[R];    // -cprime_i = c_i / denom_i (backward multiplier)
-    double Q[R];    // suffix product of g inside the lane
-    double Af[LogG<G>::v + 1];  // forward scan coefficients per step (0 = masked)
-    double Qb[LogG<G>::v + 1];  // backward scan coefficients per step
+    double vs[R];   // src_i / denom_i
+    double g[R];    // c_i / denom_i (= -cprime_i of the reference)
+    double Af[LogG<G>::v + 1];
+    double Qb[LogG<G>::v + 1];
 };
 
-// Thomas LU factorisation of the group's system (fp_shared_core.c:226-257):
-// denom_i = b_i + a_i cprime_{i-1}, cprime_i = -c_i / denom_i.
+// Thomas LU of the rows held in F (on entry al = a, iv = b, g = c), as in
+// fp_thomas_solve_ws (fp_shared_core.c:226-257): denom_i = b_i + a_i cprime_{i-1},
+// cprime_i = -c_i / denom_i, pivot guard |denom| <= 1e-300.  The pivot
+// entering each lane comes from a scan of 2x2 projective maps across the
+// group's lanes (renormalised by powers of two, so exact up to rounding of
+// the products); inside the lane the reference recurrence is applied.
 template <int R, int G>
-__device__ void factorize(int gl, int n, const double (&a)[R], const double (&b)[R], const double (&c)[R],
-                          Factor<R, G> &F, int &bad)
+__device__ __forceinline__ void factorize(int gl, int n, const double *src, double src_scale, Factor<R, G> &F,
+                                          int &bad)
 {
-    // lane product of M_i = [[0, -c_i], [a_i, b_i]] acting on (u, v), cprime = u / v
     double m00 = 1.0, m01 = 0.0, m10 = 0.0, m11 = 1.0;
 #pragma unroll
     for (int k = 0; k < R; ++k) {
-        const double n00 = -c[k] * m10, n01 = -c[k] * m11;
-        const double n10 = a[k] * m00 + b[k] * m10, n11 = a[k] * m01 + b[k] * m11;
+        const double a = F.al[k], b = F.iv[k], c = F.g[k];
+        const double n00 = -c * m10, n01 = -c * m11;
+        const double n10 = a * m00 + b * m10, n11 = a * m01 + b * m11;
         m00 = n00; m01 = n01; m10 = n10; m11 = n11;
         const double mx = fmax(fmax(fabs(m00), fabs(m01)), fmax(fabs(m10), fabs(m11)));
         if (mx > 0.0 && isfinite(mx)) {
-            int e;
-            frexp(mx, &e);
+            const int e = __builtin_amdgcn_frexp_exp(mx);
             m00 = ldexp(m00, -e); m01 = ldexp(m01, -e); m10 = ldexp(m10, -e); m11 = ldexp(m11, -e);
         }
     }
-    // inclusive scan S_l = T_l S_{l-1} (exact power-of-two renormalisation)
 #pragma unroll
     for (int s = 1; s < G; s <<= 1) {
         const double o00 = g_up<G>(m00, s), o01 = g_up<G>(m01, s);
@@ -341,8 +387,7 @@ __device__ void factorize(int gl, int n, const double (&a)[R], const double (&b)
             m00 = n00; m01 = n01; m10 = n10; m11 = n11;
             const double mx = fmax(fmax(fabs(m00), fabs(m01)), fmax(fabs(m10), fabs(m11)));
             if (mx > 0.0 && isfinite(mx)) {
-                int e;
-                frexp(mx, &e);
+                const int e = __builtin_amdgcn_frexp_exp(mx);
                 m00 = ldexp(m00, -e); m01 = ldexp(m01, -e); m10 = ldexp(m10, -e); m11 = ldexp(m11, -e);
             }
         }
@@ -350,32 +395,22 @@ __device__ void factorize(int gl, int n, const double (&a)[R], const double (&b)
     const double uin = g_up<G>(m01, 1), vin = g_up<G>(m11, 1);
     double cprev = (gl == 0) ? 0.0 : uin / vin;
     const int base = gl * R;
+    double A = 1.0, Bq = 1.0;
 #pragma unroll
     for (int k = 0; k < R; ++k) {
-        const double den = b[k] + a[k] * cprev;
+        const double a = F.al[k], b = F.iv[k], c = F.g[k];
+        const double den = b + a * cprev;
         if (!(fabs(den) > 1.0e-300)) bad |= ERR_PIVOT;
         const double inv = 1.0 / den;
-        const double cp = (base + k == n - 1) ? 0.0 : -c[k] * inv;
+        const double cp = (base + k == n - 1) ? 0.0 : -c * inv;
         F.iv[k] = inv;
-        F.al[k] = a[k] * inv;
+        F.al[k] = a * inv;
+        F.vs[k] = src ? inv * (src_scale * src[base + k]) : 0.0;
         F.g[k] = -cp;
+        A *= F.al[k];
+        Bq *= F.g[k];
         cprev = cp;
     }
-    double pr = 1.0;
-#pragma unroll
-    for (int k = 0; k < R; ++k) {
-        pr *= F.al[k];
-        F.P[k] = pr;
-    }
-    double sf = 1.0;
-#pragma unroll
-    for (int k = R - 1; k >= 0; --k) {
-        sf *= F.g[k];
-        F.Q[k] = sf;
-    }
-    // scan coefficient schedules (masked outside the group)
-    double A = F.P[R - 1];
-    double Bq = F.Q[0];
     int st = 0;
 #pragma unroll
     for (int s = 1; s < G; s <<= 1, ++st) {
@@ -388,107 +423,434 @@ __device__ void factorize(int gl, int n, const double (&a)[R], const double (&b)
     }
 }
 
-// Solve M x = r for the group, in place; rhs = x + src (src = dt*Q).
-template <int R, int G>
-__device__ __forceinline__ void apply_solve(int gl, const Factor<R, G> &F, double (&x)[R], const double (&src)[R])
+template <int G, int S, int ST>
+__device__ __forceinline__ void scan_fwd(const double (&Af)[LogG<G>::v + 1], double &B)
 {
-    double L[R];
-    double prev = 0.0;
-#pragma unroll
-    for (int k = 0; k < R; ++k) {
-        const double r = src[k] + x[k];
-        prev = fma(F.al[k], prev, F.iv[k] * r);
-        L[k] = prev;
+    if constexpr (S < G) {
+        B = fma(Af[ST], sh_up<G, S>(B), B);
+        scan_fwd<G, 2 * S, ST + 1>(Af, B);
     }
-    double B = prev;
-    int st = 0;
-#pragma unroll
-    for (int s = 1; s < G; s <<= 1, ++st) B = fma(F.Af[st], g_up<G>(B, s), B);
-    double Din = g_up<G>(B, 1);
-    if (gl == 0) Din = 0.0;
-#pragma unroll
-    for (int k = 0; k < R; ++k) L[k] = fma(F.P[k], Din, L[k]);
-    // back substitution x_k = d_k + g_k x_{k+1}
-    double M = 0.0;
-#pragma unroll
-    for (int k = R - 1; k >= 0; --k) {
-        M = fma(F.g[k], M, L[k]);
-        x[k] = M;
-    }
-    double Xb = M;
-    st = 0;
-#pragma unroll
-    for (int s = 1; s < G; s <<= 1, ++st) Xb = fma(F.Qb[st], g_down<G>(Xb, s), Xb);
-    double Xin = g_down<G>(Xb, 1);
-    if (gl == G - 1) Xin = 0.0;
-#pragma unroll
-    for (int k = 0; k < R; ++k) x[k] = fma(F.Q[k], Xin, x[k]);
 }
 
-// Build one species' Chang-Cooper rows for the lane (needs neighbour lanes).
-template <int R, int G>
-__device__ void build_cc(int gl, int n, double dt, const double *__restrict__ xg,
-                         const double *__restrict__ dxg, const double (&B)[R], const double (&C)[R],
-                         const double (&dtT)[R], double (&a)[R], double (&b)[R], double (&c)[R])
+template <int G, int S, int ST>
+__device__ __forceinline__ void scan_bwd(const double (&Qb)[LogG<G>::v + 1], double &B)
 {
-    const int base = gl * R;
-    double x[R], dx[R];
-#pragma unroll
-    for (int k = 0; k < R; ++k) {
-        x[k] = xg[base + k];
-        dx[k] = dxg[base + k];
+    if constexpr (S < G) {
+        B = fma(Qb[ST], sh_down<G, S>(B), B);
+        scan_bwd<G, 2 * S, ST + 1>(Qb, B);
     }
-    const double Bn = g_down<G>(B[0], 1), Cn = g_down<G>(C[0], 1);
-    const double xn = (gl < G - 1) ? xg[base + R] : 0.0;
-    // faces base+k (rows base+k, base+k+1); the group's last face is the ghost face (unused)
-    double Wp[R], Wm[R], Wd[R];
+}
+
+// One implicit substep: x <- M^{-1} (x + src) for the group, in place.
+template <int R, int G>
+__device__ __forceinline__ void apply_solve(int gl, const Factor<R, G> &F, double (&x)[R])
+{
+    double t[R];
+    double L = 0.0;
 #pragma unroll
     for (int k = 0; k < R; ++k) {
-        const double br = (k < R - 1) ? B[k + 1] : Bn;
-        const double cr = (k < R - 1) ? C[k + 1] : Cn;
-        const double xr = (k < R - 1) ? x[k + 1] : xn;
-        double w;
-        if (base + k < n - 1) {
-            const bool ok = d_face(B[k], br, C[k], cr, x[k], xr, Wp[k], Wm[k], w);
-            Wd[k] = w;
-            (void)ok;
+        t[k] = fma(F.iv[k], x[k], F.vs[k]);
+        L = fma(F.al[k], L, t[k]);
+    }
+    scan_fwd<G, 1, 0>(F.Af, L);
+    double d = sh_up1<G>(L);
+    if (gl == 0) d = 0.0;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        d = fma(F.al[k], d, t[k]);
+        t[k] = d;
+    }
+    double M = 0.0;
+#pragma unroll
+    for (int k = R - 1; k >= 0; --k) M = fma(F.g[k], M, t[k]);
+    scan_bwd<G, 1, 0>(F.Qb, M);
+    double xn = sh_down1<G>(M);
+    if (gl == G - 1) xn = 0.0;
+#pragma unroll
+    for (int k = R - 1; k >= 0; --k) {
+        xn = fma(F.g[k], xn, t[k]);
+        x[k] = xn;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Coefficient build of one species for one run
+// ---------------------------------------------------------------------------
+
+// Per-species device grid view.
+struct SpGrid {
+    const double *x, *h, *inv_h, *inv_dx, *two_over_p, *p2, *expcut;
+};
+
+__device__ __forceinline__ SpGrid sp_grid(const CkGrid &g, bool elec)
+{
+    SpGrid s;
+    if (elec) {
+        s.x = g.ep; s.h = g.e_h; s.inv_h = g.e_inv_h; s.inv_dx = g.e_inv_dx; s.two_over_p = g.e_2op;
+        s.p2 = g.e_p2; s.expcut = g.e_expcut;
+    } else {
+        s.x = g.pp; s.h = g.p_h; s.inv_h = g.p_inv_h; s.inv_dx = g.p_inv_dx; s.two_over_p = g.p_2op;
+        s.p2 = g.p_p2; s.expcut = g.p_expcut;
+    }
+    return s;
+}
+
+// Face between rows j and j+1 (cc_prepare_face_weight + cc_eval_face_weights,
+// Chang_Cooper.c:22-66): w = (B/C)_face * h, W+ = B(-w), W- = B(w); both zero
+// when |C_j| <= 1e-200.  Fh = C_face / h.  The Bernoulli pair is formed from
+// one exponential h2 = exp(-|w|/2):
+//   |w| <  0.1: W_big = 1 / (h2 * den), den = 1 + w^2/24 + w^4/1920
+//   |w| >= 0.1: W_big = |w| / (1 - h2^2)
+//   W_small = W_big * h2^2; W+ = W_big for w > 0 (else W_small).
+struct Face {
+    double bm, Fh, wp, wm, w;
+};
+
+__device__ __forceinline__ Face d_face2(double Bl, double Br, double Cl, double Cr, double h, double inv_h)
+{
+#pragma clang fp contract(off)
+    Face f;
+    f.bm = 0.5 * (Bl + Br);
+    const double cm = 0.5 * (Cl + Cr);
+    f.Fh = cm * inv_h;
+    f.wp = 0.0;
+    f.wm = 0.0;
+    f.w = 0.0;
+    if (fabs(Cl) > 1.0e-200) {
+        const double w = f.bm / cm * h;
+        const double aw = fabs(w);
+        const double h2 = exp(-0.5 * aw);
+        const double E = h2 * h2;
+        const bool small = aw < 0.1;
+        const double w2 = w * w;
+        const double den = small ? h2 * (1.0 + w2 / 24.0 + (w2 * w2) / 1920.0) : 1.0 - E;
+        const double big = (small ? 1.0 : aw) / den;
+        const double sm = big * E;
+        f.wp = (w > 0.0) ? big : sm;
+        f.wm = (w > 0.0) ? sm : big;
+        f.w = w;
+    }
+    return f;
+}
+
+// Row j of the Chang-Cooper system (fp_shared_core.c:660-704 ->
+// Chang_Cooper.c:68-184), A = 1, from its two faces.  kind: 0 interior, 1 left
+// boundary, 2 right boundary.  Output as positive coupling magnitudes
+// b x_j - a x_{j-1} - c x_{j+1}.
+__device__ __forceinline__ void d_cc_row2(int kind, double t1, double C, double dtT, const Face &fl,
+                                          const Face &fr, double Cl, double &a, double &b, double &c)
+{
+#pragma clang fp contract(off)
+    const bool cz = !(fabs(C) > 1.0e-200);
+    if (kind == 0) {
+        if (!cz) {
+            a = t1 * fl.Fh * fl.wm;
+            b = 1.0 + t1 * (fl.Fh * fl.wp + fr.Fh * fr.wm) + dtT;
+            c = t1 * fr.Fh * fr.wp;
         } else {
-            Wp[k] = 0.0;
-            Wm[k] = 0.0;
-            Wd[k] = 0.0;
+            a = 0.0;
+            b = 1.0 + t1 * fl.bm + dtT;
+            c = t1 * fr.bm;
+        }
+    } else if (kind == 1) {
+        a = 0.0;
+        if (!cz) {
+            b = 1.0 + t1 * (fr.Fh * fr.wm) + dtT;
+            c = t1 * fr.Fh * fr.wp;
+        } else {
+            b = 1.0 + dtT;
+            c = t1 * fr.bm;
+        }
+    } else {
+        c = 0.0;
+        if (!cz) {
+            a = t1 * fl.Fh * fl.wm;
+            b = 1.0 + t1 * (fl.Fh * fl.wp) + dtT;
+        } else {
+            // delta of the left face (Chang_Cooper.c:143-184): nonzero only
+            // when that face carries weights
+            const double delta = (fabs(Cl) > 1.0e-200) ? d_face_delta(fl.w) : 0.0;
+            a = -delta * fl.bm * t1;
+            b = 1.0 + t1 * fl.bm + dtT;
         }
     }
-    const double Wpl0 = g_up<G>(Wp[R - 1], 1), Wml0 = g_up<G>(Wm[R - 1], 1);
-    const double Wdl0 = g_up<G>(Wd[R - 1], 1);
-    const double Bl0 = g_up<G>(B[R - 1], 1), Cl0 = g_up<G>(C[R - 1], 1);
-    const double xl0 = (gl > 0) ? xg[base - 1] : 0.0;
+}
+
+// Segment-level loss inputs (backgrounds lerped to the segment midpoint).
+struct SegCoef {
+    double n_gas, kbt;
+    double adia;         // (div v) / 3
+    double synch_coeff;  // electrons: b_synch prefactor at this B
+    double coul_coeff;   // electrons: Coulomb prefactor at this n_gas
+    double alpha;        // lerp weight of the IC rows (0: current row only)
+    const double *ric_c, *ric_n;
+};
+
+// Run-level coefficient inputs: D(p) = p^2 * Dshape(p) * dfac (+ eta scale),
+// B = radp - (2/p) D, C = D; dt/T from dtT_fac * sig (protons).
+struct RunCoef {
+    double dt;
+    double dfac;     // inv_on * scale (model) or scale / 4e5 (floor)
+    bool model;      // filled form (p^2 inv exp(-E/Epmax) exp(-1/p)) vs floor p^2/4e5
+    double inv_epmax;
+    double dtT_fac;  // dt * 0.5 c n_gas Gyr (protons), 0 for electrons
+};
+
+// Loss rate of row j (prepare_cre_losses_1d, fp_shared_core.c:780-842;
+// prepare_crp_losses_1d_with_threshold :729-778), recomputed per run so no
+// per-row loss array has to stay live.
+template <bool ELEC>
+__device__ __forceinline__ double row_radp(const CkGrid &g, const SegCoef &sc, int j)
+{
+#pragma clang fp contract(off)
+    if (ELEC) {
+        const double pe = g.ep[j];
+        const double pe2 = g.e_p2[j];
+        const double beta2 = pe2 / (1.0 + pe2);
+        const double ric = (sc.alpha > 0.0) ? d_lerp(sc.ric_c[j], sc.ric_n[j], sc.alpha) : sc.ric_c[j];
+        const double coul = sc.coul_coeff * (1.0 + log(pe / sc.n_gas) / 74.8) / beta2;
+        return sc.synch_coeff * pe2 + ric + coul + sc.adia * pe;
+    } else {
+        const double p = g.pp[j];
+        return d_b_coulomb_p(sc.n_gas, p, sc.kbt) + sc.adia * p;
+    }
+}
+
+template <bool ELEC>
+__device__ __forceinline__ void row_BC(const CkGrid &g, const SpGrid &sg, const SegCoef &sc, const RunCoef &rc,
+                                       int j, double &B, double &C)
+{
+#pragma clang fp contract(off)
+    const double radp = row_radp<ELEC>(g, sc, j);
+    double D;
+    if (!rc.model) D = sg.p2[j] * rc.dfac;
+    else if (ELEC) D = sg.p2[j] * rc.dfac * sg.expcut[j];
+    else D = sg.p2[j] * rc.dfac * exp(-g.pE[j] * rc.inv_epmax) * sg.expcut[j];
+    C = D;
+    B = radp - sg.two_over_p[j] * D;
+}
+
+// Builds the rows of the lane into F (al = a, iv = b, g = c) and factorises.
+// Row k's (B, C) is computed once and carried to the next face.  src[j] *
+// src_scale is the row's source term (src NULL: none).
+template <bool ELEC, int R, int G>
+__device__ __forceinline__ void build_factor(int gl, int n, const CkGrid &g, const SegCoef &sc, const RunCoef &rc,
+                                             const double *src, double src_scale, Factor<R, G> &F, int &bad)
+{
+#pragma clang fp contract(off)
+    const SpGrid sg = sp_grid(g, ELEC);
+    const int base = gl * R;
+    double B0, C0, BL, CL;
+    row_BC<ELEC>(g, sg, sc, rc, base, B0, C0);
+    row_BC<ELEC>(g, sg, sc, rc, base + R - 1, BL, CL);
+    const double Bn = g_down<G>(B0, 1), Cn = g_down<G>(C0, 1);
+    const double Bp = g_up<G>(BL, 1), Cp = g_up<G>(CL, 1);
+    Face fl = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (gl > 0) fl = d_face2(Bp, B0, Cp, C0, sg.h[base - 1], sg.inv_h[base - 1]);
+    double Cl = Cp;
+    double Bk = B0, Ck = C0;
 #pragma unroll
     for (int k = 0; k < R; ++k) {
-        const int m = base + k;
-        const int kind = (m == 0) ? 1 : ((m == n - 1) ? 2 : 0);
-        const double xl = (k > 0) ? x[k - 1] : xl0;
-        const double Bl = (k > 0) ? B[k - 1] : Bl0;
-        const double Cl = (k > 0) ? C[k - 1] : Cl0;
-        const double Wpl = (k > 0) ? Wp[k - 1] : Wpl0;
-        const double Wml = (k > 0) ? Wm[k - 1] : Wml0;
-        const double wl = (k > 0) ? Wd[k - 1] : Wdl0;
-        const double xr = (k < R - 1) ? x[k + 1] : xn;
-        const double Br = (k < R - 1) ? B[k + 1] : Bn;
-        const double Cr = (k < R - 1) ? C[k + 1] : Cn;
-        double delta_l = 0.0;
-        if (kind == 2 && !(fabs(C[k]) > 1.0e-200) && fabs(Cl) > 1.0e-200) delta_l = d_face_delta(wl);
-        d_cc_row(kind, dt, x[k], dx[k], B[k], C[k], dtT[k], xl, Bl, Cl, Wpl, Wml, delta_l, xr, Br, Cr, Wp[k],
-                 Wm[k], a[k], b[k], c[k]);
+        const int j = base + k;
+        const bool last = (j == n - 1);
+        double Br, Cr;
+        if (k == R - 1) { Br = Bn; Cr = Cn; }
+        else if (k == R - 2) { Br = BL; Cr = CL; }
+        else row_BC<ELEC>(g, sg, sc, rc, j + 1, Br, Cr);
+        Face fr = {0.0, 0.0, 0.0, 0.0, 0.0};
+        if (!last) fr = d_face2(Bk, Br, Ck, Cr, sg.h[j], sg.inv_h[j]);
+        const int kind = (j == 0) ? 1 : (last ? 2 : 0);
+        const double t1 = rc.dt * sg.inv_dx[j];
+        const double dtT = ELEC ? 0.0 : rc.dtT_fac * g.p_sig[j];
+        double a, b, c;
+        d_cc_row2(kind, t1, Ck, dtT, fl, fr, Cl, a, b, c);
+        F.al[k] = a;
+        F.iv[k] = b;
+        F.g[k] = c;
+        fl = fr;
+        Cl = Ck;
+        Bk = Br;
+        Ck = Cr;
+        // keep the rows' transcendental chains from being interleaved (each
+        // row's temporaries die before the next row starts)
+        __builtin_amdgcn_sched_barrier(0);
     }
+    factorize<R, G>(gl, n, src, src_scale, F, bad);
 }
 
 // ---------------------------------------------------------------------------
 // Fused persistent kernel
 // ---------------------------------------------------------------------------
+
+// All runs of one species over one segment: up to three runs of constant
+// coefficients (before / inside / after the on-window, tracer_fp_cpu.c:150-157).
+template <bool ELEC, int N, int R, int G>
+__device__ __forceinline__ void species_segment(int gl, const CkGrid &g, int steps, bool windowed, int ls, int le,
+                                                const SegCoef &sc, RunCoef rc, double dfac_on, double dfac_off,
+                                                bool mode_on, const double *src, double src_scale, double (&x)[R],
+                                                int &bad)
+{
+    int s0 = 0;
+    while (s0 < steps) {
+        const bool on = !windowed || (s0 >= ls && s0 < le);
+        int s1;
+        if (!windowed) s1 = steps;
+        else if (s0 < ls) s1 = ls;
+        else if (s0 < le) s1 = le;
+        else s1 = steps;
+        // a segment wholly before the on-window has ls > steps
+        if (s1 > steps) s1 = steps;
+        rc.model = on && mode_on;
+        rc.dfac = on ? dfac_on : dfac_off;
+        Factor<R, G> F;
+        build_factor<ELEC, R, G>(gl, N, g, sc, rc, src, src_scale, F, bad);
+        for (int s = s0; s < s1; ++s) apply_solve<R, G>(gl, F, x);
+        s0 = s1;
+    }
+}
+
+// One snapshot of one tracer (tracer_fp_batch_execute_bucket,
+// tracer_fp_batch.c:651-827 -> tracer_fp_cpu_evolve, tracer_fp_cpu.c:88-196).
+// Within a segment the proton and electron updates are independent (the
+// electron source uses the segment-start protons), so the species run one
+// after the other and only one factor set is live.  s_row (LDS, n doubles
+// per tracer) holds the segment-start protons during the proton runs and
+// the electron source of the segment afterwards.
+template <int N, int R, bool PROT>
+__device__ __forceinline__ void snapshot_update(const CkGrid &g, const CkSlot &S, int t, int gl, double *s_row,
+                                                double (&xe)[R], double (&xp)[R], int &bad)
+{
+#pragma clang fp contract(off)
+    constexpr int G = N / R;
+    const int base = gl * R;
+    const CkParams &prm = S.prm;
+    const int nsub = S.nsub[t];
+    const int non = S.n_on[t];
+    const int nseg =
+        d_choose_segments(nsub, S.allow_interp, prm.coeff_interp_min_steps, prm.coeff_interp_max_segments);
+    const int on_start = (nsub - non) / 2;
+    const int on_end = on_start + non;
+    const double dt = S.dt_snap / (double)nsub;
+    const size_t toff = (size_t)t * N;
+    const double mass = S.mass ? S.mass[t] : 0.0;
+    const bool dis = S.dis ? (S.dis[t] != 0) : false;
+    const bool sec = PROT && S.fqe_t;
+
+    for (int iseg = 0; iseg < nseg; ++iseg) {
+        const int fb = (int)(((long long)iseg * nsub) / nseg);
+        const int fe = (int)(((long long)(iseg + 1) * nsub) / nseg);
+        const int steps = fe - fb;
+        if (steps <= 0) continue;
+        const double alpha = (nseg > 1) ? ((double)iseg + 0.5) / (double)nseg : 0.0;
+        double v[CK_NBG];
+#pragma unroll
+        for (int f = 0; f < CK_NBG; ++f) v[f] = (nseg > 1) ? d_lerp(S.bgc[f][t], S.bgn[f][t], alpha) : S.bgc[f][t];
+        const double n_gas = v[CK_BG_NGAS], lturb = v[CK_BG_LTURB], dv = v[CK_BG_DV], cs = v[CK_BG_CS];
+        const double beta = v[CK_BG_BETA];
+        const double divv = dis ? 0.0 : v[CK_BG_DIVV];
+        SegCoef sc;
+        sc.n_gas = n_gas;
+        sc.kbt = v[CK_BG_KBT];
+        sc.adia = 1.0 / 3.0 * divv;
+        {
+            const double b_micro = v[CK_BG_B] * 1.0e+6;
+            const double b_ratio = b_micro / 3.2;
+            sc.synch_coeff = 4.8e-4 * b_ratio * b_ratio * 9.11e-28 * 3.0e+10 * 3.1536e+16;
+            sc.coul_coeff = 3.05e-29 * n_gas * K_C * K_GYR / (K_ME * K_GEV);
+        }
+        sc.alpha = (nseg > 1) ? alpha : 0.0;
+        sc.ric_c = S.ric_c;
+        sc.ric_n = S.ric_n;
+
+        int sob = (on_start > fb) ? on_start : fb;
+        int soe = (on_end < fe) ? on_end : fe;
+        if (soe < sob) soe = sob;
+        const bool windowed = (soe - sob) < steps;
+        const int ls = sob - fb, le = ls + (soe - sob);
+        double tacc_d = prm.t_acc_direct_gyr;
+        {
+            const double tmid = S.elapsed + ((double)fb + 0.5 * (double)steps) * dt;
+            const bool direct_on = prm.dpp_model == 4 && isfinite(prm.t_acc_direct_gyr) &&
+                                   prm.t_acc_direct_gyr > 0.0 && isfinite(prm.t_off_gyr) && prm.t_off_gyr > 0.0 &&
+                                   isfinite(prm.t_acc_off_gyr) && prm.t_acc_off_gyr > 0.0;
+            if (direct_on && tmid < prm.t_off_gyr) tacc_d = prm.t_acc_off_gyr;
+        }
+        double inv_on = 0.0;
+        const bool mode_on = d_dpp_model(prm, prm.dpp_model, lturb, dv, cs, beta, tacc_d, &inv_on) != 0;
+        double scale = 1.0;
+        if (prm.dpp_model != 3 && prm.eta_dpp_cap > 0.0) {
+            double part = 0.0;
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                const int j = base + k;
+                const double wp_ = g.p_ecr_w[j];
+                const double we_ = g.e_ecr_w[j];
+                if (PROT && wp_ > 0.0 && isfinite(xp[k])) part += wp_ * xp[k] * g.pdp[j];
+                if (we_ > 0.0 && isfinite(xe[k])) part += we_ * xe[k] * g.edp[j];
+            }
+            const double ecr = g_sum<G>(part);
+            scale = d_eta_scale(prm, prm.dpp_model, lturb, dv, cs, beta, mass, tacc_d, ecr);
+        }
+        RunCoef rc;
+        rc.dt = dt;
+        rc.model = false;
+        rc.dfac = 0.0;
+        rc.inv_epmax = 1.0 / prm.epmax;
+        rc.dtT_fac = 0.0;
+        // D prefactors: on-window uses the model form times the eta scale (or
+        // the floor times the scale), outside the floor p^2/4e5
+        const double dfac_on = mode_on ? inv_on * scale : (1.0 / (4.0 * 1.0e+5)) * scale;
+        const double dfac_off = 1.0 / (4.0 * 1.0e+5);
+
+        if (PROT) {
+            // segment-start protons feed the electron secondaries
+            if (sec) {
+#pragma unroll
+                for (int k = 0; k < R; ++k) s_row[base + k] = xp[k];
+            }
+            RunCoef rp = rc;
+            rp.dtT_fac = dt * (0.5 * K_C * n_gas) * K_GYR;
+            species_segment<false, N, R, G>(gl, g, steps, windowed, ls, le, sc, rp, dfac_on, dfac_off, mode_on,
+                                            S.qpi ? S.qpi + toff : nullptr, dt, xp, bad);
+        }
+        if (sec) {
+            // inje = Gyr n_gas sum_p N_p fqe[p, e] + qepri (tracer_fp_cpu.c:142-147),
+            // stored scaled by dt into s_row for the electron runs
+            double acc[R];
+#pragma unroll
+            for (int k = 0; k < R; ++k) acc[k] = 0.0;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const double *wbase = S.fqe_t + base;
+#pragma unroll 4
+            for (int jp = S.jp0; jp < N; ++jp) {
+                const double nn = s_row[jp];
+                const double *wrow = wbase + (size_t)jp * N;
+#pragma unroll
+                for (int k = 0; k < R; ++k) acc[k] += nn * wrow[k];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                const double qe = S.qepri ? S.qepri[toff + base + k] : 0.0;
+                s_row[base + k] = dt * (acc[k] * K_GYR * n_gas + qe);
+            }
+            species_segment<true, N, R, G>(gl, g, steps, windowed, ls, le, sc, rc, dfac_on, dfac_off, mode_on, s_row,
+                                           1.0, xe, bad);
+        } else {
+            species_segment<true, N, R, G>(gl, g, steps, windowed, ls, le, sc, rc, dfac_on, dfac_off, mode_on,
+                                           S.qepri ? S.qepri + toff : nullptr, dt, xe, bad);
+        }
+    }
+}
+
 template <int N, int R, int WPB, bool PROT>
-__global__ __launch_bounds__(WPB * 64) void k_fused(CkGrid g, const CkSlot *__restrict__ slots,
-                                                    const CkTask *__restrict__ tasks, int ntasks,
-                                                    int *__restrict__ counter, int *__restrict__ err)
+__device__ __forceinline__ void k_fused_body(const CkGrid &g, const CkSlot *__restrict__ slots,
+                                             const CkTask *__restrict__ tasks, int ntasks,
+                                             int *__restrict__ counter, int *__restrict__ err)
 {
     constexpr int G = N / R;
     constexpr int GPW = 64 / G;
@@ -498,6 +860,7 @@ __global__ __launch_bounds__(WPB * 64) void k_fused(CkGrid g, const CkSlot *__re
     const int grp = lane / G;
     const int gl = lane % G;
     const int base = gl * R;
+    double *s_row = &s_crp[PROT ? wave : 0][PROT ? grp : 0][0];
     int bad = 0;
 
     for (;;) {
@@ -508,204 +871,47 @@ __global__ __launch_bounds__(WPB * 64) void k_fused(CkGrid g, const CkSlot *__re
         if (tbase >= ntasks) break;
         const int tid = tbase + grp;
         if (tid < ntasks) {
-        const CkTask task = tasks[tid];
-        const CkSlot &S = slots[task.slot];
-        const int t = task.tracer;
-        const CkParams &prm = S.prm;
-        const int nsub = S.nsub[t];
-        const int non = S.n_on[t];
-        const int nseg = d_choose_segments(nsub, S.allow_interp, prm.coeff_interp_min_steps,
-                                           prm.coeff_interp_max_segments);
-        const int on_start = (nsub - non) / 2;
-        const int on_end = on_start + non;
-        const double dt = S.dt_snap / (double)nsub;
-        const double base_dt = dt;
-        constexpr bool prot = PROT;
-        const size_t roff = (size_t)t * N + base;
-        const double mass = S.mass ? S.mass[t] : 0.0;
-        const bool dis = S.dis ? (S.dis[t] != 0) : false;
-
-        double xe[R], xp[R], qp[R], qe[R];
-#pragma unroll
-        for (int k = 0; k < R; ++k) {
-            xe[k] = S.cre[roff + k];
-            xp[k] = prot ? S.crp[roff + k] : 0.0;
-            qp[k] = (prot && S.qpi) ? S.qpi[roff + k] : 0.0;
-            qe[k] = S.qepri ? S.qepri[roff + k] : 0.0;
-        }
-        double bgc[CK_NBG], bgn[CK_NBG];
-#pragma unroll
-        for (int f = 0; f < CK_NBG; ++f) {
-            bgc[f] = S.bgc[f][t];
-            bgn[f] = S.bgn[f][t];
-        }
-
-        for (int iseg = 0; iseg < nseg; ++iseg) {
-            const int fb = (int)(((long long)iseg * nsub) / nseg);
-            const int fe = (int)(((long long)(iseg + 1) * nsub) / nseg);
-            const int steps = fe - fb;
-            if (steps <= 0) continue;
-            const double alpha = (nseg > 1) ? ((double)iseg + 0.5) / (double)nseg : 0.0;
-            double bg[CK_NBG];
-#pragma unroll
-            for (int f = 0; f < CK_NBG; ++f) bg[f] = (nseg > 1) ? d_lerp(bgc[f], bgn[f], alpha) : bgc[f];
-            const double n_gas = bg[CK_BG_NGAS], kbt = bg[CK_BG_KBT], bfield = bg[CK_BG_B];
-            const double divv = dis ? 0.0 : bg[CK_BG_DIVV];
-            const double lturb = bg[CK_BG_LTURB], dv = bg[CK_BG_DV], cs = bg[CK_BG_CS], beta = bg[CK_BG_BETA];
-            int sob = (on_start > fb) ? on_start : fb;
-            int soe = (on_end < fe) ? on_end : fe;
-            if (soe < sob) soe = sob;
-            const bool windowed = (soe - sob) < steps;
-            const int ls = sob - fb, le = ls + (soe - sob);
-            double tacc_d = prm.t_acc_direct_gyr;
-            {
-                const double tmid = S.elapsed + ((double)fb + 0.5 * (double)steps) * base_dt;
-                const bool direct_on = prm.dpp_model == 4 && isfinite(prm.t_acc_direct_gyr) &&
-                                       prm.t_acc_direct_gyr > 0.0 && isfinite(prm.t_off_gyr) &&
-                                       prm.t_off_gyr > 0.0 && isfinite(prm.t_acc_off_gyr) &&
-                                       prm.t_acc_off_gyr > 0.0;
-                if (direct_on && tmid < prm.t_off_gyr) tacc_d = prm.t_acc_off_gyr;
-            }
-
-            // --- losses (prepare_cre/crp_losses_1d) ---
-            double radpe[R], radpp[R], dtTp[R], dtTe[R];
-            {
-#pragma clang fp contract(off)
-                const double b_micro = bfield * 1.0e+6;
-                const double b_ratio = b_micro / 3.2;
-                const double synch_coeff = 4.8e-4 * b_ratio * b_ratio * 9.11e-28 * 3.0e+10 * 3.1536e+16;
-                const double coul_coeff = 3.05e-29 * n_gas * K_C * K_GYR / (K_ME * K_GEV);
-                const double adia = 1.0 / 3.0 * divv;
-#pragma unroll
-                for (int k = 0; k < R; ++k) {
-                    const int j = base + k;
-                    const double pe = g.ep[j];
-                    const double pe2 = g.e_p2[j];
-                    const double beta2 = pe2 / (1.0 + pe2);
-                    const double ric = (nseg > 1) ? d_lerp(S.ric_c[j], S.ric_n[j], alpha) : S.ric_c[j];
-                    const double coul = coul_coeff * (1.0 + log(pe / n_gas) / 74.8) / beta2;
-                    radpe[k] = synch_coeff * pe2 + ric + coul + adia * pe;
-                    dtTe[k] = dt / 1.0e+100;
-                }
-                if (prot) {
-                    const double tl_pref = 1.0 / (0.5 * K_C * n_gas) / K_GYR;
-#pragma unroll
-                    for (int k = 0; k < R; ++k) {
-                        const int j = base + k;
-                        const double p = g.pp[j];
-                        radpp[k] = d_b_coulomb_p(n_gas, p, kbt) + adia * p;
-                        dtTp[k] = dt / (tl_pref / g.p_sig[j]);
-                    }
-                }
-            }
-            // --- momentum diffusion: on-set prefactor and eta cap ---
-            double inv_on = 0.0;
-            const int mode_on = d_dpp_model(prm, prm.dpp_model, lturb, dv, cs, beta, tacc_d, &inv_on);
-            double scale = 1.0;
-            if (prm.dpp_model != 3 && prm.eta_dpp_cap > 0.0) {
-                double part = 0.0;
-#pragma unroll
-                for (int k = 0; k < R; ++k) {
-                    const int j = base + k;
-                    const double wp_ = g.p_ecr_w[j];
-                    const double we_ = g.e_ecr_w[j];
-                    if (prot && wp_ > 0.0 && isfinite(xp[k])) part += wp_ * xp[k] * g.pdp[j];
-                    if (we_ > 0.0 && isfinite(xe[k])) part += we_ * xe[k] * g.edp[j];
-                }
-                const double ecr = g_sum<G>(part);
-                scale = d_eta_scale(prm, prm.dpp_model, lturb, dv, cs, beta, mass, tacc_d, ecr);
-            }
-            // --- secondary electrons from the segment-start protons ---
-            double inje[R];
-            if (prot && S.fqe_t) {
-#pragma unroll
-                for (int k = 0; k < R; ++k) s_crp[wave][grp][base + k] = xp[k];
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                double acc[R];
-#pragma unroll
-                for (int k = 0; k < R; ++k) acc[k] = 0.0;
-                for (int jp = S.jp0; jp < N; ++jp) {
-                    const double nn = s_crp[wave][grp][jp];
-                    const double *wrow = S.fqe_t + (size_t)jp * N + base;
-#pragma unroll
-                    for (int k = 0; k < R; ++k) acc[k] += nn * wrow[k];
-                }
-                __builtin_amdgcn_wave_barrier();
-#pragma unroll
-                for (int k = 0; k < R; ++k) inje[k] = acc[k] * K_GYR * n_gas + qe[k];
-            } else {
-#pragma unroll
-                for (int k = 0; k < R; ++k) inje[k] = qe[k];
-            }
-            double srcp[R], srce[R];
+            const CkTask task = tasks[tid];
+            const CkSlot &S0 = slots[task.slot];
+            const int t = task.tracer;
+            const size_t roff = (size_t)t * N + base;
+            double xe[R], xp[R];
 #pragma unroll
             for (int k = 0; k < R; ++k) {
-                srcp[k] = dt * qp[k];
-                srce[k] = dt * inje[k];
+                xe[k] = S0.cre[roff + k];
+                xp[k] = PROT ? S0.crp[roff + k] : 0.0;
             }
-
-            // --- steps, in up to three runs of constant coefficient set ---
-            int s0 = 0;
-            while (s0 < steps) {
-                const bool on = !windowed || (s0 >= ls && s0 < le);
-                int s1;
-                if (!windowed) s1 = steps;
-                else if (s0 < ls) s1 = ls;
-                else if (s0 < le) s1 = le;
-                else s1 = steps;
-                // a segment wholly before the on-window has ls > steps
-                if (s1 > steps) s1 = steps;
-                const int use_model = on ? mode_on : 0;
-                // assemble + factorise electrons (and protons)
-                Factor<R, G> Fe, Fp;
-                {
-                    double Bv[R], Cv[R], a[R], b[R], c[R];
+            // a task advances one tracer over nslot consecutive snapshots
+            for (int sl = task.slot; sl < task.slot + task.nslot; ++sl) {
+                const CkSlot &S = slots[sl];
+                snapshot_update<N, R, PROT>(g, S, t, gl, s_row, xe, xp, bad);
+                if (S.out_e) {
 #pragma unroll
-                    for (int k = 0; k < R; ++k) {
-#pragma clang fp contract(off)
-                        const int j = base + k;
-                        double D;
-                        if (use_model) D = g.e_p2[j] * inv_on * g.e_expcut[j];
-                        else D = g.e_p2[j] * (1.0 / (4.0 * 1.0e+5));
-                        if (on) D *= scale;
-                        Cv[k] = D;
-                        Bv[k] = radpe[k] - 2.0 / g.ep[j] * D;
-                    }
-                    build_cc<R, G>(gl, N, dt, g.ep, g.edp, Bv, Cv, dtTe, a, b, c);
-                    factorize<R, G>(gl, N, a, b, c, Fe, bad);
-                    if (prot) {
+                    for (int k = 0; k < R; ++k) S.out_e[roff + k] = xe[k];
+                }
+                if (PROT && S.out_p) {
 #pragma unroll
-                        for (int k = 0; k < R; ++k) {
-#pragma clang fp contract(off)
-                            const int j = base + k;
-                            double D;
-                            if (use_model) D = g.p_p2[j] * inv_on * exp(-g.pE[j] / prm.epmax) * g.p_expcut[j];
-                            else D = g.p_p2[j] * (1.0 / (4.0 * 1.0e+5));
-                            if (on) D *= scale;
-                            Cv[k] = D;
-                            Bv[k] = radpp[k] - 2.0 / g.pp[j] * D;
-                        }
-                        build_cc<R, G>(gl, N, dt, g.pp, g.pdp, Bv, Cv, dtTp, a, b, c);
-                        factorize<R, G>(gl, N, a, b, c, Fp, bad);
-                    }
+                    for (int k = 0; k < R; ++k) S.out_p[roff + k] = xp[k];
                 }
-                for (int s = s0; s < s1; ++s) {
-                    if (prot) apply_solve<R, G>(gl, Fp, xp, srcp);
-                    apply_solve<R, G>(gl, Fe, xe, srce);
-                }
-                s0 = s1;
             }
-        }
 #pragma unroll
-        for (int k = 0; k < R; ++k) {
-            if (!isfinite(xe[k]) || !isfinite(xp[k])) bad |= ERR_NONFINITE;
-            S.cre[roff + k] = xe[k];
-            if (prot) S.crp[roff + k] = xp[k];
-        }
+            for (int k = 0; k < R; ++k) {
+                if (!isfinite(xe[k]) || !isfinite(xp[k])) bad |= ERR_NONFINITE;
+                S0.cre[roff + k] = xe[k];
+                if (PROT) S0.crp[roff + k] = xp[k];
+            }
         }
     }
     if (bad) atomicOr(err, bad);
 }
 
+// The kernel: WPB waves per block, register budget set for OCC waves per SIMD
+// (the solve is latency-bound, so the budget trades unrolled state against
+// waves available to hide cross-lane and memory latency).
+template <int N, int R, int WPB, bool PROT, int OCC>
+__global__ __launch_bounds__(WPB * 64) __attribute__((amdgpu_waves_per_eu(OCC, 8))) void k_fused(
+    CkGrid g, const CkSlot *__restrict__ slots, const CkTask *__restrict__ tasks, int ntasks,
+    int *__restrict__ counter, int *__restrict__ err)
+{
+    k_fused_body<N, R, WPB, PROT>(g, slots, tasks, ntasks, counter, err);
+}

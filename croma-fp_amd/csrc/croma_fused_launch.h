@@ -1,0 +1,32 @@
+// croma_fused_launch.h -- host-side dispatch of the fused kernel's template
+// instances for one grid size (included by croma_fused_n*.hip, one
+// translation unit per n so the instances compile in parallel).
+#pragma once
+#include "croma_device.h"
+
+#define CK_FUSED_WPB 4
+
+template <int N, int R, bool PROT, int OCC>
+static int ck_launch_inst(int nblocks, const CkGrid &grid, const CkSlot *slots, const CkTask *tasks, int ntasks,
+                          int *counter, int *err, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_fused<N, R, CK_FUSED_WPB, PROT, OCC>), dim3(nblocks), dim3(CK_FUSED_WPB * 64), 0, st,
+                       grid, slots, tasks, ntasks, counter, err);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <int N, int R>
+static int ck_launch_r(int occ, int protons, int nblocks, const CkGrid &grid, const CkSlot *slots,
+                       const CkTask *tasks, int ntasks, int *counter, int *err, hipStream_t st)
+{
+#define CK_OCC_CASE(O)                                                                                                 \
+    case O:                                                                                                            \
+        return protons ? ck_launch_inst<N, R, true, O>(nblocks, grid, slots, tasks, ntasks, counter, err, st)          \
+                       : ck_launch_inst<N, R, false, O>(nblocks, grid, slots, tasks, ntasks, counter, err, st);
+    switch (occ) {
+        CK_OCC_CASE(1)
+        CK_OCC_CASE(2)
+    default: return -1;
+    }
+#undef CK_OCC_CASE
+}

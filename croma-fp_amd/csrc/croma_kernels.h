@@ -44,6 +44,10 @@ typedef struct {
     const double *pp, *pdp, *pE, *p_p2, *p_expcut, *p_sig; /* p_sig: sigma_pp(j or j_pp) * sigmoid */
     const double *ep, *edp, *eE, *e_p2, *e_expcut;
     const double *p_ecr_w, *e_ecr_w; /* eta-cap energy weights (E-m) GeV dp on bins 2..n-3, else 0 */
+    /* derived tables of the Chang-Cooper build: face width h_j = x_{j+1} - x_j
+     * (0 for j = n-1), 1/h_j, 1/dx_j and 2/p_j */
+    const double *p_h, *p_inv_h, *p_inv_dx, *p_2op;
+    const double *e_h, *e_inv_h, *e_inv_dx, *e_2op;
     double ppm1, ppp1, epm1, epp1;
 } CkGrid;
 
@@ -68,13 +72,18 @@ typedef struct {
     int protons;                  /* proton species active (else crp == 0 exactly) */
     double dt_snap;
     double elapsed;
-    double ricm1_c, ricp1_c, ricm1_n, ricp1_n;
+    double *out_e;                /* optional post-snapshot electron state [ntr * n] */
+    double *out_p;                /* optional post-snapshot proton state [ntr * n] */
     CkParams prm;
 } CkSlot;
 
+/* One unit of the persistent queue: tracer `tracer` of the request whose
+ * first snapshot is slots[slot], advanced over nslot consecutive slots
+ * (snapshots) in one go.  State lives in slots[slot].crp / .cre. */
 typedef struct {
     int slot;
     int tracer;
+    int nslot;
 } CkTask;
 
 /* Runtime helpers (thin wrappers over the HIP runtime). */
@@ -110,15 +119,25 @@ void *ck_graph_end(void *stream);          /* returns executable graph or NULL *
 int ck_graph_launch(void *graph_exec, void *stream);
 int ck_graph_destroy(void *graph_exec);
 
+/* Inverse-Compton cooling rows on the device: for each of nz redshifts,
+ * zb[3 iz .. 3 iz + 2] = {z, e0min, e0max} (CMB bounds from the host),
+ * rows[iz * n + j] for the n electron bins and ghosts[2 iz], ghosts[2 iz + 1]
+ * for the ghost momenta epm1, epp1 (ghosts may be NULL).  err |= 4 when the
+ * CMB bounds span more nodes than the kernel holds. */
+int ck_ic_rows(int n, const double *ep_dev, double epm1, double epp1, int nz, const double *zb_dev,
+               double *rows_dev, double *ghosts_dev, int *err_dev, void *stream);
+
 /* Fill kernel (used instead of memset so it is safe under graph capture). */
 int ck_fill_i32(int *dst, int value, size_t count, void *stream);
 int ck_fill_f64(double *dst, double value, size_t count, void *stream);
 
 /* Fused path: persistent cost-ordered lane-group kernel.  rows_per_lane is
- * 4, 8 or 16; waves_per_block 1, 2 or 4.  protons selects the two-species
- * variant; every slot of one launch must agree.  The counter must be zero on
- * entry. */
-int ck_launch_fused(int n, int rows_per_lane, int waves_per_block, int nblocks, int protons,
+ * 2, 4, 8 or 16 (n / rows_per_lane lanes per tracer, 4..64); occupancy 1 or 2
+ * waves per SIMD selects the register budget (launch cu_count * occupancy
+ * blocks of 4 waves for one resident wave per SIMD and occupancy step).
+ * protons selects the two-species variant; every slot of one launch must
+ * agree.  The counter must be zero on entry. */
+int ck_launch_fused(int n, int rows_per_lane, int occupancy, int nblocks, int protons,
                     CkGrid grid, const CkSlot *slots_dev, const CkTask *tasks_dev, int ntasks,
                     int *counter_dev, int *err_dev, void *stream);
 int ck_fused_supported(int n, int rows_per_lane);

@@ -47,23 +47,23 @@ extern "C" int ck_launch_fused_n512(int, int, int, int, CkGrid, const CkSlot *, 
 extern "C" int ck_fused_supported(int n, int r)
 {
     if (!(n == 64 || n == 128 || n == 256 || n == 512)) return 0;
-    if (!(r == 4 || r == 8 || r == 16)) return 0;
-    if (n / r > 64) return 0;
+    if (!(r == 2 || r == 4 || r == 8 || r == 16)) return 0;
+    if (n / r > 64 || n / r < 4) return 0;
     return 1;
 }
 
-extern "C" int ck_launch_fused(int n, int r, int wpb, int nblocks, int protons, CkGrid grid, const CkSlot *slots,
+extern "C" int ck_launch_fused(int n, int r, int occ, int nblocks, int protons, CkGrid grid, const CkSlot *slots,
                                const CkTask *tasks, int ntasks, int *counter, int *err, void *stream)
 {
     int rc = -1;
     if (!ck_fused_supported(n, r) || ntasks < 0 || nblocks <= 0) return -1;
-    if (!(wpb == 1 || wpb == 2 || wpb == 4)) return -1;
+    if (occ < 1 || occ > 2) return -1;
     if (ntasks == 0) return 0;
     switch (n) {
-    case 64: rc = ck_launch_fused_n64(r, wpb, nblocks, protons, grid, slots, tasks, ntasks, counter, err, stream); break;
-    case 128: rc = ck_launch_fused_n128(r, wpb, nblocks, protons, grid, slots, tasks, ntasks, counter, err, stream); break;
-    case 256: rc = ck_launch_fused_n256(r, wpb, nblocks, protons, grid, slots, tasks, ntasks, counter, err, stream); break;
-    case 512: rc = ck_launch_fused_n512(r, wpb, nblocks, protons, grid, slots, tasks, ntasks, counter, err, stream); break;
+    case 64: rc = ck_launch_fused_n64(r, occ, nblocks, protons, grid, slots, tasks, ntasks, counter, err, stream); break;
+    case 128: rc = ck_launch_fused_n128(r, occ, nblocks, protons, grid, slots, tasks, ntasks, counter, err, stream); break;
+    case 256: rc = ck_launch_fused_n256(r, occ, nblocks, protons, grid, slots, tasks, ntasks, counter, err, stream); break;
+    case 512: rc = ck_launch_fused_n512(r, occ, nblocks, protons, grid, slots, tasks, ntasks, counter, err, stream); break;
     }
     if (rc) hip_ok(hipGetLastError(), "k_fused launch");
     return rc;

@@ -132,6 +132,7 @@ static int valid_params(const CromaHipParams *p)
 
 /* ------------------------------------------------------------------ */
 #define CH_IC_CACHE 8
+#define CH_IC_ENDS 256
 
 struct CromaHipContext {
     int device;
@@ -152,6 +153,12 @@ struct CromaHipContext {
         double row[CH_NMAX];
     } ic[CH_IC_CACHE];
     int ic_next;
+    /* end bins (0, n-1) of IC rows, for the nsub estimate */
+    struct {
+        int valid;
+        double z, v[2];
+    } ic_end[CH_IC_ENDS];
+    int ic_end_next;
     /* secondary weights, built on first use */
     int fqe_ready;
     int jp0;
@@ -164,50 +171,95 @@ struct CromaHipContext {
     int pws_cap;
 };
 
+/* Derived tables of the Chang-Cooper build for one species: face width
+ * h_j = x_{j+1} - x_j (0 at j = n-1), 1/h_j, 1/dx_j, 2/x_j. */
+static void derived_tables(int n, const double *x, const double *dx, double *h, double *inv_h, double *inv_dx,
+                           double *two_op)
+{
+    int j;
+    for (j = 0; j < n; j++) {
+        h[j] = (j + 1 < n) ? x[j + 1] - x[j] : 0.0;
+        inv_h[j] = (j + 1 < n) ? 1.0 / h[j] : 0.0;
+        inv_dx[j] = 1.0 / dx[j];
+        two_op[j] = 2.0 / x[j];
+    }
+}
+
+#define CH_NTAB 21
+
 static int ctx_upload_grid(CromaHipContext *c)
 {
     const int n = c->g.n;
     const size_t nb = (size_t)n * sizeof(double);
+    double *tab = (double *)malloc(CH_NTAB * nb);
     const double *src[13] = {c->g.pp, c->g.pdp, c->g.pE, c->g.p_p2, c->g.p_expcut, c->g.p_sig, c->g.ep,
                              c->g.edp, c->g.eE, c->g.e_p2, c->g.e_expcut, c->g.p_ecr_w, c->g.e_ecr_w};
+    const double *d;
     int k;
-    c->d_grid_mem = (double *)ck_malloc(13 * nb);
-    if (!c->d_grid_mem) return dev_fail("grid alloc");
-    for (k = 0; k < 13; k++)
-        if (ck_h2d(c->d_grid_mem + (size_t)k * n, src[k], nb, c->stream)) return dev_fail("grid upload");
+    if (!tab) return CROMA_HIP_ENOMEM;
+    for (k = 0; k < 13; k++) memcpy(tab + (size_t)k * n, src[k], nb);
+    derived_tables(n, c->g.pp, c->g.pdp, tab + 13 * (size_t)n, tab + 14 * (size_t)n, tab + 15 * (size_t)n,
+                   tab + 16 * (size_t)n);
+    derived_tables(n, c->g.ep, c->g.edp, tab + 17 * (size_t)n, tab + 18 * (size_t)n, tab + 19 * (size_t)n,
+                   tab + 20 * (size_t)n);
+    c->d_grid_mem = (double *)ck_malloc(CH_NTAB * nb);
+    if (!c->d_grid_mem) {
+        free(tab);
+        return dev_fail("grid alloc");
+    }
+    if (ck_h2d(c->d_grid_mem, tab, CH_NTAB * nb, c->stream) || ck_stream_sync(c->stream)) {
+        free(tab);
+        return dev_fail("grid upload");
+    }
+    free(tab);
+    d = c->d_grid_mem;
     c->dgrid.n = n;
     c->dgrid.j_pp = c->g.j_pp;
-    c->dgrid.pp = c->d_grid_mem + 0 * (size_t)n;
-    c->dgrid.pdp = c->d_grid_mem + 1 * (size_t)n;
-    c->dgrid.pE = c->d_grid_mem + 2 * (size_t)n;
-    c->dgrid.p_p2 = c->d_grid_mem + 3 * (size_t)n;
-    c->dgrid.p_expcut = c->d_grid_mem + 4 * (size_t)n;
-    c->dgrid.p_sig = c->d_grid_mem + 5 * (size_t)n;
-    c->dgrid.ep = c->d_grid_mem + 6 * (size_t)n;
-    c->dgrid.edp = c->d_grid_mem + 7 * (size_t)n;
-    c->dgrid.eE = c->d_grid_mem + 8 * (size_t)n;
-    c->dgrid.e_p2 = c->d_grid_mem + 9 * (size_t)n;
-    c->dgrid.e_expcut = c->d_grid_mem + 10 * (size_t)n;
-    c->dgrid.p_ecr_w = c->d_grid_mem + 11 * (size_t)n;
-    c->dgrid.e_ecr_w = c->d_grid_mem + 12 * (size_t)n;
+    c->dgrid.pp = d + 0 * (size_t)n;
+    c->dgrid.pdp = d + 1 * (size_t)n;
+    c->dgrid.pE = d + 2 * (size_t)n;
+    c->dgrid.p_p2 = d + 3 * (size_t)n;
+    c->dgrid.p_expcut = d + 4 * (size_t)n;
+    c->dgrid.p_sig = d + 5 * (size_t)n;
+    c->dgrid.ep = d + 6 * (size_t)n;
+    c->dgrid.edp = d + 7 * (size_t)n;
+    c->dgrid.eE = d + 8 * (size_t)n;
+    c->dgrid.e_p2 = d + 9 * (size_t)n;
+    c->dgrid.e_expcut = d + 10 * (size_t)n;
+    c->dgrid.p_ecr_w = d + 11 * (size_t)n;
+    c->dgrid.e_ecr_w = d + 12 * (size_t)n;
+    c->dgrid.p_h = d + 13 * (size_t)n;
+    c->dgrid.p_inv_h = d + 14 * (size_t)n;
+    c->dgrid.p_inv_dx = d + 15 * (size_t)n;
+    c->dgrid.p_2op = d + 16 * (size_t)n;
+    c->dgrid.e_h = d + 17 * (size_t)n;
+    c->dgrid.e_inv_h = d + 18 * (size_t)n;
+    c->dgrid.e_inv_dx = d + 19 * (size_t)n;
+    c->dgrid.e_2op = d + 20 * (size_t)n;
     c->dgrid.ppm1 = c->g.ppm1;
     c->dgrid.ppp1 = c->g.ppp1;
     c->dgrid.epm1 = c->g.epm1;
     c->dgrid.epp1 = c->g.epp1;
-    return ck_stream_sync(c->stream) ? dev_fail("grid sync") : 0;
+    return 0;
 }
 
 static void default_tiles(CromaHipContext *c)
 {
     const int n = c->g.n;
-    /* rows per lane: 8 keeps each tracer's spectrum and its factors in
-     * registers; for two species a smaller R halves the register set. */
-    c->tile[0].rows_per_lane = (n >= 128) ? 8 : 4;
-    c->tile[0].waves_per_block = 4;
-    c->tile[0].blocks_per_cu = 2;
-    c->tile[1].rows_per_lane = (n == 512) ? 8 : ((n >= 128) ? 8 : 4);
-    c->tile[1].waves_per_block = 4;
-    c->tile[1].blocks_per_cu = 1;
+    /* rows per lane: the spectrum and one factor set stay in registers;
+     * two waves per SIMD hide the cross-lane latency of the substep scans */
+    c->tile[0].rows_per_lane = (n >= 256) ? 8 : 4;
+    c->tile[0].waves_per_simd = 2;
+    c->tile[0].chain = 1;
+    c->tile[1].rows_per_lane = (n >= 256) ? 8 : 4;
+    c->tile[1].waves_per_simd = 2;
+    c->tile[1].chain = 1;
+}
+
+static int tile_ok(int n, const CromaHipTile *t)
+{
+    return ck_fused_supported(n, t->rows_per_lane) && t->waves_per_simd >= 1 && t->waves_per_simd <= 2 &&
+           t->chain >= 1 && t->chain <= 4096;
 }
 
 CromaHipContext *croma_hip_context_create(const CromaHipGridSpec *gs, const CromaHipParams *prm)
@@ -284,14 +336,12 @@ int croma_hip_context_set_params(CromaHipContext *c, const CromaHipParams *prm)
     return 0;
 }
 
-int croma_hip_context_set_tile(CromaHipContext *c, const CromaHipTile *t)
+int croma_hip_context_set_tile(CromaHipContext *c, int protons, const CromaHipTile *t)
 {
-    int k;
-    if (!c || !t) return CROMA_HIP_EINVAL;
-    if (!ck_fused_supported(c->g.n, t->rows_per_lane)) return CROMA_HIP_EUNSUPPORTED;
-    if (!(t->waves_per_block == 1 || t->waves_per_block == 2 || t->waves_per_block == 4)) return CROMA_HIP_EINVAL;
-    if (t->blocks_per_cu < 1 || t->blocks_per_cu > 64) return CROMA_HIP_EINVAL;
-    for (k = 0; k < 2; k++) c->tile[k] = *t;
+    if (!c || !t || protons < -1 || protons > 1) return CROMA_HIP_EINVAL;
+    if (!tile_ok(c->g.n, t)) return CROMA_HIP_EUNSUPPORTED;
+    if (protons != 1) c->tile[0] = *t;
+    if (protons != 0) c->tile[1] = *t;
     return 0;
 }
 
@@ -321,6 +371,68 @@ int croma_hip_context_grid(const CromaHipContext *c, int species, double *p, dou
         if (pp1) *pp1 = c->g.epp1;
     }
     return 0;
+}
+
+int croma_hip_ic_cooling_rows_device(CromaHipContext *c, int nz, const double *z, double *rows, double *ghosts)
+{
+    const int n = c ? c->g.n : 0;
+    double *h = 0, *d = 0;
+    int q, rc = 0, err = 0;
+    if (!c || nz <= 0 || !z || !rows) return CROMA_HIP_EINVAL;
+    h = (double *)malloc((size_t)nz * 3 * sizeof(double));
+    d = (double *)ck_malloc((size_t)nz * (3 + n + 2) * sizeof(double) + sizeof(int));
+    if (!h || !d) {
+        free(h);
+        ck_free(d);
+        return dev_fail("IC rows alloc");
+    }
+    for (q = 0; q < nz; q++) {
+        h[3 * q] = z[q];
+        ch_ic_bounds(z[q], &h[3 * q + 1], &h[3 * q + 2]);
+    }
+    {
+        double *d_zb = d, *d_rows = d + (size_t)nz * 3, *d_gh = d_rows + (size_t)nz * n;
+        int *d_err = (int *)(d_gh + (size_t)nz * 2);
+        if (ck_fill_i32(d_err, 0, 1, c->stream) || ck_h2d(d_zb, h, (size_t)nz * 3 * sizeof(double), c->stream) ||
+            ck_ic_rows(n, c->dgrid.ep, c->g.epm1, c->g.epp1, nz, d_zb, d_rows, d_gh, d_err, c->stream) ||
+            ck_d2h(rows, d_rows, (size_t)nz * n * sizeof(double), c->stream) ||
+            (ghosts && ck_d2h(ghosts, d_gh, (size_t)nz * 2 * sizeof(double), c->stream)) ||
+            ck_d2h(&err, d_err, sizeof(int), c->stream) || ck_stream_sync(c->stream))
+            rc = dev_fail("IC rows");
+    }
+    if (!rc && err) {
+        ch_set_error("IC cooling row: CMB bounds exceed the device quadrature size");
+        rc = CROMA_HIP_EUNSUPPORTED;
+    }
+    free(h);
+    ck_free(d);
+    return rc;
+}
+
+/* IC cooling rate at the first and last electron bins at z (cached). */
+static void ctx_ic_ends(CromaHipContext *c, double z, double ends[2])
+{
+    int k, bins[2];
+    pthread_mutex_lock(&c->ic_lock);
+    for (k = 0; k < CH_IC_ENDS; k++)
+        if (c->ic_end[k].valid && c->ic_end[k].z == z) {
+            ends[0] = c->ic_end[k].v[0];
+            ends[1] = c->ic_end[k].v[1];
+            pthread_mutex_unlock(&c->ic_lock);
+            return;
+        }
+    pthread_mutex_unlock(&c->ic_lock);
+    bins[0] = 0;
+    bins[1] = c->g.n - 1;
+    ch_ic_cooling_bins(&c->g, z, 2, bins, ends);
+    pthread_mutex_lock(&c->ic_lock);
+    k = c->ic_end_next;
+    c->ic_end_next = (k + 1) % CH_IC_ENDS;
+    c->ic_end[k].z = z;
+    c->ic_end[k].v[0] = ends[0];
+    c->ic_end[k].v[1] = ends[1];
+    c->ic_end[k].valid = 1;
+    pthread_mutex_unlock(&c->ic_lock);
 }
 
 int croma_hip_ic_cooling_row(const CromaHipContext *cc, double z, double *row, double *m1, double *p1)
@@ -398,35 +510,67 @@ int croma_hip_secondary_weights(const CromaHipContext *cc, double *fqe_t, int *n
 }
 
 /* ------------------------------------------------------------------ */
-/* task ordering: counting sort of tracers by descending cost          */
-static void order_tasks(const CkParams *prm, int allow, int slot, int ntr, const int *nsub, int protons,
-                        CkTask *out, int *hist_buf, int hist_n)
+/* task ordering: counting sort of tracers by descending cost, cost =
+ * sum over the task's snapshots of nsub x species + 3 x segments (the
+ * coefficient builds), bucketed on a log scale (1/256 octave). */
+#define CH_HIST 8192
+
+static int cost_bucket(double cost)
 {
-    int i;
+    int k;
+    if (cost < 1.0) return 0;
+    k = (int)(log2(cost) * 256.0) + 1;
+    return k >= CH_HIST ? CH_HIST - 1 : k;
+}
+
+static void order_tasks(const CkParams *prm, const int *allow, int nslot, int ntr, const int *nsub, int protons,
+                        CkTask *out, int *hist_buf, int *key_buf)
+{
+    int i, s;
     long pos = 0;
-    memset(hist_buf, 0, (size_t)hist_n * sizeof(int));
+    memset(hist_buf, 0, (size_t)CH_HIST * sizeof(int));
     for (i = 0; i < ntr; i++) {
-        long cost = (long)nsub[i] * (protons ? 2 : 1) + 3L * ch_choose_segments(prm, nsub[i], allow);
-        if (cost >= hist_n) cost = hist_n - 1;
-        hist_buf[cost]++;
+        double cost = 0.0;
+        for (s = 0; s < nslot; s++) {
+            const int ns = nsub[(size_t)s * ntr + i];
+            cost += (double)ns * (protons ? 2 : 1) + 3.0 * ch_choose_segments(prm, ns, allow[s]);
+        }
+        key_buf[i] = cost_bucket(cost);
+        hist_buf[key_buf[i]]++;
     }
-    for (i = hist_n - 1; i >= 0; i--) {
+    for (i = CH_HIST - 1; i >= 0; i--) {
         const int cnt = hist_buf[i];
         hist_buf[i] = (int)pos;
         pos += cnt;
     }
     for (i = 0; i < ntr; i++) {
-        long cost = (long)nsub[i] * (protons ? 2 : 1) + 3L * ch_choose_segments(prm, nsub[i], allow);
-        if (cost >= hist_n) cost = hist_n - 1;
-        out[hist_buf[cost]].slot = slot;
-        out[hist_buf[cost]].tracer = i;
-        hist_buf[cost]++;
+        CkTask *t = &out[hist_buf[key_buf[i]]++];
+        t->slot = 0;
+        t->tracer = i;
+        t->nslot = nslot;
     }
 }
 
-#define CH_HIST 65536
-
 /* ------------------------------------------------------------------ */
+/* A window is the unit of one fused launch: W consecutive snapshots, one
+ * queue task per tracer advancing it over all of them.  Two window buffer
+ * sets alternate so the host prepares window w+1 while the GPU runs w. */
+typedef struct {
+    int cap;          /* snapshots the buffers hold */
+    int *d_nsub, *d_non;
+    CkTask *d_tasks;
+    CkSlot *d_slot;
+    double *d_ric;    /* [cap + 1][n]: IC cooling rows at the window's redshifts (built on the device) */
+    double *d_zb;     /* [cap + 1][3]: z, CMB e0 bounds */
+    int *h_nsub, *h_non;     /* pinned [cap][ntr] */
+    CkTask *h_tasks;         /* pinned [ntr] */
+    CkSlot *h_slot;          /* pinned [cap] */
+    double *h_zb;            /* pinned [cap + 1][3] */
+    int count;               /* snapshots prepared */
+    int nz;                  /* IC rows to build */
+    void *ev_up;
+} WinBuf;
+
 struct CromaHipBatch {
     CromaHipContext *ctx;
     int ntr, species, nsnap_res, protons;
@@ -437,24 +581,59 @@ struct CromaHipBatch {
     double *h_bg;  /* host copy for nsub */
     double *d_src; /* optional [nsnap_res][2][ntr*n] per-snapshot sources */
     unsigned char *src_set;
-    /* double-buffered per-snapshot scheduling state */
-    int *d_nsub[2], *d_non[2];
-    CkTask *d_tasks[2];
-    CkSlot *d_slot[2];
-    double *d_ric[2]; /* [2][n]: z_curr, z_next rows */
-    int *h_nsub[2], *h_non[2]; /* pinned */
-    CkTask *h_tasks[2];        /* pinned */
-    CkSlot *h_slot[2];         /* pinned */
-    double *h_ric[2];          /* pinned [2][n] */
-    int *hist;
-    void *ev_up[2];
+    WinBuf win[2];
+    int *hist, *keys;
     void *ev_a, *ev_b;
-    void **ev_k; /* per-launch timing events, 2 per snapshot */
+    void **ev_k; /* per-launch timing events, 2 per window */
     int ev_k_cap;
     /* plain path */
     int *d_idx;
     int *h_idx;
 };
+
+static void win_free(WinBuf *w)
+{
+    ck_free(w->d_nsub);
+    ck_free(w->d_non);
+    ck_free(w->d_tasks);
+    ck_free(w->d_slot);
+    ck_free(w->d_ric);
+    ck_free(w->d_zb);
+    ck_host_free(w->h_nsub);
+    ck_host_free(w->h_non);
+    ck_host_free(w->h_tasks);
+    ck_host_free(w->h_slot);
+    ck_host_free(w->h_zb);
+    ck_event_destroy(w->ev_up);
+    memset(w, 0, sizeof(*w));
+}
+
+static int win_reserve(WinBuf *w, int cap, int ntr, int n, void *stream)
+{
+    if (w->cap >= cap) return 0;
+    if (w->ev_up) ck_event_sync(w->ev_up);
+    ck_stream_sync(stream);
+    win_free(w);
+    w->d_nsub = (int *)ck_malloc((size_t)cap * ntr * sizeof(int));
+    w->d_non = (int *)ck_malloc((size_t)cap * ntr * sizeof(int));
+    w->d_tasks = (CkTask *)ck_malloc((size_t)ntr * sizeof(CkTask));
+    w->d_slot = (CkSlot *)ck_malloc((size_t)cap * sizeof(CkSlot));
+    w->d_ric = (double *)ck_malloc((size_t)(cap + 1) * n * sizeof(double));
+    w->d_zb = (double *)ck_malloc((size_t)(cap + 1) * 3 * sizeof(double));
+    w->h_nsub = (int *)ck_host_alloc((size_t)cap * ntr * sizeof(int));
+    w->h_non = (int *)ck_host_alloc((size_t)cap * ntr * sizeof(int));
+    w->h_tasks = (CkTask *)ck_host_alloc((size_t)ntr * sizeof(CkTask));
+    w->h_slot = (CkSlot *)ck_host_alloc((size_t)cap * sizeof(CkSlot));
+    w->h_zb = (double *)ck_host_alloc((size_t)(cap + 1) * 3 * sizeof(double));
+    w->ev_up = ck_event_create();
+    if (!w->d_nsub || !w->d_non || !w->d_tasks || !w->d_slot || !w->d_ric || !w->d_zb || !w->h_nsub ||
+        !w->h_non || !w->h_tasks || !w->h_slot || !w->h_zb || !w->ev_up) {
+        win_free(w);
+        return dev_fail("window buffers");
+    }
+    w->cap = cap;
+    return 0;
+}
 
 void croma_hip_batch_destroy(CromaHipBatch *b)
 {
@@ -473,20 +652,9 @@ void croma_hip_batch_destroy(CromaHipBatch *b)
     free(b->h_bg);
     free(b->src_set);
     free(b->hist);
+    free(b->keys);
     ck_host_free(b->h_idx);
-    for (k = 0; k < 2; k++) {
-        ck_free(b->d_nsub[k]);
-        ck_free(b->d_non[k]);
-        ck_free(b->d_tasks[k]);
-        ck_free(b->d_slot[k]);
-        ck_free(b->d_ric[k]);
-        ck_host_free(b->h_nsub[k]);
-        ck_host_free(b->h_non[k]);
-        ck_host_free(b->h_tasks[k]);
-        ck_host_free(b->h_slot[k]);
-        ck_host_free(b->h_ric[k]);
-        ck_event_destroy(b->ev_up[k]);
-    }
+    for (k = 0; k < 2; k++) win_free(&b->win[k]);
     ck_event_destroy(b->ev_a);
     ck_event_destroy(b->ev_b);
     for (k = 0; k < b->ev_k_cap; k++) ck_event_destroy(b->ev_k[k]);
@@ -517,28 +685,15 @@ CromaHipBatch *croma_hip_batch_create(CromaHipContext *c, int ntr, int species, 
     b->d_bg = (double *)ck_malloc((size_t)nsnap_res * CK_NBG * ntr * sizeof(double));
     b->h_bg = (double *)malloc((size_t)nsnap_res * CK_NBG * ntr * sizeof(double));
     b->hist = (int *)malloc(CH_HIST * sizeof(int));
+    b->keys = (int *)malloc((size_t)ntr * sizeof(int));
     b->src_set = (unsigned char *)calloc((size_t)nsnap_res, 1);
     b->ev_a = ck_event_create();
     b->ev_b = ck_event_create();
-    if (!b->d_cre || (b->protons && !b->d_crp) || !b->d_bg || !b->h_bg || !b->hist || !b->src_set || !b->ev_a ||
-        !b->ev_b)
+    if (!b->d_cre || (b->protons && !b->d_crp) || !b->d_bg || !b->h_bg || !b->hist || !b->keys || !b->src_set ||
+        !b->ev_a || !b->ev_b)
         goto fail;
-    for (k = 0; k < 2; k++) {
-        b->d_nsub[k] = (int *)ck_malloc((size_t)ntr * sizeof(int));
-        b->d_non[k] = (int *)ck_malloc((size_t)ntr * sizeof(int));
-        b->d_tasks[k] = (CkTask *)ck_malloc((size_t)ntr * sizeof(CkTask));
-        b->d_slot[k] = (CkSlot *)ck_malloc(sizeof(CkSlot));
-        b->d_ric[k] = (double *)ck_malloc(2 * (size_t)n * sizeof(double));
-        b->h_nsub[k] = (int *)ck_host_alloc((size_t)ntr * sizeof(int));
-        b->h_non[k] = (int *)ck_host_alloc((size_t)ntr * sizeof(int));
-        b->h_tasks[k] = (CkTask *)ck_host_alloc((size_t)ntr * sizeof(CkTask));
-        b->h_slot[k] = (CkSlot *)ck_host_alloc(sizeof(CkSlot));
-        b->h_ric[k] = (double *)ck_host_alloc(2 * (size_t)n * sizeof(double));
-        b->ev_up[k] = ck_event_create();
-        if (!b->d_nsub[k] || !b->d_non[k] || !b->d_tasks[k] || !b->d_slot[k] || !b->d_ric[k] || !b->h_nsub[k] ||
-            !b->h_non[k] || !b->h_tasks[k] || !b->h_slot[k] || !b->h_ric[k] || !b->ev_up[k])
-            goto fail;
-    }
+    for (k = 0; k < 2; k++)
+        if (win_reserve(&b->win[k], 1, ntr, n, c->stream)) goto fail;
     if (b->protons && ctx_fqe(c) != 0) goto fail;
     return b;
 fail:
@@ -663,108 +818,122 @@ int croma_hip_batch_set_sources(CromaHipBatch *b, int isnap, const double *qpi, 
     return ck_stream_sync(b->ctx->stream) ? dev_fail("source sync") : 0;
 }
 
-/* Host-side preparation of snapshot s into buffer set k. */
+/* Host-side preparation of snapshots [s, s + count) into window w. */
 typedef struct {
     double ic_ms, nsub_ms, order_ms;
     long long cell_steps, species_steps;
 } PrepStats;
 
-static int prepare_snapshot(CromaHipBatch *b, int k, int s, int nsnap_total, const double *z, const double *dt_snap,
-                            double elapsed, PrepStats *ps)
+static int prepare_window(CromaHipBatch *b, WinBuf *w, int s_first, int count, int nsnap_total, const double *z,
+                          const double *dt_snap, double elapsed, PrepStats *ps)
 {
     CromaHipContext *c = b->ctx;
     const int n = c->g.n, ntr = b->ntr;
-    const int allow = (s + 1 < nsnap_total) ? 1 : 0;
-    const int sn = (s + 1 < b->nsnap_res && allow) ? s + 1 : s;
-    const double *hb = b->h_bg + (size_t)s * CK_NBG * ntr;
-    CkSlot *S = b->h_slot[k];
-    double m1, p1, m1n, p1n, t0;
-    int f, i;
+    int allow[64], *allow_v = allow;
+    double t0;
+    int q, i, f;
     long long cs = 0;
 
-    if (allow && s + 1 >= b->nsnap_res) {
-        ch_set_error("snapshot %d needs background slot %d (resident %d)", s, s + 1, b->nsnap_res);
-        return CROMA_HIP_EINVAL;
+    if (count > 64) {
+        allow_v = (int *)malloc((size_t)count * sizeof(int));
+        if (!allow_v) return CROMA_HIP_ENOMEM;
+    }
+    /* IC rows at z_first .. z_last (+ z_last+1 when the last snapshot has a
+     * successor) are built on the device; the host only evaluates the two
+     * end bins nsub needs (same arithmetic as the reference row). */
+    w->nz = count + ((s_first + count < nsnap_total) ? 1 : 0);
+    for (q = 0; q < w->nz; q++) {
+        w->h_zb[3 * q] = z[s_first + q];
+        ch_ic_bounds(z[s_first + q], &w->h_zb[3 * q + 1], &w->h_zb[3 * q + 2]);
+    }
+    for (q = 0; q < count; q++) {
+        const int s = s_first + q;
+        const int al = (s + 1 < nsnap_total) ? 1 : 0;
+        const int sn = (s + 1 < b->nsnap_res && al) ? s + 1 : s;
+        const double *hb = b->h_bg + (size_t)s * CK_NBG * ntr;
+        int *nsub = w->h_nsub + (size_t)q * ntr, *non = w->h_non + (size_t)q * ntr;
+        CkSlot *S = &w->h_slot[q];
+        double ends[2];
+        if (al && s + 1 >= b->nsnap_res) {
+            ch_set_error("snapshot %d needs background slot %d (resident %d)", s, s + 1, b->nsnap_res);
+            if (allow_v != allow) free(allow_v);
+            return CROMA_HIP_EINVAL;
+        }
+        allow_v[q] = al;
+        t0 = ch_now_s();
+        ctx_ic_ends(c, z[s], ends);
+        ps->ic_ms += 1e3 * (ch_now_s() - t0);
+        t0 = ch_now_s();
+        ch_nsub_estimate(&c->g, &c->prm, ntr, dt_snap[s], ends[0], ends[1], hb + 0 * (size_t)ntr,
+                         hb + 1 * (size_t)ntr, hb + 2 * (size_t)ntr, hb + 3 * (size_t)ntr, hb + 4 * (size_t)ntr,
+                         hb + 5 * (size_t)ntr, nsub, non);
+        ps->nsub_ms += 1e3 * (ch_now_s() - t0);
+        for (i = 0; i < ntr; i++) cs += nsub[i];
+
+        memset(S, 0, sizeof(*S));
+        S->crp = b->d_crp;
+        S->cre = b->d_cre;
+        if (b->d_src && b->src_set[s]) {
+            const size_t m = (size_t)ntr * n;
+            S->qpi = b->protons ? b->d_src + (size_t)s * 2 * m : 0;
+            S->qepri = b->d_src + (size_t)s * 2 * m + m;
+        } else {
+            S->qpi = (b->protons && b->has_qpi) ? b->d_qpi : 0;
+            S->qepri = b->has_qepri ? b->d_qepri : 0;
+        }
+        S->mass = b->has_mass ? b->d_mass : 0;
+        S->dis = b->has_dis ? b->d_dis : 0;
+        for (f = 0; f < CK_NBG; f++) {
+            S->bgc[f] = b->d_bg + ((size_t)s * CK_NBG + f) * ntr;
+            S->bgn[f] = b->d_bg + ((size_t)sn * CK_NBG + f) * ntr;
+        }
+        S->ric_c = w->d_ric + (size_t)q * n;
+        S->ric_n = al ? w->d_ric + (size_t)(q + 1) * n : S->ric_c;
+        S->nsub = w->d_nsub + (size_t)q * ntr;
+        S->n_on = w->d_non + (size_t)q * ntr;
+        S->fqe_t = b->protons ? c->d_fqe : 0;
+        S->jp0 = c->jp0;
+        S->ntr = ntr;
+        S->allow_interp = al;
+        S->protons = b->protons;
+        S->dt_snap = dt_snap[s];
+        S->elapsed = elapsed;
+        S->prm = c->prm;
+        elapsed += dt_snap[s];
     }
     t0 = ch_now_s();
-    croma_hip_ic_cooling_row(c, z[s], b->h_ric[k], &m1, &p1);
-    if (allow) croma_hip_ic_cooling_row(c, z[s + 1], b->h_ric[k] + n, &m1n, &p1n);
-    else {
-        memcpy(b->h_ric[k] + n, b->h_ric[k], (size_t)n * sizeof(double));
-        m1n = m1;
-        p1n = p1;
-    }
-    ps->ic_ms += 1e3 * (ch_now_s() - t0);
-    t0 = ch_now_s();
-    ch_nsub_estimate(&c->g, &c->prm, ntr, dt_snap[s], b->h_ric[k][0], b->h_ric[k][n - 1], hb + 0 * (size_t)ntr,
-                     hb + 1 * (size_t)ntr, hb + 2 * (size_t)ntr, hb + 3 * (size_t)ntr, hb + 4 * (size_t)ntr,
-                     hb + 5 * (size_t)ntr, b->h_nsub[k], b->h_non[k]);
-    ps->nsub_ms += 1e3 * (ch_now_s() - t0);
-    t0 = ch_now_s();
-    order_tasks(&c->prm, allow, 0, ntr, b->h_nsub[k], b->protons, b->h_tasks[k], b->hist, CH_HIST);
-    for (i = 0; i < ntr; i++) cs += b->h_nsub[k][i];
+    order_tasks(&c->prm, allow_v, count, ntr, w->h_nsub, b->protons, w->h_tasks, b->hist, b->keys);
+    ps->order_ms += 1e3 * (ch_now_s() - t0);
     ps->cell_steps += cs;
     ps->species_steps += cs * (b->protons ? 2 : 1);
-    ps->order_ms += 1e3 * (ch_now_s() - t0);
-
-    memset(S, 0, sizeof(*S));
-    S->crp = b->d_crp;
-    S->cre = b->d_cre;
-    if (b->d_src && b->src_set[s]) {
-        const size_t m = (size_t)ntr * n;
-        S->qpi = b->protons ? b->d_src + (size_t)s * 2 * m : 0;
-        S->qepri = b->d_src + (size_t)s * 2 * m + m;
-    } else {
-        S->qpi = (b->protons && b->has_qpi) ? b->d_qpi : 0;
-        S->qepri = b->has_qepri ? b->d_qepri : 0;
-    }
-    S->mass = b->has_mass ? b->d_mass : 0;
-    S->dis = b->has_dis ? b->d_dis : 0;
-    for (f = 0; f < CK_NBG; f++) {
-        S->bgc[f] = b->d_bg + ((size_t)s * CK_NBG + f) * ntr;
-        S->bgn[f] = b->d_bg + ((size_t)sn * CK_NBG + f) * ntr;
-    }
-    S->ric_c = b->d_ric[k];
-    S->ric_n = b->d_ric[k] + n;
-    S->nsub = b->d_nsub[k];
-    S->n_on = b->d_non[k];
-    S->fqe_t = b->protons ? c->d_fqe : 0;
-    S->jp0 = c->jp0;
-    S->ntr = ntr;
-    S->allow_interp = allow;
-    S->protons = b->protons;
-    S->dt_snap = dt_snap[s];
-    S->elapsed = elapsed;
-    S->ricm1_c = m1;
-    S->ricp1_c = p1;
-    S->ricm1_n = m1n;
-    S->ricp1_n = p1n;
-    S->prm = c->prm;
+    w->count = count;
+    if (allow_v != allow) free(allow_v);
     return 0;
 }
 
-static int upload_snapshot(CromaHipBatch *b, int k)
+static int upload_window(CromaHipBatch *b, WinBuf *w)
 {
     CromaHipContext *c = b->ctx;
-    const int ntr = b->ntr, n = c->g.n;
-    if (ck_h2d(b->d_nsub[k], b->h_nsub[k], (size_t)ntr * sizeof(int), c->stream) ||
-        ck_h2d(b->d_non[k], b->h_non[k], (size_t)ntr * sizeof(int), c->stream) ||
-        ck_h2d(b->d_tasks[k], b->h_tasks[k], (size_t)ntr * sizeof(CkTask), c->stream) ||
-        ck_h2d(b->d_slot[k], b->h_slot[k], sizeof(CkSlot), c->stream) ||
-        ck_h2d(b->d_ric[k], b->h_ric[k], 2 * (size_t)n * sizeof(double), c->stream) ||
-        ck_event_record(b->ev_up[k], c->stream))
-        return dev_fail("snapshot upload");
+    const int ntr = b->ntr, n = c->g.n, q = w->count;
+    if (ck_h2d(w->d_nsub, w->h_nsub, (size_t)q * ntr * sizeof(int), c->stream) ||
+        ck_h2d(w->d_non, w->h_non, (size_t)q * ntr * sizeof(int), c->stream) ||
+        ck_h2d(w->d_tasks, w->h_tasks, (size_t)ntr * sizeof(CkTask), c->stream) ||
+        ck_h2d(w->d_slot, w->h_slot, (size_t)q * sizeof(CkSlot), c->stream) ||
+        ck_h2d(w->d_zb, w->h_zb, (size_t)w->nz * 3 * sizeof(double), c->stream) ||
+        ck_event_record(w->ev_up, c->stream))
+        return dev_fail("window upload");
+    if (ck_ic_rows(n, c->dgrid.ep, c->g.epm1, c->g.epp1, w->nz, w->d_zb, w->d_ric, 0, c->d_err, c->stream))
+        return dev_fail("IC rows");
     return 0;
 }
 
-static int launch_fused(CromaHipBatch *b, int k)
+static int launch_tasks(CromaHipContext *c, int protons, const CkSlot *d_slot, const CkTask *d_tasks, int ntasks)
 {
-    CromaHipContext *c = b->ctx;
-    const CromaHipTile *t = &c->tile[b->protons ? 1 : 0];
-    const int nblocks = c->cu_count * t->blocks_per_cu;
+    const CromaHipTile *t = &c->tile[protons ? 1 : 0];
+    const int nblocks = c->cu_count * t->waves_per_simd;
     if (ck_fill_i32(c->d_counter, 0, 1, c->stream)) return dev_fail("counter reset");
-    if (ck_launch_fused(c->g.n, t->rows_per_lane, t->waves_per_block, nblocks, b->protons, c->dgrid, b->d_slot[k],
-                        b->d_tasks[k], b->ntr, c->d_counter, c->d_err, c->stream))
+    if (ck_launch_fused(c->g.n, t->rows_per_lane, t->waves_per_simd, nblocks, protons, c->dgrid, d_slot, d_tasks,
+                        ntasks, c->d_counter, c->d_err, c->stream))
         return dev_fail("fused launch");
     return 0;
 }
@@ -859,12 +1028,12 @@ static int radix_sort_pairs(unsigned long long *key, int *idx, long n)
  * coefficient segments with separate launches: background lerp, the
  * coefficient kernel, the secondary source and the thread-per-tracer Thomas
  * substeps (fp_cuda_tracer_backend.cu:335-830). */
-static int run_plain_snapshot(CromaHipBatch *b, int k)
+static int run_plain_snapshot(CromaHipBatch *b, WinBuf *w)
 {
     CromaHipContext *c = b->ctx;
-    const CkSlot *S = b->h_slot[k];
+    const CkSlot *S = &w->h_slot[0];
     const int n = c->g.n, ntr = b->ntr;
-    const int *nsub = b->h_nsub[k], *non = b->h_non[k];
+    const int *nsub = w->h_nsub, *non = w->h_non;
     unsigned long long *key;
     long i0;
     int maxb = 0;
@@ -962,6 +1131,10 @@ int croma_hip_batch_sync(CromaHipBatch *b)
             ch_set_error("Thomas pivot below 1e-300");
             return CROMA_HIP_EPIVOT;
         }
+        if (err & 4) {
+            ch_set_error("IC cooling row: CMB bounds exceed the device quadrature size");
+            return CROMA_HIP_EUNSUPPORTED;
+        }
         ch_set_error("non-finite spectrum");
         return CROMA_HIP_ENONFINITE;
     }
@@ -974,7 +1147,7 @@ int croma_hip_batch_run(CromaHipBatch *b, int s0, int s1, int nsnap_total, const
     CromaHipContext *c;
     PrepStats ps;
     double t_start, el = elapsed;
-    int s, rc = 0, k = 0;
+    int s, rc = 0, k = 0, wsz, nwin, iw;
     float dev_ms = 0.0f;
     double kern_ms = 0.0;
     if (!b || !z || !dt_snap || s0 < 0 || s1 < s0 || s1 > b->nsnap_res || nsnap_total < s1 ||
@@ -982,8 +1155,13 @@ int croma_hip_batch_run(CromaHipBatch *b, int s0, int s1, int nsnap_total, const
         return CROMA_HIP_EINVAL;
     if (s1 == s0) return 0;
     c = b->ctx;
-    if (path == CROMA_HIP_PATH_FUSED && b->ev_k_cap < 2 * (s1 - s0)) {
-        int need = 2 * (s1 - s0), j;
+    wsz = (path == CROMA_HIP_PATH_FUSED) ? c->tile[b->protons ? 1 : 0].chain : 1;
+    if (wsz > s1 - s0) wsz = s1 - s0;
+    nwin = (s1 - s0 + wsz - 1) / wsz;
+    for (k = 0; k < 2; k++)
+        if ((rc = win_reserve(&b->win[k], wsz, b->ntr, c->g.n, c->stream)) != 0) return rc;
+    if (path == CROMA_HIP_PATH_FUSED && b->ev_k_cap < 2 * nwin) {
+        int need = 2 * nwin, j;
         void **ne = (void **)realloc(b->ev_k, (size_t)need * sizeof(void *));
         if (!ne) return CROMA_HIP_ENOMEM;
         b->ev_k = ne;
@@ -999,24 +1177,27 @@ int croma_hip_batch_run(CromaHipBatch *b, int s0, int s1, int nsnap_total, const
     memset(&ps, 0, sizeof(ps));
     t_start = ch_now_s();
     if (ck_event_record(b->ev_a, c->stream)) return dev_fail("event");
-    rc = prepare_snapshot(b, 0, s0, nsnap_total, z, dt_snap, el, &ps);
+    rc = prepare_window(b, &b->win[0], s0, (s1 - s0 < wsz) ? s1 - s0 : wsz, nsnap_total, z, dt_snap, el, &ps);
     if (rc) return rc;
-    for (s = s0; s < s1; s++) {
-        k = (s - s0) & 1;
-        if ((rc = upload_snapshot(b, k)) != 0) return rc;
+    for (iw = 0, s = s0; s < s1; iw++) {
+        WinBuf *w = &b->win[iw & 1];
+        const int cnt = w->count;
+        if ((rc = upload_window(b, w)) != 0) return rc;
         if (path == CROMA_HIP_PATH_FUSED) {
-            if (ck_event_record(b->ev_k[2 * (s - s0)], c->stream)) return dev_fail("event");
-            rc = launch_fused(b, k);
-            if (!rc && ck_event_record(b->ev_k[2 * (s - s0) + 1], c->stream)) return dev_fail("event");
+            if (ck_event_record(b->ev_k[2 * iw], c->stream)) return dev_fail("event");
+            rc = launch_tasks(c, b->protons, w->d_slot, w->d_tasks, b->ntr);
+            if (!rc && ck_event_record(b->ev_k[2 * iw + 1], c->stream)) return dev_fail("event");
         } else {
-            rc = run_plain_snapshot(b, k);
+            rc = run_plain_snapshot(b, w);
         }
         if (rc) return rc;
-        el += dt_snap[s];
-        if (s + 1 < s1) {
-            /* host buffers of set k^1 were consumed by the upload two snapshots ago */
-            if (ck_event_sync(b->ev_up[k ^ 1])) return dev_fail("upload event");
-            rc = prepare_snapshot(b, k ^ 1, s + 1, nsnap_total, z, dt_snap, el, &ps);
+        for (k = 0; k < cnt; k++) el += dt_snap[s + k];
+        s += cnt;
+        if (s < s1) {
+            /* the other window's host buffers were consumed by its upload */
+            WinBuf *o = &b->win[(iw + 1) & 1];
+            if (ck_event_sync(o->ev_up)) return dev_fail("upload event");
+            rc = prepare_window(b, o, s, (s1 - s < wsz) ? s1 - s : wsz, nsnap_total, z, dt_snap, el, &ps);
             if (rc) return rc;
         }
     }
@@ -1024,10 +1205,10 @@ int croma_hip_batch_run(CromaHipBatch *b, int s0, int s1, int nsnap_total, const
     rc = croma_hip_batch_sync(b);
     dev_ms = ck_event_elapsed_ms(b->ev_a, b->ev_b);
     if (path == CROMA_HIP_PATH_FUSED)
-        for (s = s0; s < s1; s++) kern_ms += ck_event_elapsed_ms(b->ev_k[2 * (s - s0)], b->ev_k[2 * (s - s0) + 1]);
+        for (k = 0; k < nwin; k++) kern_ms += ck_event_elapsed_ms(b->ev_k[2 * k], b->ev_k[2 * k + 1]);
     if (times) {
         times->kernel_ms += kern_ms;
-        if (path == CROMA_HIP_PATH_FUSED) times->kernel_launches += s1 - s0;
+        if (path == CROMA_HIP_PATH_FUSED) times->kernel_launches += nwin;
         times->ic_rows_ms += ps.ic_ms;
         times->nsub_estimate_ms += ps.nsub_ms;
         times->bucket_build_ms += ps.order_ms;
@@ -1066,6 +1247,7 @@ int croma_hip_execute_snapshot(
     CkSlot *d_slot = 0;
     double *d_ric = 0, *d_fqe = 0;
     CkParams prm;
+    double ric2[2 * CH_NMAX]; /* staging of the caller's rows; read by the async upload before the final sync */
     (void)z_curr;
     (void)z_next;
     if (!c || ntr <= 0 || !nsubsteps || !n_onsteps || !base_nsteps || !fp_cadence_steps || !rad_ic_zero ||
@@ -1123,13 +1305,14 @@ int croma_hip_execute_snapshot(
         d_tasks = (CkTask *)p;
         p += (((size_t)ntr * sizeof(CkTask)) + 255) & ~(size_t)255;
         d_slot = (CkSlot *)p;
-        p += 512;
+        p += (sizeof(CkSlot) + 255) & ~(size_t)255;
         d_ric = (double *)p;
         p += ((2 * (size_t)n * sizeof(double)) + 255) & ~(size_t)255;
         d_fqe = (double *)p;
     }
     {
-        double *ric2 = b->h_ric[0];
+        WinBuf *w = &b->win[0];
+        int allow1 = allow;
         int jp0 = n;
         for (j = 0; j < n; j++) ric2[j] = rad_ic_zero[(size_t)j * ntr];
         memcpy(ric2 + n, row_next, (size_t)n * sizeof(double));
@@ -1137,12 +1320,12 @@ int croma_hip_execute_snapshot(
             int v = np_min_qe[j] < 0 ? 0 : np_min_qe[j];
             if (v < jp0) jp0 = v;
         }
-        memcpy(b->h_nsub[0], nsubsteps, (size_t)ntr * sizeof(int));
-        memcpy(b->h_non[0], n_onsteps, (size_t)ntr * sizeof(int));
+        memcpy(w->h_nsub, nsubsteps, (size_t)ntr * sizeof(int));
+        memcpy(w->h_non, n_onsteps, (size_t)ntr * sizeof(int));
         ch_params_to_ck(&c->uprm, &prm);
         prm.t_acc_direct_gyr = t_acc_direct;
-        order_tasks(&prm, allow, 0, ntr, nsubsteps, protons, b->h_tasks[0], b->hist, CH_HIST);
-        S = b->h_slot[0];
+        order_tasks(&prm, &allow1, 1, ntr, nsubsteps, protons, w->h_tasks, b->hist, b->keys);
+        S = w->h_slot;
         memset(S, 0, sizeof(*S));
         S->crp = b->d_crp;
         S->cre = b->d_cre;
@@ -1165,28 +1348,19 @@ int croma_hip_execute_snapshot(
         S->protons = protons;
         S->dt_snap = dt_snap;
         S->elapsed = elapsed;
-        S->ricm1_c = rad_ic_m1_zero[0];
-        S->ricp1_c = rad_ic_p1_zero[0];
-        S->ricm1_n = m1n;
-        S->ricp1_n = p1n;
         S->prm = prm;
-        if (ck_h2d(d_nsub, b->h_nsub[0], (size_t)ntr * sizeof(int), c->stream) ||
-            ck_h2d(d_non, b->h_non[0], (size_t)ntr * sizeof(int), c->stream) ||
-            ck_h2d(d_tasks, b->h_tasks[0], (size_t)ntr * sizeof(CkTask), c->stream) ||
+        if (ck_h2d(d_nsub, w->h_nsub, (size_t)ntr * sizeof(int), c->stream) ||
+            ck_h2d(d_non, w->h_non, (size_t)ntr * sizeof(int), c->stream) ||
+            ck_h2d(d_tasks, w->h_tasks, (size_t)ntr * sizeof(CkTask), c->stream) ||
             ck_h2d(d_slot, S, sizeof(CkSlot), c->stream) ||
             ck_h2d(d_ric, ric2, 2 * (size_t)n * sizeof(double), c->stream) ||
             (protons && ck_h2d(d_fqe, fqe_flat, (size_t)n * n * sizeof(double), c->stream))) {
             rc = dev_fail("execute upload");
             goto done;
         }
-        {
-            const CromaHipTile *t = &c->tile[protons ? 1 : 0];
-            if (ck_fill_i32(c->d_counter, 0, 1, c->stream) ||
-                ck_launch_fused(n, t->rows_per_lane, t->waves_per_block, c->cu_count * t->blocks_per_cu, protons,
-                                c->dgrid, d_slot, d_tasks, ntr, c->d_counter, c->d_err, c->stream)) {
-                rc = dev_fail("execute launch");
-                goto done;
-            }
+        if (launch_tasks(c, protons, d_slot, d_tasks, ntr)) {
+            rc = dev_fail("execute launch");
+            goto done;
         }
     }
     rc = croma_hip_batch_sync(b);

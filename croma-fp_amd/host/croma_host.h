@@ -35,6 +35,11 @@ int ch_nsub_estimate(const ChGrid *g, const CkParams *prm, long ntr, double dt_s
                      const double *divv, const double *l_turb, const double *dv, int *nsub, int *n_on);
 int ch_choose_segments(const CkParams *prm, int nsub, int allow);
 int ch_ic_cooling_row(const ChGrid *g, double z, double *row, double *m1, double *p1);
+/* CMB photon-energy bounds of the IC integral at z (FP_Coef.c:413-436). */
+void ch_ic_bounds(double z, double *e0min, double *e0max);
+/* IC cooling rate of the electron bins bins[0..count) at z (same arithmetic
+ * as ch_ic_cooling_row, which fills every bin). */
+int ch_ic_cooling_bins(const ChGrid *g, double z, int count, const int *bins, double *out);
 int ch_secondary_weights(const ChGrid *g, double *fqe_t, int *np_min_qe, int *jp0_out);
 double ch_gamma_kernel(double eg, double p);
 double ch_nu_kernel(double e, double p);

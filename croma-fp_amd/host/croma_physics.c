@@ -486,6 +486,41 @@ static void ch_ic_range(void *arg, long i0, long i1)
     }
 }
 
+void ch_ic_bounds(double z, double *e0min, double *e0max) { ch_cmb_bounds(z, e0min, e0max); }
+
+typedef struct {
+    const ChGrid *g;
+    double z, lo, hi;
+    const int *bins;
+    double *out;
+} ChIcBinsJob;
+
+static void ch_ic_bins_range(void *arg, long i0, long i1)
+{
+    ChIcBinsJob *J = (ChIcBinsJob *)arg;
+    long i;
+    for (i = i0; i < i1; i++) {
+        const double p = J->g->ep[J->bins[i]];
+        J->out[i] = ch_b_ic(sqrt(1.0 + p * p), J->lo, J->hi, J->z);
+    }
+}
+
+int ch_ic_cooling_bins(const ChGrid *g, double z, int count, const int *bins, double *out)
+{
+    ChIcBinsJob J;
+    int i;
+    if (!g || count < 0 || !bins || !out) return -1;
+    for (i = 0; i < count; i++)
+        if (bins[i] < 0 || bins[i] >= g->n) return -1;
+    J.g = g;
+    J.z = z;
+    J.bins = bins;
+    J.out = out;
+    ch_cmb_bounds(z, &J.lo, &J.hi);
+    ch_parallel_for(count, 1, ch_ic_bins_range, &J);
+    return 0;
+}
+
 int ch_ic_cooling_row(const ChGrid *g, double z, double *row, double *m1, double *p1)
 {
     ChIcJob J;

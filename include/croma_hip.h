@@ -85,11 +85,11 @@ void croma_hip_default_params(CromaHipParams *prm);
 #define CROMA_HIP_PATH_FUSED 0   /* persistent lane-group kernel (product) */
 #define CROMA_HIP_PATH_PLAIN 1   /* one tracer per thread, per-bucket launches */
 
-/* Tile shape of the fused kernel; 0 fields mean "autotuned / default". */
+/* Launch shape of the fused kernel. */
 typedef struct {
-    int rows_per_lane;    /* 4, 8 or 16: rows of the spectrum per lane */
-    int waves_per_block;  /* 1, 2 or 4 */
-    int blocks_per_cu;    /* persistent blocks per compute unit */
+    int rows_per_lane;    /* 2, 4, 8 or 16 spectrum rows per lane: n / rows_per_lane lanes (4..64) per tracer */
+    int waves_per_simd;   /* 1 or 2: register budget of the kernel and resident persistent waves per SIMD */
+    int chain;            /* batch runs: snapshots advanced per queue task (1 = one launch per snapshot) */
 } CromaHipTile;
 
 /* Stage timers, the subset of TracerFpGpuTimes (include/tracer_fp_types.h:10-57)
@@ -115,7 +115,8 @@ typedef struct CromaHipContext CromaHipContext;
 CromaHipContext *croma_hip_context_create(const CromaHipGridSpec *grid, const CromaHipParams *prm);
 void croma_hip_context_destroy(CromaHipContext *ctx);
 int croma_hip_context_set_params(CromaHipContext *ctx, const CromaHipParams *prm);
-int croma_hip_context_set_tile(CromaHipContext *ctx, const CromaHipTile *tile);
+/* protons: 0 electron_only tile, 1 electron_proton tile, -1 both. */
+int croma_hip_context_set_tile(CromaHipContext *ctx, int protons, const CromaHipTile *tile);
 int croma_hip_context_get_tile(const CromaHipContext *ctx, int protons, CromaHipTile *tile);
 /* Grid arrays (momentum_bin, FP_Coef.c:291-330); species 0 proton, 1 electron. */
 int croma_hip_context_grid(const CromaHipContext *ctx, int species, double *p, double *dp, double *E,
@@ -160,6 +161,9 @@ int croma_hip_nsub_estimate(const CromaHipContext *ctx, int ntracer, double dt_s
                             const double *l_turb, const double *dv, int *nsubsteps, int *n_onsteps);
 /* IC cooling row at redshift z (prepare_ic_cooling_row, fp_shared_core.c:844-864). */
 int croma_hip_ic_cooling_row(const CromaHipContext *ctx, double z, double *row, double *m1, double *p1);
+/* The same rows built on the device (the batch path's rows): rows [nz][n],
+ * ghosts [nz][2] (bins at epm1, epp1; may be NULL). */
+int croma_hip_ic_cooling_rows_device(CromaHipContext *ctx, int nz, const double *z, double *rows, double *ghosts);
 /* Secondary-electron weights, transposed layout (prepare_secondary_kernel_flat,
  * fp_shared_core.c:1453-1506). */
 int croma_hip_secondary_weights(const CromaHipContext *ctx, double *fqe_t, int *np_min_qe);
