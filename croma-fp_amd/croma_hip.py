@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(HERE, "libcroma_hip.so")
 OK, EINVAL, EPIVOT, ENOMEM, EDEVICE, ENONFINITE, EBUDGET, ECANCELED, EUNSUPPORTED, EBUSY = (
     0, -1, -2, -3, -4, -5, -6, -7, -8, -9)
 PATH_FUSED, PATH_PLAIN = 0, 1
-SPECIES_ELECTRON, SPECIES_ELECTRON_PROTON = 1, 3
+SPECIES_ELECTRON, SPECIES_PROTON, SPECIES_ELECTRON_PROTON = 1, 2, 3
 DPP_ASA, DPP_TTD, DPP_OFF, DPP_DIRECT = 1, 2, 3, 4
 EMIS_FP32, EMIS_MFMA_XF32, EMIS_MFMA_BF16, EMIS_FP64 = 0, 1, 2, 3
 
@@ -215,6 +215,35 @@ class Context:
              "ic_cooling_rows_device")
         return rows, gh
 
+    def emission_ic(self, cre, fic, nbins, mode=EMIS_MFMA_XF32):
+        """eps[i][k] = sum_j fic[k][j] cre[i][j] dp[j]; returns (eps [ntr, nbins], device ms)."""
+        cre = f64(cre)
+        nt = cre.size // self.n
+        eps = np.zeros(nt * nbins)
+        ms = C.c_double()
+        _chk(lib().croma_hip_emission_ic(C.c_void_p(self.h), nt, int(nbins), _p(cre), _p(f64(fic)), _p(eps),
+                                         int(mode), C.byref(ms)), "emission_ic")
+        return eps.reshape(nt, nbins), ms.value
+
+    def emission_hadronic(self, crp, n_gas, fk, nbins, emin, emax, mode=EMIS_MFMA_XF32):
+        crp = f64(crp)
+        nt = len(n_gas)
+        eps = np.zeros(nt * nbins)
+        ms = C.c_double()
+        _chk(lib().croma_hip_emission_hadronic(C.c_void_p(self.h), nt, int(nbins), C.c_double(emin),
+                                               C.c_double(emax), _p(f64(n_gas)), _p(crp), _p(f64(fk)), _p(eps),
+                                               int(mode), C.byref(ms)), "emission_hadronic")
+        return eps.reshape(nt, nbins), ms.value
+
+    def emission_synch(self, cre, b_field, table, nfreq, nlogb, logb_min, inv_dlogb):
+        nt = len(b_field)
+        eps = np.zeros(nt * nfreq)
+        ms = C.c_double()
+        _chk(lib().croma_hip_emission_synch(C.c_void_p(self.h), nt, int(nfreq), int(nlogb), C.c_double(logb_min),
+                                            C.c_double(inv_dlogb), _p(f64(table)), _p(f64(cre)),
+                                            _p(f64(b_field)), _p(eps), C.byref(ms)), "emission_synch")
+        return eps.reshape(nt, nfreq), ms.value
+
     def nsub(self, dt_snap, n_gas, kbt, b_field, divv, ric_lo, ric_hi, l_turb, dv):
         nt = len(n_gas)
         ns = np.zeros(nt, np.int32)
@@ -318,3 +347,82 @@ def solve_cc_batch(a, b, c, source, state, dt):
     _chk(lib().croma_hip_solve_cc_batch(nsys, nrow, C.c_double(dt), _p(f64(a)), _p(f64(b)), _p(f64(c)),
                                         _p(f64(source)), _p(f64(state)), _p(out), C.byref(ms)), "solve_cc_batch")
     return out, ms.value
+
+
+class Server:
+    """Request loop (croma_hip_serve_*): chunks of tracers are submitted at any
+    time, admitted against an HBM budget and advanced together on the device."""
+
+    def __init__(self, budget_bytes, params=None, pmin=-1.0, pmax=8.0, pemin=0.0, pemax=6.0):
+        L = lib()
+        L.croma_hip_serve_create.restype = C.c_void_p
+        L.croma_hip_serve_create.argtypes = [C.POINTER(Params), C.c_size_t, C.c_double, C.c_double, C.c_double,
+                                             C.c_double]
+        L.croma_hip_serve_destroy.argtypes = [C.c_void_p]
+        L.croma_hip_serve_destroy.restype = None
+        L.croma_hip_serve_submit.argtypes = [C.c_void_p, C.POINTER(Chunk), C.POINTER(C.c_longlong)]
+        L.croma_hip_serve_poll.argtypes = [C.c_void_p, C.c_longlong]
+        L.croma_hip_serve_wait.argtypes = [C.c_void_p, C.c_longlong, _dp, _dp, C.POINTER(ChunkStats)]
+        L.croma_hip_serve_cancel.argtypes = [C.c_void_p, C.c_longlong]
+        L.croma_hip_serve_stats.argtypes = [C.c_void_p, C.POINTER(ServeStats)]
+        L.croma_hip_chunk_bytes.restype = C.c_size_t
+        L.croma_hip_chunk_bytes.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
+        self.params = params if params is not None else default_params()
+        self.h = L.croma_hip_serve_create(C.byref(self.params), int(budget_bytes), pmin, pmax, pemin, pemax)
+        if not self.h:
+            raise CromaHipError(-1, "serve_create")
+        self.shapes = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().croma_hip_serve_destroy(C.c_void_p(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, n, species, dpp_model, z, dt_snap, bg, cre=None, crp=None, qpi=None, qepri=None, mass=None):
+        """bg: [(nsnap + 1), 8, ntracer]; spectra [ntracer * n]. Returns the request id."""
+        z, dt_snap, bg = f64(z), f64(dt_snap), f64(bg)
+        nsnap = len(dt_snap)
+        ntr = bg.shape[2]
+        assert bg.shape == (nsnap + 1, 8, ntr) and len(z) == nsnap + 1
+        arrs = [None if a is None else f64(a) for a in (crp, cre, qpi, qepri, mass)]
+        ch = Chunk(ntr, n, species, dpp_model, nsnap, _p(z), _p(dt_snap), _p(bg), *[_p(a) for a in arrs])
+        rid = C.c_longlong()
+        _chk(lib().croma_hip_serve_submit(C.c_void_p(self.h), C.byref(ch), C.byref(rid)), "serve_submit")
+        self.shapes[rid.value] = (ntr, n)
+        return rid.value
+
+    def poll(self, rid):
+        return lib().croma_hip_serve_poll(C.c_void_p(self.h), C.c_longlong(rid))
+
+    def wait(self, rid):
+        """Blocks until the request is done; returns (crp, cre, stats dict)."""
+        ntr, n = self.shapes[rid]
+        crp = np.zeros(ntr * n)
+        cre = np.zeros(ntr * n)
+        st = ChunkStats()
+        rc = lib().croma_hip_serve_wait(C.c_void_p(self.h), C.c_longlong(rid), _p(crp), _p(cre), C.byref(st))
+        stats = {"submit_s": st.submit_s, "admit_s": st.admit_s, "done_s": st.done_s, "cell_steps": st.cell_steps,
+                 "status": st.status, "latency_s": st.done_s - st.submit_s}
+        _chk(rc, "serve_wait")
+        return crp, cre, stats
+
+    def cancel(self, rid):
+        _chk(lib().croma_hip_serve_cancel(C.c_void_p(self.h), C.c_longlong(rid)), "serve_cancel")
+
+    def stats(self):
+        st = ServeStats()
+        _chk(lib().croma_hip_serve_stats(C.c_void_p(self.h), C.byref(st)), "serve_stats")
+        return {k: getattr(st, k) for k, _ in ServeStats._fields_}
+
+
+def chunk_bytes(ntracer, n, species, nsnap):
+    L = lib()
+    L.croma_hip_chunk_bytes.restype = C.c_size_t
+    L.croma_hip_chunk_bytes.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
+    return int(L.croma_hip_chunk_bytes(ntracer, n, species, nsnap))

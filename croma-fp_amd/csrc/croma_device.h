@@ -361,8 +361,7 @@ struct Factor {
 // group's lanes (renormalised by powers of two, so exact up to rounding of
 // the products); inside the lane the reference recurrence is applied.
 template <int R, int G>
-__device__ __forceinline__ void factorize(int gl, int n, const double *src, double src_scale, Factor<R, G> &F,
-                                          int &bad)
+__device__ __forceinline__ void factorize(int gl, int n, const double (&src)[R], Factor<R, G> &F, int &bad)
 {
     double m00 = 1.0, m01 = 0.0, m10 = 0.0, m11 = 1.0;
 #pragma unroll
@@ -405,7 +404,7 @@ __device__ __forceinline__ void factorize(int gl, int n, const double *src, doub
         const double cp = (base + k == n - 1) ? 0.0 : -c * inv;
         F.iv[k] = inv;
         F.al[k] = a * inv;
-        F.vs[k] = src ? inv * (src_scale * src[base + k]) : 0.0;
+        F.vs[k] = inv * src[k];
         F.g[k] = -cp;
         A *= F.al[k];
         Bq *= F.g[k];
@@ -576,16 +575,6 @@ __device__ __forceinline__ void d_cc_row2(int kind, double t1, double C, double 
     }
 }
 
-// Segment-level loss inputs (backgrounds lerped to the segment midpoint).
-struct SegCoef {
-    double n_gas, kbt;
-    double adia;         // (div v) / 3
-    double synch_coeff;  // electrons: b_synch prefactor at this B
-    double coul_coeff;   // electrons: Coulomb prefactor at this n_gas
-    double alpha;        // lerp weight of the IC rows (0: current row only)
-    const double *ric_c, *ric_n;
-};
-
 // Run-level coefficient inputs: D(p) = p^2 * Dshape(p) * dfac (+ eta scale),
 // B = radp - (2/p) D, C = D; dt/T from dtT_fac * sig (protons).
 struct RunCoef {
@@ -596,32 +585,11 @@ struct RunCoef {
     double dtT_fac;  // dt * 0.5 c n_gas Gyr (protons), 0 for electrons
 };
 
-// Loss rate of row j (prepare_cre_losses_1d, fp_shared_core.c:780-842;
-// prepare_crp_losses_1d_with_threshold :729-778), recomputed per run so no
-// per-row loss array has to stay live.
 template <bool ELEC>
-__device__ __forceinline__ double row_radp(const CkGrid &g, const SegCoef &sc, int j)
+__device__ __forceinline__ void row_BC(const CkGrid &g, const SpGrid &sg, const RunCoef &rc, int j, double radp,
+                                       double &B, double &C)
 {
 #pragma clang fp contract(off)
-    if (ELEC) {
-        const double pe = g.ep[j];
-        const double pe2 = g.e_p2[j];
-        const double beta2 = pe2 / (1.0 + pe2);
-        const double ric = (sc.alpha > 0.0) ? d_lerp(sc.ric_c[j], sc.ric_n[j], sc.alpha) : sc.ric_c[j];
-        const double coul = sc.coul_coeff * (1.0 + log(pe / sc.n_gas) / 74.8) / beta2;
-        return sc.synch_coeff * pe2 + ric + coul + sc.adia * pe;
-    } else {
-        const double p = g.pp[j];
-        return d_b_coulomb_p(sc.n_gas, p, sc.kbt) + sc.adia * p;
-    }
-}
-
-template <bool ELEC>
-__device__ __forceinline__ void row_BC(const CkGrid &g, const SpGrid &sg, const SegCoef &sc, const RunCoef &rc,
-                                       int j, double &B, double &C)
-{
-#pragma clang fp contract(off)
-    const double radp = row_radp<ELEC>(g, sc, j);
     double D;
     if (!rc.model) D = sg.p2[j] * rc.dfac;
     else if (ELEC) D = sg.p2[j] * rc.dfac * sg.expcut[j];
@@ -631,18 +599,17 @@ __device__ __forceinline__ void row_BC(const CkGrid &g, const SpGrid &sg, const 
 }
 
 // Builds the rows of the lane into F (al = a, iv = b, g = c) and factorises.
-// Row k's (B, C) is computed once and carried to the next face.  src[j] *
-// src_scale is the row's source term (src NULL: none).
 template <bool ELEC, int R, int G>
-__device__ __forceinline__ void build_factor(int gl, int n, const CkGrid &g, const SegCoef &sc, const RunCoef &rc,
-                                             const double *src, double src_scale, Factor<R, G> &F, int &bad)
+__device__ __forceinline__ void build_factor(int gl, int n, const CkGrid &g, const RunCoef &rc,
+                                             const double (&radp)[R], const double (&src)[R], Factor<R, G> &F,
+                                             int &bad)
 {
 #pragma clang fp contract(off)
     const SpGrid sg = sp_grid(g, ELEC);
     const int base = gl * R;
     double B0, C0, BL, CL;
-    row_BC<ELEC>(g, sg, sc, rc, base, B0, C0);
-    row_BC<ELEC>(g, sg, sc, rc, base + R - 1, BL, CL);
+    row_BC<ELEC>(g, sg, rc, base, radp[0], B0, C0);
+    row_BC<ELEC>(g, sg, rc, base + R - 1, radp[R - 1], BL, CL);
     const double Bn = g_down<G>(B0, 1), Cn = g_down<G>(C0, 1);
     const double Bp = g_up<G>(BL, 1), Cp = g_up<G>(CL, 1);
     Face fl = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -656,7 +623,7 @@ __device__ __forceinline__ void build_factor(int gl, int n, const CkGrid &g, con
         double Br, Cr;
         if (k == R - 1) { Br = Bn; Cr = Cn; }
         else if (k == R - 2) { Br = BL; Cr = CL; }
-        else row_BC<ELEC>(g, sg, sc, rc, j + 1, Br, Cr);
+        else row_BC<ELEC>(g, sg, rc, j + 1, radp[k + 1], Br, Cr);
         Face fr = {0.0, 0.0, 0.0, 0.0, 0.0};
         if (!last) fr = d_face2(Bk, Br, Ck, Cr, sg.h[j], sg.inv_h[j]);
         const int kind = (j == 0) ? 1 : (last ? 2 : 0);
@@ -671,23 +638,56 @@ __device__ __forceinline__ void build_factor(int gl, int n, const CkGrid &g, con
         Cl = Ck;
         Bk = Br;
         Ck = Cr;
-        // keep the rows' transcendental chains from being interleaved (each
-        // row's temporaries die before the next row starts)
-        __builtin_amdgcn_sched_barrier(0);
     }
-    factorize<R, G>(gl, n, src, src_scale, F, bad);
+    factorize<R, G>(gl, n, src, F, bad);
 }
 
 // ---------------------------------------------------------------------------
 // Fused persistent kernel
 // ---------------------------------------------------------------------------
+struct SegScalars {
+    double n_gas, kbt, bfield, divv, lturb, dv, cs, beta;
+};
+
+// Loss rates of the lane's rows for one segment (prepare_cre_losses_1d,
+// fp_shared_core.c:780-842; prepare_crp_losses_1d_with_threshold :729-778).
+template <bool ELEC, int R>
+__device__ __forceinline__ void seg_radp(const CkGrid &g, const CkSlot &S, int base, int nseg, double alpha,
+                                         const SegScalars &q, double (&radp)[R])
+{
+#pragma clang fp contract(off)
+    if (ELEC) {
+        const double b_micro = q.bfield * 1.0e+6;
+        const double b_ratio = b_micro / 3.2;
+        const double synch_coeff = 4.8e-4 * b_ratio * b_ratio * 9.11e-28 * 3.0e+10 * 3.1536e+16;
+        const double coul_coeff = 3.05e-29 * q.n_gas * K_C * K_GYR / (K_ME * K_GEV);
+        const double adia = 1.0 / 3.0 * q.divv;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const int j = base + k;
+            const double pe = g.ep[j];
+            const double pe2 = g.e_p2[j];
+            const double beta2 = pe2 / (1.0 + pe2);
+            const double ric = (nseg > 1) ? d_lerp(S.ric_c[j], S.ric_n[j], alpha) : S.ric_c[j];
+            const double coul = coul_coeff * (1.0 + log(pe / q.n_gas) / 74.8) / beta2;
+            radp[k] = synch_coeff * pe2 + ric + coul + adia * pe;
+        }
+    } else {
+        const double adia = 1.0 / 3.0 * q.divv;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const double p = g.pp[base + k];
+            radp[k] = d_b_coulomb_p(q.n_gas, p, q.kbt) + adia * p;
+        }
+    }
+}
 
 // All runs of one species over one segment: up to three runs of constant
 // coefficients (before / inside / after the on-window, tracer_fp_cpu.c:150-157).
 template <bool ELEC, int N, int R, int G>
 __device__ __forceinline__ void species_segment(int gl, const CkGrid &g, int steps, bool windowed, int ls, int le,
-                                                const SegCoef &sc, RunCoef rc, double dfac_on, double dfac_off,
-                                                bool mode_on, const double *src, double src_scale, double (&x)[R],
+                                                RunCoef rc, double dfac_on, double dfac_off, bool mode_on,
+                                                const double (&radp)[R], const double (&src)[R], double (&x)[R],
                                                 int &bad)
 {
     int s0 = 0;
@@ -703,7 +703,7 @@ __device__ __forceinline__ void species_segment(int gl, const CkGrid &g, int ste
         rc.model = on && mode_on;
         rc.dfac = on ? dfac_on : dfac_off;
         Factor<R, G> F;
-        build_factor<ELEC, R, G>(gl, N, g, sc, rc, src, src_scale, F, bad);
+        build_factor<ELEC, R, G>(gl, N, g, rc, radp, src, F, bad);
         for (int s = s0; s < s1; ++s) apply_solve<R, G>(gl, F, x);
         s0 = s1;
     }
@@ -713,14 +713,11 @@ __device__ __forceinline__ void species_segment(int gl, const CkGrid &g, int ste
 // tracer_fp_batch.c:651-827 -> tracer_fp_cpu_evolve, tracer_fp_cpu.c:88-196).
 // Within a segment the proton and electron updates are independent (the
 // electron source uses the segment-start protons), so the species run one
-// after the other and only one factor set is live.  s_row (LDS, n doubles
-// per tracer) holds the segment-start protons during the proton runs and
-// the electron source of the segment afterwards.
+// after the other and only one factor set is live.
 template <int N, int R, bool PROT>
 __device__ __forceinline__ void snapshot_update(const CkGrid &g, const CkSlot &S, int t, int gl, double *s_row,
                                                 double (&xe)[R], double (&xp)[R], int &bad)
 {
-#pragma clang fp contract(off)
     constexpr int G = N / R;
     const int base = gl * R;
     const CkParams &prm = S.prm;
@@ -731,7 +728,7 @@ __device__ __forceinline__ void snapshot_update(const CkGrid &g, const CkSlot &S
     const int on_start = (nsub - non) / 2;
     const int on_end = on_start + non;
     const double dt = S.dt_snap / (double)nsub;
-    const size_t toff = (size_t)t * N;
+    const size_t roff = (size_t)t * N + base;
     const double mass = S.mass ? S.mass[t] : 0.0;
     const bool dis = S.dis ? (S.dis[t] != 0) : false;
     const bool sec = PROT && S.fqe_t;
@@ -742,26 +739,21 @@ __device__ __forceinline__ void snapshot_update(const CkGrid &g, const CkSlot &S
         const int steps = fe - fb;
         if (steps <= 0) continue;
         const double alpha = (nseg > 1) ? ((double)iseg + 0.5) / (double)nseg : 0.0;
-        double v[CK_NBG];
-#pragma unroll
-        for (int f = 0; f < CK_NBG; ++f) v[f] = (nseg > 1) ? d_lerp(S.bgc[f][t], S.bgn[f][t], alpha) : S.bgc[f][t];
-        const double n_gas = v[CK_BG_NGAS], lturb = v[CK_BG_LTURB], dv = v[CK_BG_DV], cs = v[CK_BG_CS];
-        const double beta = v[CK_BG_BETA];
-        const double divv = dis ? 0.0 : v[CK_BG_DIVV];
-        SegCoef sc;
-        sc.n_gas = n_gas;
-        sc.kbt = v[CK_BG_KBT];
-        sc.adia = 1.0 / 3.0 * divv;
+        SegScalars q;
         {
-            const double b_micro = v[CK_BG_B] * 1.0e+6;
-            const double b_ratio = b_micro / 3.2;
-            sc.synch_coeff = 4.8e-4 * b_ratio * b_ratio * 9.11e-28 * 3.0e+10 * 3.1536e+16;
-            sc.coul_coeff = 3.05e-29 * n_gas * K_C * K_GYR / (K_ME * K_GEV);
+            double v[CK_NBG];
+#pragma unroll
+            for (int f = 0; f < CK_NBG; ++f)
+                v[f] = (nseg > 1) ? d_lerp(S.bgc[f][t], S.bgn[f][t], alpha) : S.bgc[f][t];
+            q.n_gas = v[CK_BG_NGAS];
+            q.kbt = v[CK_BG_KBT];
+            q.bfield = v[CK_BG_B];
+            q.divv = dis ? 0.0 : v[CK_BG_DIVV];
+            q.lturb = v[CK_BG_LTURB];
+            q.dv = v[CK_BG_DV];
+            q.cs = v[CK_BG_CS];
+            q.beta = v[CK_BG_BETA];
         }
-        sc.alpha = (nseg > 1) ? alpha : 0.0;
-        sc.ric_c = S.ric_c;
-        sc.ric_n = S.ric_n;
-
         int sob = (on_start > fb) ? on_start : fb;
         int soe = (on_end < fe) ? on_end : fe;
         if (soe < sob) soe = sob;
@@ -776,7 +768,7 @@ __device__ __forceinline__ void snapshot_update(const CkGrid &g, const CkSlot &S
             if (direct_on && tmid < prm.t_off_gyr) tacc_d = prm.t_acc_off_gyr;
         }
         double inv_on = 0.0;
-        const bool mode_on = d_dpp_model(prm, prm.dpp_model, lturb, dv, cs, beta, tacc_d, &inv_on) != 0;
+        const bool mode_on = d_dpp_model(prm, prm.dpp_model, q.lturb, q.dv, q.cs, q.beta, tacc_d, &inv_on) != 0;
         double scale = 1.0;
         if (prm.dpp_model != 3 && prm.eta_dpp_cap > 0.0) {
             double part = 0.0;
@@ -789,7 +781,7 @@ __device__ __forceinline__ void snapshot_update(const CkGrid &g, const CkSlot &S
                 if (we_ > 0.0 && isfinite(xe[k])) part += we_ * xe[k] * g.edp[j];
             }
             const double ecr = g_sum<G>(part);
-            scale = d_eta_scale(prm, prm.dpp_model, lturb, dv, cs, beta, mass, tacc_d, ecr);
+            scale = d_eta_scale(prm, prm.dpp_model, q.lturb, q.dv, q.cs, q.beta, mass, tacc_d, ecr);
         }
         RunCoef rc;
         rc.dt = dt;
@@ -808,41 +800,43 @@ __device__ __forceinline__ void snapshot_update(const CkGrid &g, const CkSlot &S
 #pragma unroll
                 for (int k = 0; k < R; ++k) s_row[base + k] = xp[k];
             }
+            double radp[R], src[R];
+            seg_radp<false, R>(g, S, base, nseg, alpha, q, radp);
+#pragma unroll
+            for (int k = 0; k < R; ++k) src[k] = S.qpi ? dt * S.qpi[roff + k] : 0.0;
             RunCoef rp = rc;
-            rp.dtT_fac = dt * (0.5 * K_C * n_gas) * K_GYR;
-            species_segment<false, N, R, G>(gl, g, steps, windowed, ls, le, sc, rp, dfac_on, dfac_off, mode_on,
-                                            S.qpi ? S.qpi + toff : nullptr, dt, xp, bad);
+            rp.dtT_fac = dt * (0.5 * K_C * q.n_gas) * K_GYR;
+            species_segment<false, N, R, G>(gl, g, steps, windowed, ls, le, rp, dfac_on, dfac_off, mode_on, radp, src,
+                                            xp, bad);
         }
-        if (sec) {
-            // inje = Gyr n_gas sum_p N_p fqe[p, e] + qepri (tracer_fp_cpu.c:142-147),
-            // stored scaled by dt into s_row for the electron runs
-            double acc[R];
+        {
+            double src[R];
 #pragma unroll
-            for (int k = 0; k < R; ++k) acc[k] = 0.0;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            const double *wbase = S.fqe_t + base;
-#pragma unroll 4
-            for (int jp = S.jp0; jp < N; ++jp) {
-                const double nn = s_row[jp];
-                const double *wrow = wbase + (size_t)jp * N;
+            for (int k = 0; k < R; ++k) src[k] = S.qepri ? S.qepri[roff + k] : 0.0;
+            if (sec) {
+                // inje = Gyr n_gas sum_p N_p fqe[p, e] + qepri (tracer_fp_cpu.c:142-147)
+                double acc[R];
 #pragma unroll
-                for (int k = 0; k < R; ++k) acc[k] += nn * wrow[k];
+                for (int k = 0; k < R; ++k) acc[k] = 0.0;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                for (int jp = S.jp0; jp < N; ++jp) {
+                    const double nn = s_row[jp];
+                    const double *wrow = S.fqe_t + (size_t)jp * N + base;
+#pragma unroll
+                    for (int k = 0; k < R; ++k) acc[k] += nn * wrow[k];
+                }
+                __builtin_amdgcn_wave_barrier();
+#pragma unroll
+                for (int k = 0; k < R; ++k) src[k] = acc[k] * K_GYR * q.n_gas + src[k];
             }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
-            for (int k = 0; k < R; ++k) {
-                const double qe = S.qepri ? S.qepri[toff + base + k] : 0.0;
-                s_row[base + k] = dt * (acc[k] * K_GYR * n_gas + qe);
-            }
-            species_segment<true, N, R, G>(gl, g, steps, windowed, ls, le, sc, rc, dfac_on, dfac_off, mode_on, s_row,
-                                           1.0, xe, bad);
-        } else {
-            species_segment<true, N, R, G>(gl, g, steps, windowed, ls, le, sc, rc, dfac_on, dfac_off, mode_on,
-                                           S.qepri ? S.qepri + toff : nullptr, dt, xe, bad);
+            for (int k = 0; k < R; ++k) src[k] = dt * src[k];
+            double radp[R];
+            seg_radp<true, R>(g, S, base, nseg, alpha, q, radp);
+            species_segment<true, N, R, G>(gl, g, steps, windowed, ls, le, rc, dfac_on, dfac_off, mode_on, radp, src,
+                                           xe, bad);
         }
     }
 }

@@ -1,0 +1,367 @@
+// croma_emission.hip -- emission contractions on the device.
+//
+// IC, pp gamma and pp neutrino emissivities are one contraction each:
+//   out[i][k] = row_scale[i] * sum_j spectra[i][j] * kmat[j][k]
+// (prepare_ic_emission_cell_major fp_shared_core.c:2131-2157 with the dp
+// weights folded into kmat; prepare_{gamma,neutrino}_emission_cell_major
+// :2166-2243 with the trapezoid weights, beta and E_k folded into kmat and
+// n_gas as the row scale).  Tiles of 64 tracers x 64 bins per block:
+//   mode 0  FP32 on the vector ALUs
+//   mode 1  split-BF16 x3 on MFMA (v_mfma_f32_16x16x32_bf16), FP32 accumulate
+//   mode 2  one BF16 pass on MFMA, FP32 accumulate
+//   mode 3  FP64 on the vector ALUs
+// The reduced-precision modes scale every tracer's spectrum by a power of two
+// (its largest entry) and every kernel column likewise before rounding, and
+// undo both on output, so spectra spanning many decades keep their relative
+// precision.  Synchrotron (prepare_synch_emission_cell_major :2070-2129)
+// interpolates its kernel per tracer in log B and runs on the vector ALUs.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "croma_kernels.h"
+
+#define EM_TR 64   // tracers per block
+#define EM_TB 64   // bins per block
+#define EM_KC 128  // K chunk staged in LDS (reduced-precision modes)
+#define EM_KC64 64 // K chunk (FP64 mode)
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t em_bf16_bits(float f)
+{
+    // round to nearest even (inputs are finite)
+    const uint32_t u = __float_as_uint(f);
+    return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+
+__device__ __forceinline__ float em_bf16_val(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// Largest binary exponent of row i of spectra (frexp convention), 0 for an
+// all-zero row.  4 threads per row.
+__device__ __forceinline__ int em_row_shift(const double *__restrict__ row, int K, int sub)
+{
+    int e = -100000;
+    for (int j = sub; j < K; j += 4) {
+        const double v = fabs(row[j]);
+        if (v > 0.0 && isfinite(v)) {
+            const int ev = __builtin_amdgcn_frexp_exp(v);
+            e = ev > e ? ev : e;
+        }
+    }
+    e = max(e, __shfl_xor(e, 1, 4));
+    e = max(e, __shfl_xor(e, 2, 4));
+    return e == -100000 ? 0 : e;
+}
+
+// kmat_t: [nbins_pad][K] as bf16 hi and lo halves (column-scaled on the host
+// side of this file), col_shift [nbins_pad].
+template <int SPLIT>
+__global__ __launch_bounds__(256) void k_emis_mfma(int ntr, int K, int nbins, int ld_out,
+                                                   const double *__restrict__ spectra,
+                                                   const uint16_t *__restrict__ kt_hi,
+                                                   const uint16_t *__restrict__ kt_lo,
+                                                   const int *__restrict__ col_shift,
+                                                   const double *__restrict__ row_scale,
+                                                   double *__restrict__ out)
+{
+    __shared__ __attribute__((aligned(16))) uint16_t a_hi[EM_TR][EM_KC + 8];
+    __shared__ __attribute__((aligned(16))) uint16_t a_lo[EM_TR][EM_KC + 8];
+    __shared__ __attribute__((aligned(16))) uint16_t b_hi[EM_TB][EM_KC + 8];
+    __shared__ __attribute__((aligned(16))) uint16_t b_lo[EM_TB][EM_KC + 8];
+    __shared__ int s_shift[EM_TR];
+    const int t = threadIdx.x;
+    const int lane = t & 63, w = t >> 6;
+    const int i0 = blockIdx.x * EM_TR;
+    const int c0 = blockIdx.y * EM_TB;
+    {
+        const int r = t >> 2, sub = t & 3;
+        int sh = 0;
+        if (i0 + r < ntr) sh = em_row_shift(spectra + (size_t)(i0 + r) * K, K, sub);
+        if (sub == 0) s_shift[r] = sh;
+    }
+    __syncthreads();
+    floatx4 acc[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[c] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < K; k0 += EM_KC) {
+        const int kc = min(EM_KC, K - k0);
+        // A chunk: rows i0.., columns k0..k0+kc, coalesced along each row
+        for (int e = t; e < EM_TR * EM_KC; e += 256) {
+            const int r = e / EM_KC, kk = e % EM_KC;
+            float v = 0.f;
+            if (i0 + r < ntr && kk < kc) v = (float)ldexp(spectra[(size_t)(i0 + r) * K + k0 + kk], -s_shift[r]);
+            const uint16_t h = em_bf16_bits(v);
+            a_hi[r][kk] = h;
+            a_lo[r][kk] = SPLIT ? em_bf16_bits(v - em_bf16_val(h)) : 0;
+        }
+        for (int e = t; e < EM_TB * EM_KC; e += 256) {
+            const int cc = e / EM_KC, kk = e % EM_KC;
+            const size_t g = (size_t)(c0 + cc) * K + k0 + kk;
+            b_hi[cc][kk] = (kk < kc) ? kt_hi[g] : 0;
+            b_lo[cc][kk] = (SPLIT && kk < kc) ? kt_lo[g] : 0;
+        }
+        __syncthreads();
+        for (int kk = 0; kk < kc; kk += 32) {
+            const int ar = 16 * w + (lane & 15), ak = kk + (lane >> 4) * 8;
+            const bf16x8 ah = *reinterpret_cast<const bf16x8 *>(&a_hi[ar][ak]);
+            const bf16x8 al = *reinterpret_cast<const bf16x8 *>(&a_lo[ar][ak]);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int bc = 16 * c + (lane & 15);
+                const bf16x8 bh = *reinterpret_cast<const bf16x8 *>(&b_hi[bc][ak]);
+                acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[c], 0, 0, 0);
+                if (SPLIT) {
+                    const bf16x8 bl = *reinterpret_cast<const bf16x8 *>(&b_lo[bc][ak]);
+                    acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc[c], 0, 0, 0);
+                    acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[c], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // D layout of the 16x16 MFMA: lane holds rows 4*(lane/16)+q, column lane%16
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int col = c0 + 16 * c + (lane & 15);
+        if (col >= nbins) continue;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int r = 16 * w + 4 * (lane >> 4) + q;
+            if (i0 + r >= ntr) continue;
+            double v = ldexp((double)acc[c][q], s_shift[r] + col_shift[col]);
+            if (row_scale) v *= row_scale[i0 + r];
+            out[(size_t)(i0 + r) * ld_out + col] = v;
+        }
+    }
+}
+
+// FP32 vector-ALU contraction: 4 x 4 outputs per thread.
+__global__ __launch_bounds__(256) void k_emis_f32(int ntr, int K, int nbins, int ld_out,
+                                                  const double *__restrict__ spectra,
+                                                  const float *__restrict__ kt, const int *__restrict__ col_shift,
+                                                  const double *__restrict__ row_scale, double *__restrict__ out)
+{
+    __shared__ float a_s[EM_KC][EM_TR + 4];
+    __shared__ float b_s[EM_KC][EM_TB + 4];
+    __shared__ int s_shift[EM_TR];
+    const int t = threadIdx.x;
+    const int i0 = blockIdx.x * EM_TR;
+    const int c0 = blockIdx.y * EM_TB;
+    {
+        const int r = t >> 2, sub = t & 3;
+        int sh = 0;
+        if (i0 + r < ntr) sh = em_row_shift(spectra + (size_t)(i0 + r) * K, K, sub);
+        if (sub == 0) s_shift[r] = sh;
+    }
+    __syncthreads();
+    const int tr = t >> 4, tc = t & 15;
+    float acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = 0.f;
+    for (int k0 = 0; k0 < K; k0 += EM_KC) {
+        const int kc = min(EM_KC, K - k0);
+        for (int e = t; e < EM_TR * EM_KC; e += 256) {
+            const int r = e / EM_KC, kk = e % EM_KC;
+            float v = 0.f;
+            if (i0 + r < ntr && kk < kc) v = (float)ldexp(spectra[(size_t)(i0 + r) * K + k0 + kk], -s_shift[r]);
+            a_s[kk][r] = v;
+        }
+        for (int e = t; e < EM_TB * EM_KC; e += 256) {
+            const int cc = e / EM_KC, kk = e % EM_KC;
+            b_s[kk][cc] = (kk < kc) ? kt[(size_t)(c0 + cc) * K + k0 + kk] : 0.f;
+        }
+        __syncthreads();
+        for (int kk = 0; kk < kc; ++kk) {
+            float a[4], b[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                a[q] = a_s[kk][4 * tr + q];
+                b[q] = b_s[kk][tc + 16 * q];
+            }
+#pragma unroll
+            for (int p = 0; p < 4; ++p)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[p][q] = fmaf(a[p], b[q], acc[p][q]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        const int r = 4 * tr + p;
+        if (i0 + r >= ntr) continue;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int col = c0 + tc + 16 * q;
+            if (col >= nbins) continue;
+            double v = ldexp((double)acc[p][q], s_shift[r] + col_shift[col]);
+            if (row_scale) v *= row_scale[i0 + r];
+            out[(size_t)(i0 + r) * ld_out + col] = v;
+        }
+    }
+}
+
+// FP64 vector-ALU contraction (the reference precision), 4 x 4 per thread.
+__global__ __launch_bounds__(256) void k_emis_f64(int ntr, int K, int nbins, int ld_out,
+                                                  const double *__restrict__ spectra,
+                                                  const double *__restrict__ kt64,
+                                                  const double *__restrict__ row_scale, double *__restrict__ out)
+{
+    __shared__ double a_s[EM_KC64][EM_TR + 2];
+    __shared__ double b_s[EM_KC64][EM_TB + 2];
+    const int t = threadIdx.x;
+    const int i0 = blockIdx.x * EM_TR;
+    const int c0 = blockIdx.y * EM_TB;
+    const int tr = t >> 4, tc = t & 15;
+    double acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+    for (int k0 = 0; k0 < K; k0 += EM_KC64) {
+        const int kc = min(EM_KC64, K - k0);
+        for (int e = t; e < EM_TR * EM_KC64; e += 256) {
+            const int r = e / EM_KC64, kk = e % EM_KC64;
+            a_s[kk][r] = (i0 + r < ntr && kk < kc) ? spectra[(size_t)(i0 + r) * K + k0 + kk] : 0.0;
+        }
+        for (int e = t; e < EM_TB * EM_KC64; e += 256) {
+            const int cc = e / EM_KC64, kk = e % EM_KC64;
+            b_s[kk][cc] = (kk < kc) ? kt64[(size_t)(c0 + cc) * K + k0 + kk] : 0.0;
+        }
+        __syncthreads();
+        for (int kk = 0; kk < kc; ++kk) {
+            double a[4], b[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                a[q] = a_s[kk][4 * tr + q];
+                b[q] = b_s[kk][tc + 16 * q];
+            }
+#pragma unroll
+            for (int p = 0; p < 4; ++p)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[p][q] = fma(a[p], b[q], acc[p][q]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        const int r = 4 * tr + p;
+        if (i0 + r >= ntr) continue;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int col = c0 + tc + 16 * q;
+            if (col >= nbins) continue;
+            double v = acc[p][q];
+            if (row_scale) v *= row_scale[i0 + r];
+            out[(size_t)(i0 + r) * ld_out + col] = v;
+        }
+    }
+}
+
+extern "C" int ck_emission_contract(int ntr, int K, int nbins, int nbins_pad, const double *spectra,
+                                    const uint16_t *kt_hi, const uint16_t *kt_lo, const float *kt32,
+                                    const double *kt64, const int *col_shift, const double *row_scale,
+                                    double *out, int mode, void *stream)
+{
+    if (ntr <= 0 || K <= 0 || nbins <= 0 || nbins_pad < nbins || nbins_pad % EM_TB) return -1;
+    const dim3 grid((ntr + EM_TR - 1) / EM_TR, nbins_pad / EM_TB);
+    hipStream_t st = (hipStream_t)stream;
+    switch (mode) {
+    case 0:
+        hipLaunchKernelGGL(k_emis_f32, grid, dim3(256), 0, st, ntr, K, nbins, nbins, spectra, kt32, col_shift,
+                           row_scale, out);
+        break;
+    case 1:
+        hipLaunchKernelGGL(k_emis_mfma<1>, grid, dim3(256), 0, st, ntr, K, nbins, nbins, spectra, kt_hi, kt_lo,
+                           col_shift, row_scale, out);
+        break;
+    case 2:
+        hipLaunchKernelGGL(k_emis_mfma<0>, grid, dim3(256), 0, st, ntr, K, nbins, nbins, spectra, kt_hi, kt_lo,
+                           col_shift, row_scale, out);
+        break;
+    case 3:
+        hipLaunchKernelGGL(k_emis_f64, grid, dim3(256), 0, st, ntr, K, nbins, nbins, spectra, kt64, row_scale,
+                           out);
+        break;
+    default: return -1;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// Synchrotron: one wave per tracer, lanes over frequencies.  The kernel is
+// interpolated in log B between the table planes ib, ib+1 as the reference
+// does (log-linear when both entries are positive, else linear); the sum
+// runs over je = 2.. up to the first bin whose successor is not positive.
+template <typename T>
+__global__ __launch_bounds__(256) void k_emis_synch(int ntr, int n, int nfreq, int nlogb, double logb_min,
+                                                    double inv_dlogb, const double *__restrict__ table,
+                                                    const double *__restrict__ cre, const double *__restrict__ dp,
+                                                    const double *__restrict__ b_field, double *__restrict__ eps)
+{
+    __shared__ T s_w[4][512];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int it = blockIdx.x * 4 + w;
+    if (it >= ntr) return;
+    const double *cl = cre + (size_t)it * n;
+    // end of the summation: first je >= 2 with cl[je + 1] <= 0 (or n - 1)
+    int jend = n - 1;
+    for (int base = 2; base < n - 1; base += 64) {
+        const int je = base + lane;
+        const bool stop = je < n - 1 && !(cl[je + 1] > 0.0);
+        const unsigned long long m = __ballot(stop);
+        if (m) {
+            jend = base + __ffsll((long long)m) - 1;
+            break;
+        }
+    }
+    for (int je = lane; je < n; je += 64) s_w[w][je] = (T)(cl[je] * dp[je]);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    double bval = b_field[it];
+    if (!(bval > 0.0) || !isfinite(bval)) bval = 1.0e-30;
+    const double logb = log10(bval / 1.0e-6);
+    const double u = (logb - logb_min) * inv_dlogb;
+    int ib = (int)floor(u);
+    if (ib < 0) ib = 0;
+    if (ib > nlogb - 2) ib = nlogb - 2;
+    double tt = u - (double)ib;
+    if (tt < 0.0) tt = 0.0;
+    if (tt > 1.0) tt = 1.0;
+    const T t = (T)tt;
+    const double A = sqrt(3.0) * pow(4.8e-10, 3.0) * b_field[it] / (2.0 * 9.11e-28 * 2.999e10 * 2.999e10) /
+                     (4.0 * 3.14159265358979323846);
+    for (int nf = lane; nf < nfreq; nf += 64) {
+        const double *r0 = table + ((size_t)ib * nfreq + nf) * n;
+        const double *r1 = table + ((size_t)(ib + 1) * nfreq + nf) * n;
+        T integral = 0;
+        for (int je = 2; je < jend; ++je) {
+            const T k0 = (T)r0[je], k1 = (T)r1[je];
+            T K;
+            if (tt <= 0.0) K = k0;
+            else if (tt >= 1.0) K = k1;
+            else if (k0 > (T)0 && k1 > (T)0) K = exp(((T)1 - t) * log(k0) + t * log(k1));
+            else K = ((T)1 - t) * k0 + t * k1;
+            integral += s_w[w][je] * K;
+        }
+        eps[(size_t)it * nfreq + nf] = A * (double)integral;
+    }
+}
+
+extern "C" int ck_emission_synch(int ntr, int n, int nfreq, int nlogb, double logb_min, double inv_dlogb,
+                                 const double *table, const double *cre, const double *dp, const double *b_field,
+                                 double *eps, int fp64, void *stream)
+{
+    if (ntr <= 0 || n <= 3 || n > 512 || nfreq <= 0 || nlogb < 2) return -1;
+    const dim3 grid((ntr + 3) / 4);
+    if (fp64)
+        hipLaunchKernelGGL(k_emis_synch<double>, grid, dim3(256), 0, (hipStream_t)stream, ntr, n, nfreq, nlogb,
+                           logb_min, inv_dlogb, table, cre, dp, b_field, eps);
+    else
+        hipLaunchKernelGGL(k_emis_synch<float>, grid, dim3(256), 0, (hipStream_t)stream, ntr, n, nfreq, nlogb,
+                           logb_min, inv_dlogb, table, cre, dp, b_field, eps);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}

@@ -175,20 +175,19 @@ int ck_solve_cc_batch(int nsys, int nrow, double dt, const double *a, const doub
                       const double *c, const double *q, const double *state_in, double *state_out,
                       double *cprime_ws, int *err_dev, void *stream);
 
-/* Emission contractions.  spectra [ntr x K] FP64 cell-major; kernel matrix
- * K_T [K x nbins] FP32 (weights folded in); row_scale [ntr] (may be NULL);
- * out [ntr x nbins] FP32.  mode 0: FP32 VALU reference kernel, 1: FP32
- * inputs on MFMA (xf32 emulation via split-BF16 x3), 2: split-BF16 x3 MFMA. */
-int ck_emission_contract(int ntr, int K, int nbins, const double *spectra, const float *kmat,
-                         const double *row_scale, float *out, int mode, void *stream);
-/* FP64 reference contraction used for checks (out FP64). */
-int ck_emission_contract_f64(int ntr, int K, int nbins, const double *spectra,
-                             const double *kmat, const double *row_scale, double *out,
-                             void *stream);
-/* Synchrotron: per-tracer log-interpolated table contraction (VALU). */
+/* Emission contractions: out[i][k] = row_scale[i] * sum_j spectra[i][j] kmat[j][k].
+ * spectra [ntr x K] FP64; the kernel matrix in bin-major layout [nbins_pad][K]
+ * (nbins_pad a multiple of 64, zero padded) as FP64 (kt64), FP32 scaled by
+ * 2^-col_shift[k] (kt32) and that FP32 value split into BF16 halves (kt_hi,
+ * kt_lo); row_scale [ntr] or NULL; out [ntr x nbins] FP64.  mode 0 FP32
+ * VALU, 1 split-BF16 x3 on MFMA, 2 one BF16 pass on MFMA, 3 FP64 VALU. */
+int ck_emission_contract(int ntr, int K, int nbins, int nbins_pad, const double *spectra, const uint16_t *kt_hi,
+                         const uint16_t *kt_lo, const float *kt32, const double *kt64, const int *col_shift,
+                         const double *row_scale, double *out, int mode, void *stream);
+/* Synchrotron: per-tracer log-B interpolated table contraction (VALU, FP64 or FP32). */
 int ck_emission_synch(int ntr, int n, int nfreq, int nlogb, double logb_min, double inv_dlogb,
                       const double *table, const double *cre, const double *dp,
-                      const double *b_field, double *eps, void *stream);
+                      const double *b_field, double *eps, int fp64, void *stream);
 
 #ifdef __cplusplus
 }

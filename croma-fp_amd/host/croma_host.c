@@ -410,7 +410,7 @@ int croma_hip_ic_cooling_rows_device(CromaHipContext *c, int nz, const double *z
 }
 
 /* IC cooling rate at the first and last electron bins at z (cached). */
-static void ctx_ic_ends(CromaHipContext *c, double z, double ends[2])
+void ch_ctx_ic_ends(CromaHipContext *c, double z, double ends[2])
 {
     int k, bins[2];
     pthread_mutex_lock(&c->ic_lock);
@@ -481,7 +481,7 @@ int croma_hip_nsub_estimate(const CromaHipContext *c, int ntr, double dt_snap, c
                : CROMA_HIP_EINVAL;
 }
 
-static int ctx_fqe(CromaHipContext *c)
+int ch_ctx_fqe(CromaHipContext *c)
 {
     const int n = c->g.n;
     if (c->fqe_ready) return 0;
@@ -502,20 +502,28 @@ int croma_hip_secondary_weights(const CromaHipContext *cc, double *fqe_t, int *n
     CromaHipContext *c = (CromaHipContext *)cc;
     int rc;
     if (!c || !fqe_t || !np_min_qe) return CROMA_HIP_EINVAL;
-    rc = ctx_fqe(c);
+    rc = ch_ctx_fqe(c);
     if (rc) return rc;
     memcpy(fqe_t, c->h_fqe, (size_t)c->g.n * c->g.n * sizeof(double));
     memcpy(np_min_qe, c->h_npmin, (size_t)c->g.n * sizeof(int));
     return 0;
 }
 
+/* Accessors for the serving loop (croma_serve.c). */
+const ChGrid *ch_ctx_grid(const CromaHipContext *c) { return &c->g; }
+const CkGrid *ch_ctx_dgrid(const CromaHipContext *c) { return &c->dgrid; }
+const CkParams *ch_ctx_prm(const CromaHipContext *c) { return &c->prm; }
+int ch_ctx_cu_count(const CromaHipContext *c) { return c->cu_count; }
+CromaHipTile ch_ctx_tile(const CromaHipContext *c, int protons) { return c->tile[protons ? 1 : 0]; }
+const double *ch_ctx_d_fqe(const CromaHipContext *c) { return c->d_fqe; }
+int ch_ctx_jp0(const CromaHipContext *c) { return c->jp0; }
+
 /* ------------------------------------------------------------------ */
 /* task ordering: counting sort of tracers by descending cost, cost =
  * sum over the task's snapshots of nsub x species + 3 x segments (the
  * coefficient builds), bucketed on a log scale (1/256 octave). */
-#define CH_HIST 8192
 
-static int cost_bucket(double cost)
+int ch_cost_bucket(double cost)
 {
     int k;
     if (cost < 1.0) return 0;
@@ -535,7 +543,7 @@ static void order_tasks(const CkParams *prm, const int *allow, int nslot, int nt
             const int ns = nsub[(size_t)s * ntr + i];
             cost += (double)ns * (protons ? 2 : 1) + 3.0 * ch_choose_segments(prm, ns, allow[s]);
         }
-        key_buf[i] = cost_bucket(cost);
+        key_buf[i] = ch_cost_bucket(cost);
         hist_buf[key_buf[i]]++;
     }
     for (i = CH_HIST - 1; i >= 0; i--) {
@@ -669,7 +677,8 @@ CromaHipBatch *croma_hip_batch_create(CromaHipContext *c, int ntr, int species, 
     const size_t sn = (size_t)ntr * n * sizeof(double);
     int k;
     if (!c || ntr <= 0 || nsnap_res <= 0 ||
-        !(species == CROMA_HIP_SPECIES_ELECTRON || species == CROMA_HIP_SPECIES_ELECTRON_PROTON)) {
+        !(species == CROMA_HIP_SPECIES_ELECTRON || species == CROMA_HIP_SPECIES_PROTON ||
+          species == CROMA_HIP_SPECIES_ELECTRON_PROTON)) {
         ch_set_error("invalid batch arguments");
         return 0;
     }
@@ -678,7 +687,7 @@ CromaHipBatch *croma_hip_batch_create(CromaHipContext *c, int ntr, int species, 
     b->ctx = c;
     b->ntr = ntr;
     b->species = species;
-    b->protons = (species == CROMA_HIP_SPECIES_ELECTRON_PROTON);
+    b->protons = (species != CROMA_HIP_SPECIES_ELECTRON);
     b->nsnap_res = nsnap_res;
     b->d_cre = (double *)ck_malloc(sn);
     b->d_crp = b->protons ? (double *)ck_malloc(sn) : 0;
@@ -694,7 +703,7 @@ CromaHipBatch *croma_hip_batch_create(CromaHipContext *c, int ntr, int species, 
         goto fail;
     for (k = 0; k < 2; k++)
         if (win_reserve(&b->win[k], 1, ntr, n, c->stream)) goto fail;
-    if (b->protons && ctx_fqe(c) != 0) goto fail;
+    if (b->protons && ch_ctx_fqe(c) != 0) goto fail;
     return b;
 fail:
     if (!g_err[0]) ch_set_error("batch allocation failed: %s", ck_last_error());
@@ -861,7 +870,7 @@ static int prepare_window(CromaHipBatch *b, WinBuf *w, int s_first, int count, i
         }
         allow_v[q] = al;
         t0 = ch_now_s();
-        ctx_ic_ends(c, z[s], ends);
+        ch_ctx_ic_ends(c, z[s], ends);
         ps->ic_ms += 1e3 * (ch_now_s() - t0);
         t0 = ch_now_s();
         ch_nsub_estimate(&c->g, &c->prm, ntr, dt_snap[s], ends[0], ends[1], hb + 0 * (size_t)ntr,
@@ -1417,5 +1426,117 @@ out:
     ck_event_destroy(e0);
     ck_event_destroy(e1);
     ck_stream_destroy(s);
+    return rc;
+}
+
+/* ------------------------------------------------------------------ */
+/* Autotuner: times every launch shape of the fused kernel on a seeded
+ * synthetic probe (backgrounds drawn like the benchmark workloads) and keeps
+ * the fastest.  Ties within 2% go to the larger rows_per_lane (fewer lanes
+ * per tracer leave more tracers per wave for the tail). */
+static double at_uniform(unsigned long long *s)
+{
+    /* splitmix64 */
+    unsigned long long z = (*s += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    z ^= z >> 31;
+    return ((double)(z >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+static double at_normal(unsigned long long *s)
+{
+    const double u1 = at_uniform(s), u2 = at_uniform(s);
+    return sqrt(-2.0 * log(u1)) * cos(2.0 * 3.14159265358979323846 * u2);
+}
+
+int croma_hip_autotune(CromaHipContext *c, int protons, int ntracer, CromaHipTile *best)
+{
+    static const int rows[4] = {2, 4, 8, 16};
+    const int n = c ? c->g.n : 0;
+    const int nsnap = 3;
+    double *bg = 0, *cre = 0, *crp = 0, z[4], dt[3];
+    double best_ms = 1e300;
+    CromaHipTile keep, bt;
+    CromaHipBatch *b = 0;
+    unsigned long long seed = 0x5EEDULL + (unsigned long long)n;
+    int i, s, f, r, w, rc = 0;
+    if (!c || ntracer <= 0 || !best) return CROMA_HIP_EINVAL;
+    keep = c->tile[protons ? 1 : 0];
+    bt = keep;
+    bg = (double *)malloc((size_t)(nsnap + 1) * CK_NBG * ntracer * sizeof(double));
+    cre = (double *)malloc((size_t)ntracer * n * sizeof(double));
+    crp = (double *)malloc((size_t)ntracer * n * sizeof(double));
+    if (!bg || !cre || !crp) {
+        rc = CROMA_HIP_ENOMEM;
+        goto out;
+    }
+    for (i = 0; i < ntracer; i++) {
+        const double ln = -5.0 + 3.0 * at_uniform(&seed), lt = 6.0 + 2.0 * at_uniform(&seed);
+        const double lb = -2.0 + 3.0 * at_uniform(&seed), curl = 3.0e-17 * exp(at_normal(&seed));
+        const double amp = pow(10.0, at_uniform(&seed) - 0.5);
+        int j;
+        for (s = 0; s <= nsnap; s++) {
+            const double drift = 0.02 * s;
+            const double ngas = pow(10.0, ln + drift), temp = pow(10.0, lt), bf = pow(10.0, lb - drift) * 1.0e-6;
+            const double cs = sqrt(5.0 / 3.0 * 8.62e-5 * temp / (0.59 * 0.938 * 1.0e9)) * 2.999e10;
+            double dv = curl * 0.15 * 3.0857e24;
+            double *o = bg + (size_t)s * CK_NBG * ntracer;
+            if (dv / cs > 0.5) dv = 0.5 * cs;
+            o[CK_BG_NGAS * (size_t)ntracer + i] = ngas;
+            o[CK_BG_KBT * (size_t)ntracer + i] = 8.62e-5 * temp * 1.0e-9;
+            o[CK_BG_B * (size_t)ntracer + i] = bf;
+            o[CK_BG_DIVV * (size_t)ntracer + i] = 1.0e-17 * at_normal(&seed) * 3.1536e16;
+            o[CK_BG_LTURB * (size_t)ntracer + i] = 0.15;
+            o[CK_BG_DV * (size_t)ntracer + i] = dv;
+            o[CK_BG_CS * (size_t)ntracer + i] = cs;
+            o[CK_BG_BETA * (size_t)ntracer + i] = ngas * 8.62e-5 * temp * 1.6e-12 / (bf * bf / (8.0 * 3.14159265358979323846));
+        }
+        for (j = 0; j < n; j++) {
+            cre[(size_t)i * n + j] = amp * pow(c->g.ep[j], -2.2) * exp(-c->g.ep[j] / 1.0e4);
+            crp[(size_t)i * n + j] = 100.0 * amp * pow(c->g.pp[j], -2.2) * exp(-c->g.pp[j] / 1.0e6);
+        }
+    }
+    for (s = 0; s <= nsnap; s++) z[s] = 0.5 - 0.01 * s;
+    for (s = 0; s < nsnap; s++) dt[s] = 0.077;
+    b = croma_hip_batch_create(c, ntracer, protons ? CROMA_HIP_SPECIES_ELECTRON_PROTON : CROMA_HIP_SPECIES_ELECTRON,
+                               nsnap + 1);
+    if (!b) {
+        rc = CROMA_HIP_ENOMEM;
+        goto out;
+    }
+    for (s = 0; s <= nsnap; s++) {
+        const double *fl[CK_NBG];
+        for (f = 0; f < CK_NBG; f++) fl[f] = bg + ((size_t)s * CK_NBG + f) * ntracer;
+        if ((rc = croma_hip_batch_set_background(b, s, fl)) != 0) goto out;
+    }
+    for (r = 0; r < 4; r++) {
+        if (!ck_fused_supported(n, rows[r])) continue;
+        for (w = 1; w <= 2; w++) {
+            CromaHipTile t;
+            CromaHipTimes tm;
+            t.rows_per_lane = rows[r];
+            t.waves_per_simd = w;
+            t.chain = 1;
+            c->tile[protons ? 1 : 0] = t;
+            memset(&tm, 0, sizeof(tm));
+            if ((rc = croma_hip_batch_set_state(b, protons ? crp : 0, cre, 0, 0, 0, 0)) != 0) goto out;
+            /* first snapshot warms the instance, the next two are timed */
+            if ((rc = croma_hip_batch_run(b, 0, 1, nsnap + 1, z, dt, 0.0, CROMA_HIP_PATH_FUSED, 0)) != 0) goto out;
+            if ((rc = croma_hip_batch_run(b, 1, nsnap, nsnap + 1, z, dt, dt[0], CROMA_HIP_PATH_FUSED, &tm)) != 0)
+                goto out;
+            if (tm.kernel_ms < best_ms * 0.98 || (tm.kernel_ms < best_ms * 1.02 && rows[r] > bt.rows_per_lane)) {
+                if (tm.kernel_ms < best_ms) best_ms = tm.kernel_ms;
+                bt = t;
+            }
+        }
+    }
+out:
+    croma_hip_batch_destroy(b);
+    free(bg);
+    free(cre);
+    free(crp);
+    c->tile[protons ? 1 : 0] = rc ? keep : bt;
+    if (!rc) *best = bt;
     return rc;
 }

@@ -52,4 +52,17 @@ double ch_now_s(void);
 void ch_set_error(const char *fmt, ...);
 void ch_params_to_ck(const CromaHipParams *in, CkParams *out);
 
+/* context internals shared with the serving loop */
+#define CH_HIST 8192
+const ChGrid *ch_ctx_grid(const CromaHipContext *c);
+const CkGrid *ch_ctx_dgrid(const CromaHipContext *c);
+const CkParams *ch_ctx_prm(const CromaHipContext *c);
+int ch_ctx_cu_count(const CromaHipContext *c);
+CromaHipTile ch_ctx_tile(const CromaHipContext *c, int protons);
+int ch_ctx_fqe(CromaHipContext *c);              /* builds the secondary weights once */
+const double *ch_ctx_d_fqe(const CromaHipContext *c);
+int ch_ctx_jp0(const CromaHipContext *c);
+void ch_ctx_ic_ends(CromaHipContext *c, double z, double ends[2]); /* IC rate at bins 0, n-1 (cached) */
+int ch_cost_bucket(double cost);                 /* log-scale bucket in [0, CH_HIST) */
+
 #endif

@@ -54,7 +54,7 @@ class Workload:
     n: int
     ntracer: int
     nsnap: int
-    species: int                  # 1 electron_only, 3 electron_proton
+    species: int                  # seed species: 1 electron_only, 2 proton_only, 3 electron_proton
     dpp_model: int                # 1 asa, 2 ttd, 3 off
     z: np.ndarray                 # [nsnap + 1]
     dt_snap: np.ndarray           # [nsnap]
@@ -71,7 +71,7 @@ class Workload:
 
     @property
     def protons(self):
-        return self.species == 3
+        return self.species in (2, 3)
 
     def subset(self, idx):
         """Tracer subset (same snapshots), e.g. for a CPU baseline sample."""
@@ -129,7 +129,9 @@ def make_workload(n=128, ntracer=1024, nsnap=100, species=1, dpp_model=3, seed=1
     pp = grid_axis(n, -1.0, 8.0)
     spec_e = pe ** -2.2 * np.exp(-pe / e_cut)
     cre0 = np.tile(spec_e, nt) * np.repeat(10.0 ** rng.uniform(-0.5, 0.5, nt), n)
-    if species == 3:
+    if species == 2:
+        cre0 = np.zeros(nt * n)
+    if species in (2, 3):
         spec_p = 100.0 * pp ** -2.2 * np.exp(-pp / p_cut)
         crp0 = np.tile(spec_p, nt) * np.repeat(10.0 ** rng.uniform(-0.5, 0.5, nt), n)
     else:
@@ -144,7 +146,7 @@ def make_workload(n=128, ntracer=1024, nsnap=100, species=1, dpp_model=3, seed=1
             q = 2.0 * (mach ** 2 + 1.0) / (mach ** 2 - 1.0)
             qe = np.where(hit[:, None], pe[None, :] ** -q[:, None] * np.exp(-pe[None, :] / 1.0e5), 0.0)
             w.qepri[s] = np.ascontiguousarray(qe.ravel())
-            if species == 3:
+            if species in (2, 3):
                 qp = np.where(hit[:, None], 100.0 * pp[None, :] ** -q[:, None] * np.exp(-pp[None, :] / 1.0e6), 0.0)
                 w.qpi[s] = np.ascontiguousarray(qp.ravel())
     return w

@@ -172,8 +172,9 @@ int croma_hip_secondary_weights(const CromaHipContext *ctx, double *fqe_t, int *
  * run_tracer_pipeline, fp_cuda_backend.h:124-150) --------------------- */
 typedef struct CromaHipBatch CromaHipBatch;
 
-#define CROMA_HIP_SPECIES_ELECTRON 1          /* electron_only: crp == 0 */
-#define CROMA_HIP_SPECIES_ELECTRON_PROTON 3   /* both species evolve */
+#define CROMA_HIP_SPECIES_ELECTRON 1          /* electron_only seed: crp == 0, only electrons evolve */
+#define CROMA_HIP_SPECIES_PROTON 2            /* proton_only seed: both species evolve (secondaries) */
+#define CROMA_HIP_SPECIES_ELECTRON_PROTON 3   /* electron_proton seed: both species evolve */
 
 /* nsnap_resident: number of snapshot background sets kept in HBM. */
 CromaHipBatch *croma_hip_batch_create(CromaHipContext *ctx, int ntracer, int species, int nsnap_resident);

@@ -1,0 +1,99 @@
+"""Emission contractions on the device vs the reference / oracle.
+
+Tolerances (stated per mode): FP64 1e-12 relative; FP32 VALU and split-BF16
+x3 on MFMA 1e-4 relative (the emissivity bar); one BF16 pass 2e-2 relative
+(its 8-bit mantissa, documented as the fast lossy mode).  IC is checked
+against the compiled reference's own kernel table and contraction (golden
+fixtures); hadronic and synchrotron against the oracle restatement."""
+import os
+
+import numpy as np
+import pytest
+
+import croma_hip as ch
+import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+TOL = {ch.EMIS_FP64: 1e-12, ch.EMIS_FP32: 1e-4, ch.EMIS_MFMA_XF32: 1e-4, ch.EMIS_MFMA_BF16: 2e-2}
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def rel(got, want):
+    """Max relative error over the entries within 25 decades of their
+    tracer's largest emissivity (and above the FP64 denormal range): deeper
+    entries are sums over the exponential tails of the spectrum, below what
+    a reduced-precision contraction resolves and irrelevant to the emissivity."""
+    want = np.atleast_2d(np.asarray(want))
+    got = np.asarray(got).reshape(want.shape)
+    peak = np.max(np.abs(want), axis=1, keepdims=True)
+    m = np.abs(want) > np.maximum(1e-280, 1e-25 * peak)
+    return float(np.max(np.abs(got[m] - want[m]) / np.abs(want[m])))
+
+
+@pytest.fixture(scope="module")
+def ctx128():
+    c = ch.Context(128, ch.default_params())
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("mode", list(TOL))
+def test_ic_matches_reference_fixture(ctx128, mode):
+    d = np.load(os.path.join(GOLD, "ref_n128.npz"))
+    nb = d["ic_fic"].size // 128
+    eps, _ = ctx128.emission_ic(d["ic_cre"], d["ic_fic"], nb, mode)
+    err = rel(eps, d["ic_eps"].reshape(eps.shape))
+    print(f"[emission] ic mode={mode} max rel err {err:.3e}", flush=True)
+    assert err < TOL[mode]
+
+
+@pytest.mark.parametrize("mode", list(TOL))
+def test_ic_many_tracers(ctx128, mode):
+    o = po.Oracle(128)
+    rng = np.random.default_rng(3)
+    nb = 64
+    fic = po.ic_kernel_flat(o, nb, 8.0, 28.0, 0.4)
+    nt = 777  # not a multiple of the 64-tracer tile
+    q = rng.uniform(2.0, 3.5, nt)
+    cut = 10 ** rng.uniform(2, 6, nt)
+    cre = (o.ep[None, :] ** -q[:, None]) * np.exp(-o.ep[None, :] / cut[:, None]) * 10 ** rng.uniform(-3, 3, nt)[:, None]
+    ref = po.emission_ic(128, cre.ravel(), o.edp, fic, nb).reshape(nt, nb)
+    eps, ms = ctx128.emission_ic(cre, fic, nb, mode)
+    err = rel(eps, ref)
+    print(f"[emission] ic 777x64 mode={mode} max rel err {err:.3e} ({ms:.3f} ms)", flush=True)
+    assert err < TOL[mode]
+
+
+@pytest.mark.parametrize("mode", list(TOL))
+def test_hadronic_matches_oracle(ctx128, mode):
+    o = po.Oracle(128)
+    rng = np.random.default_rng(4)
+    nb, emin, emax = 40, -1.0, 5.0
+    fk = po.gamma_kernel_flat(o, nb, emin, emax)
+    nt = 300
+    q = rng.uniform(2.0, 3.0, nt)
+    crp = (o.pp[None, :] ** -q[:, None]) * np.exp(-o.pp[None, :] / 1e6)
+    ng = 10 ** rng.uniform(-5, -2, nt)
+    ref = po.emission_hadronic(128, nb, emin, emax, ng, crp.ravel(), o.beta_p, o.pdp, fk).reshape(nt, nb)
+    eps, _ = ctx128.emission_hadronic(crp, ng, fk, nb, emin, emax, mode)
+    err = rel(eps, ref)
+    print(f"[emission] gamma mode={mode} max rel err {err:.3e}", flush=True)
+    assert err < TOL[mode]
+
+
+def test_synch_matches_oracle(ctx128):
+    d = np.load(os.path.join(GOLD, "ref_n128.npz"))
+    nfreq, _, nlogb = (int(v) for v in d["sy_dims"])
+    inv_dlogb = float(d["sy_inv_dlogb"][0])
+    o = po.Oracle(128)
+    rng = np.random.default_rng(5)
+    nt = 130
+    cre = (o.ep[None, :] ** -2.4) * np.exp(-o.ep[None, :] / 10 ** rng.uniform(3, 5, nt)[:, None])
+    cre[5, 70:] = 0.0  # truncated spectrum: the sum stops at the first empty successor
+    b = 10 ** rng.uniform(-8.5, -4.5, nt)
+    ref = po.emission_synch(128, nfreq, nlogb, -6.0, inv_dlogb, d["sy_table"], cre.ravel(), o.edp, b)
+    eps, _ = ctx128.emission_synch(cre, b, d["sy_table"], nfreq, nlogb, -6.0, inv_dlogb)
+    err = rel(eps, ref.reshape(eps.shape))
+    print(f"[emission] synch max rel err {err:.3e}", flush=True)
+    assert err < 1e-12

@@ -1,0 +1,100 @@
+"""Request loop (croma_hip_serve_*): results are the same whether a chunk runs
+alone or in a mix of chunks of other sizes, grids, species and D_pp models
+served concurrently; every result matches the CPU oracle (|hip - cpu| <=
+1e-30 + 1e-10 |cpu|); admission respects the HBM budget; cancellation."""
+import numpy as np
+import pytest
+
+import croma_hip as ch
+import run_oracle
+import serve_stream
+from workloads import rel_err
+
+pytestmark = pytest.mark.gpu
+
+MIX = [
+    dict(ntracer=96, n=128, species=ch.SPECIES_ELECTRON, dpp=ch.DPP_ASA, nsnap=3, seed=11, z0=0.4),
+    dict(ntracer=40, n=256, species=ch.SPECIES_ELECTRON_PROTON, dpp=ch.DPP_TTD, nsnap=2, seed=12, z0=0.8),
+    dict(ntracer=64, n=64, species=ch.SPECIES_PROTON, dpp=ch.DPP_OFF, nsnap=3, seed=13, z0=0.3),
+    dict(ntracer=24, n=512, species=ch.SPECIES_ELECTRON, dpp=ch.DPP_TTD, nsnap=2, seed=14, z0=0.6),
+    dict(ntracer=200, n=128, species=ch.SPECIES_ELECTRON, dpp=ch.DPP_OFF, nsnap=2, seed=15, z0=0.2),
+]
+
+
+def oracle_result(spec):
+    w, _ = serve_stream.make_chunk(spec)
+    return run_oracle.run_workload(w, run_oracle.oracle_params(w), s0=0, s1=spec["nsnap"])
+
+
+@pytest.fixture(scope="module")
+def server():
+    s = ch.Server(2 * 2 ** 30, serve_stream.server_params())
+    yield s
+    s.close()
+
+
+def test_alone_equals_mix_and_oracle(server):
+    alone = []
+    for spec in MIX:
+        print(f"[serve] alone {spec}", flush=True)
+        _, kw = serve_stream.make_chunk(spec)
+        rid = server.submit(**kw)
+        alone.append(server.wait(rid)[:2])
+    print("[serve] mix", flush=True)
+    ids = [server.submit(**serve_stream.make_chunk(spec)[1]) for spec in MIX]
+    mixed = [server.wait(i)[:2] for i in ids]
+    for spec, (pa, ea), (pm, em) in zip(MIX, alone, mixed):
+        assert np.array_equal(ea, em) and np.array_equal(pa, pm), f"alone != mix for {spec}"
+        crp_ref, cre_ref = oracle_result(spec)
+        e_err, e_ok = rel_err(em, cre_ref)
+        print(f"  n={spec['n']} species={spec['species']} electron max rel err {e_err:.3e}", flush=True)
+        assert e_ok
+        if spec["species"] != ch.SPECIES_ELECTRON:
+            p_err, p_ok = rel_err(pm, crp_ref)
+            print(f"    proton max rel err {p_err:.3e}", flush=True)
+            assert p_ok
+        else:
+            assert np.all(pm == 0.0)
+    st = server.stats()
+    assert st["completed"] == 2 * len(MIX) and st["failed"] == 0
+
+
+def test_budget_admission_and_rejection():
+    spec = dict(ntracer=512, n=128, species=ch.SPECIES_ELECTRON, dpp=ch.DPP_ASA, nsnap=2, seed=21, z0=0.5)
+    need = ch.chunk_bytes(512, 128, ch.SPECIES_ELECTRON, 2)
+    # room for one such chunk at a time: later chunks wait for memory
+    s = ch.Server(int(need * 1.5), serve_stream.server_params())
+    try:
+        _, kw = serve_stream.make_chunk(spec)
+        ids = [s.submit(**kw) for _ in range(3)]
+        outs = [s.wait(i) for i in ids]
+        for o in outs[1:]:
+            assert np.array_equal(o[1], outs[0][1])
+        st = s.stats()
+        assert st["peak_bytes"] <= st["budget_bytes"]
+        assert st["peak_bytes"] < 2 * need
+        big = dict(spec, ntracer=4096)
+        with pytest.raises(ch.CromaHipError) as ei:
+            s.submit(**serve_stream.make_chunk(big)[1])
+        assert ei.value.rc == ch.EBUDGET
+    finally:
+        s.close()
+
+
+def test_cancel_queued():
+    spec = dict(ntracer=2048, n=64, species=ch.SPECIES_ELECTRON, dpp=ch.DPP_OFF, nsnap=2, seed=31, z0=0.5)
+    need = ch.chunk_bytes(2048, 64, ch.SPECIES_ELECTRON, 2)
+    # one chunk at a time: the last submission is still queued when cancelled
+    s = ch.Server(int(need * 1.5), serve_stream.server_params())
+    try:
+        _, kw = serve_stream.make_chunk(spec)
+        ids = [s.submit(**kw) for _ in range(4)]
+        s.cancel(ids[-1])
+        with pytest.raises(ch.CromaHipError) as ei:
+            s.wait(ids[-1])
+        assert ei.value.rc == ch.ECANCELED
+        for i in ids[:-1]:
+            s.wait(i)
+        assert s.stats()["cancelled"] == 1
+    finally:
+        s.close()
