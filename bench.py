@@ -129,6 +129,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--no-tune", action="store_true", help="keep the default tile instead of autotuning it")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -148,6 +149,9 @@ def main():
     w = synth.make_workload(seed=2026, **cfg)
     steps = min(args.steps, w.nsnap - args.warmup)
     ctx = ch.Context(w.n, hip_params(w))
+    t_tune = time.perf_counter()
+    tile = ctx.autotune(w.protons, ntracer=min(w.ntracer, 16384)) if not args.no_tune else ctx.get_tile(w.protons)
+    t_tune = time.perf_counter() - t_tune
 
     wall, t = run_path(ch, ctx, w, ch.PATH_FUSED, args.warmup, steps)
     value = t.cell_steps / wall
@@ -170,7 +174,8 @@ def main():
                                f"{'electron_proton' if w.protons else 'electron_only'}, "
                                f"dpp={['', 'asa', 'ttd', 'off'][w.dpp_model]}, {w.nsnap} snapshots z=1->0",
                    "ntracer": w.ntracer, "n": w.n, "species": species, "snapshots_timed": steps,
-                   "tile": ctx.get_tile(w.protons)},
+                   "tile": {"rows_per_lane": tile[0], "waves_per_simd": tile[1], "snapshots_per_task": tile[2],
+                            "autotune_s": round(t_tune, 2)}},
         "tracer_substeps": int(t.cell_steps),
         "species_substeps": int(t.species_steps),
         "host_ms": {"ic_rows": t.ic_rows_ms, "nsub": t.nsub_estimate_ms, "order": t.bucket_build_ms},
