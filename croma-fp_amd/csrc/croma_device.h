@@ -338,6 +338,23 @@ struct LogG {
     static constexpr int v = (N <= 1) ? 0 : 1 + LogG<N / 2>::v;
 };
 
+// value of lane L (a wave-uniform lane index) as a scalar
+__device__ __forceinline__ double rdlane64(double v, int L)
+{
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), L);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), L);
+    return __hiloint2double(hi, lo);
+}
+
+// lane l - S / l + S inside the 16-lane row (0 where that lane is outside the row)
+template <int S>
+__device__ __forceinline__ double row_up(double v) { return dpp64<0x110 + S>(v); }
+template <int S>
+__device__ __forceinline__ double row_dn(double v) { return dpp64<0x100 + S>(v); }
+// lane l - 1 / l + 1 across the whole wave (wave_shr:1 / wave_shl:1)
+__device__ __forceinline__ double wave_up1(double v) { return dpp64<0x138>(v); }
+__device__ __forceinline__ double wave_dn1(double v) { return dpp64<0x130>(v); }
+
 // Factorised Chang-Cooper operator of one species for one coefficient run,
 // held in registers.  One implicit substep is x <- M^{-1} (x + src):
 //   forward   d_i = al_i d_{i-1} + iv_i x_i + vs_i
@@ -346,13 +363,36 @@ struct LogG {
 // resp. g over lane ranges, zero where the source lane is outside the group).
 template <int R, int G>
 struct Factor {
+    static constexpr int LV = LogG<G>::v < 4 ? LogG<G>::v : 4;  // scan levels inside a 16-lane row
     double al[R];   // a_i / denom_i
     double iv[R];   // 1 / denom_i
     double vs[R];   // src_i / denom_i
     double g[R];    // c_i / denom_i (= -cprime_i of the reference)
-    double Af[LogG<G>::v + 1];
-    double Qb[LogG<G>::v + 1];
+    double Af[LV + 1];  // forward scan multipliers, zero where the source lane is outside the row / group
+    double Qb[LV + 1];  // backward scan multipliers
+    double PA, PQ;      // groups wider than a row: in-row inclusive products (prefix of A, suffix of Q)
+    double Tf[4], Tb[4];  // and the row totals of those products (uniform across the wave)
 };
+
+// Multipliers of the in-row scan levels (and the running in-row products).
+template <int G, int ST, int R>
+__device__ __forceinline__ void scan_coeffs(int gl, int rl, double &A, double &Bq, Factor<R, G> &F)
+{
+    if constexpr (ST < Factor<R, G>::LV) {
+        constexpr int s = 1 << ST;
+        const double Ao = row_up<s>(A);
+        const double Bo = row_dn<s>(Bq);
+        // a source lane outside the row (or, for groups narrower than a row,
+        // outside the group) contributes nothing
+        const bool okf = (G >= 16) ? (rl >= s) : (gl >= s);
+        const bool okb = (G >= 16) ? (rl + s <= 15) : (gl + s < G);
+        F.Af[ST] = okf ? A : 0.0;
+        F.Qb[ST] = okb ? Bq : 0.0;
+        if (okf) A = A * Ao;
+        if (okb) Bq = Bq * Bo;
+        scan_coeffs<G, ST + 1, R>(gl, rl, A, Bq, F);
+    }
+}
 
 // Thomas LU of the rows held in F (on entry al = a, iv = b, g = c), as in
 // fp_thomas_solve_ws (fp_shared_core.c:226-257): denom_i = b_i + a_i cprime_{i-1},
@@ -410,33 +450,39 @@ __device__ __forceinline__ void factorize(int gl, int n, const double (&src)[R],
         Bq *= F.g[k];
         cprev = cp;
     }
-    int st = 0;
+    // Scan multipliers of the substep solve.  Hillis-Steele levels run
+    // inside 16-lane rows (DPP); a group spanning several rows completes the
+    // scan with per-row carries (see apply_solve).
+    const int rl = (int)(threadIdx.x & 15);
+    scan_coeffs<G, 0>(gl, rl, A, Bq, F);
+    F.PA = A;
+    F.PQ = Bq;
+    if constexpr (G > 16) {
 #pragma unroll
-    for (int s = 1; s < G; s <<= 1, ++st) {
-        const double Ao = g_up<G>(A, s);
-        const double Bo = g_down<G>(Bq, s);
-        F.Af[st] = (gl >= s) ? A : 0.0;
-        F.Qb[st] = (gl + s < G) ? Bq : 0.0;
-        if (gl >= s) A = A * Ao;
-        if (gl + s < G) Bq = Bq * Bo;
+        for (int r = 0; r < 4; ++r) {
+            F.Tf[r] = rdlane64(A, 16 * r + 15);
+            F.Tb[r] = rdlane64(Bq, 16 * r);
+        }
     }
 }
 
-template <int G, int S, int ST>
-__device__ __forceinline__ void scan_fwd(const double (&Af)[LogG<G>::v + 1], double &B)
+// In-row Hillis-Steele levels of the forward (prefix) and backward (suffix)
+// affine scans: B_l += mult_l * B_{l -/+ s}.
+template <int G, int ST, int LV>
+__device__ __forceinline__ void scan_fwd(const double *Af, double &B)
 {
-    if constexpr (S < G) {
-        B = fma(Af[ST], sh_up<G, S>(B), B);
-        scan_fwd<G, 2 * S, ST + 1>(Af, B);
+    if constexpr (ST < LV) {
+        B = fma(Af[ST], row_up<(1 << ST)>(B), B);
+        scan_fwd<G, ST + 1, LV>(Af, B);
     }
 }
 
-template <int G, int S, int ST>
-__device__ __forceinline__ void scan_bwd(const double (&Qb)[LogG<G>::v + 1], double &B)
+template <int G, int ST, int LV>
+__device__ __forceinline__ void scan_bwd(const double *Qb, double &B)
 {
-    if constexpr (S < G) {
-        B = fma(Qb[ST], sh_down<G, S>(B), B);
-        scan_bwd<G, 2 * S, ST + 1>(Qb, B);
+    if constexpr (ST < LV) {
+        B = fma(Qb[ST], row_dn<(1 << ST)>(B), B);
+        scan_bwd<G, ST + 1, LV>(Qb, B);
     }
 }
 
@@ -451,8 +497,21 @@ __device__ __forceinline__ void apply_solve(int gl, const Factor<R, G> &F, doubl
         t[k] = fma(F.iv[k], x[k], F.vs[k]);
         L = fma(F.al[k], L, t[k]);
     }
-    scan_fwd<G, 1, 0>(F.Af, L);
-    double d = sh_up1<G>(L);
+    scan_fwd<G, 0, Factor<R, G>::LV>(F.Af, L);
+    if constexpr (G > 16) {
+        // carries between the rows of a group: the running value entering
+        // row r is T_{r-1} + Tf_{r-1} C_{r-1} (T: inclusive in-row value at
+        // the row's last lane); rows starting a group take none
+        constexpr int RG = G / 16;
+        const double T0 = rdlane64(L, 15), T1 = rdlane64(L, 31), T2 = rdlane64(L, 47);
+        const double C1 = (1 % RG == 0) ? 0.0 : T0;
+        const double C2 = (2 % RG == 0) ? 0.0 : fma(F.Tf[1], C1, T1);
+        const double C3 = (3 % RG == 0) ? 0.0 : fma(F.Tf[2], C2, T2);
+        const int row = (int)((threadIdx.x & 63) >> 4);
+        const double C = (row == 1) ? C1 : (row == 2) ? C2 : (row == 3) ? C3 : 0.0;
+        L = fma(F.PA, C, L);
+    }
+    double d = wave_up1(L);
     if (gl == 0) d = 0.0;
 #pragma unroll
     for (int k = 0; k < R; ++k) {
@@ -462,8 +521,18 @@ __device__ __forceinline__ void apply_solve(int gl, const Factor<R, G> &F, doubl
     double M = 0.0;
 #pragma unroll
     for (int k = R - 1; k >= 0; --k) M = fma(F.g[k], M, t[k]);
-    scan_bwd<G, 1, 0>(F.Qb, M);
-    double xn = sh_down1<G>(M);
+    scan_bwd<G, 0, Factor<R, G>::LV>(F.Qb, M);
+    if constexpr (G > 16) {
+        constexpr int RG = G / 16;
+        const double U1 = rdlane64(M, 16), U2 = rdlane64(M, 32), U3 = rdlane64(M, 48);
+        const double D2 = (3 % RG == 0) ? 0.0 : U3;
+        const double D1 = (2 % RG == 0) ? 0.0 : fma(F.Tb[2], D2, U2);
+        const double D0 = (1 % RG == 0) ? 0.0 : fma(F.Tb[1], D1, U1);
+        const int row = (int)((threadIdx.x & 63) >> 4);
+        const double D = (row == 0) ? D0 : (row == 1) ? D1 : (row == 2) ? D2 : 0.0;
+        M = fma(F.PQ, D, M);
+    }
+    double xn = wave_dn1(M);
     if (gl == G - 1) xn = 0.0;
 #pragma unroll
     for (int k = R - 1; k >= 0; --k) {
