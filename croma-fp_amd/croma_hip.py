@@ -50,7 +50,8 @@ class Params(C.Structure):
 
 
 class Tile(C.Structure):
-    _fields_ = [("rows_per_lane", C.c_int), ("waves_per_simd", C.c_int), ("chain", C.c_int)]
+    _fields_ = [("rows_per_lane", C.c_int), ("waves_per_simd", C.c_int), ("chain", C.c_int),
+                ("secondary_gemm", C.c_int)]
 
 
 class Times(C.Structure):
@@ -177,20 +178,20 @@ class Context:
         self.params = params
         _chk(lib().croma_hip_context_set_params(C.c_void_p(self.h), C.byref(params)), "set_params")
 
-    def set_tile(self, rows_per_lane, waves_per_simd, chain=1, protons=-1):
+    def set_tile(self, rows_per_lane, waves_per_simd, chain=1, protons=-1, secondary_gemm=1):
         """protons: 0 electron_only tile, 1 electron_proton tile, -1 both."""
-        t = Tile(rows_per_lane, waves_per_simd, chain)
+        t = Tile(rows_per_lane, waves_per_simd, chain, secondary_gemm)
         _chk(lib().croma_hip_context_set_tile(C.c_void_p(self.h), int(protons), C.byref(t)), "set_tile")
 
     def get_tile(self, protons=False):
         t = Tile()
         _chk(lib().croma_hip_context_get_tile(C.c_void_p(self.h), int(protons), C.byref(t)), "get_tile")
-        return (t.rows_per_lane, t.waves_per_simd, t.chain)
+        return (t.rows_per_lane, t.waves_per_simd, t.chain, t.secondary_gemm)
 
     def autotune(self, protons=False, ntracer=16384):
         t = Tile()
         _chk(lib().croma_hip_autotune(C.c_void_p(self.h), int(protons), int(ntracer), C.byref(t)), "autotune")
-        return (t.rows_per_lane, t.waves_per_simd, t.chain)
+        return (t.rows_per_lane, t.waves_per_simd, t.chain, t.secondary_gemm)
 
     def grid(self, species):
         n = self.n

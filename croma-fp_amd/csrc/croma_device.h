@@ -783,11 +783,20 @@ __device__ __forceinline__ void species_segment(int gl, const CkGrid &g, int ste
 // Within a segment the proton and electron updates are independent (the
 // electron source uses the segment-start protons), so the species run one
 // after the other and only one factor set is live.
-template <int N, int R, bool PROT>
+// Launch modes of the fused kernel.
+enum { CK_MODE_E = 0,      // electron_only: protons are exactly zero
+       CK_MODE_EP = 1,     // both species, secondaries contracted in the kernel
+       CK_MODE_P_REC = 2,  // protons only; records the segment-start proton rows
+       CK_MODE_E_SEC = 3   // electrons only; secondaries from the contracted rows
+};
+
+template <int N, int R, int MODE>
 __device__ __forceinline__ void snapshot_update(const CkGrid &g, const CkSlot &S, int t, int gl, double *s_row,
                                                 double (&xe)[R], double (&xp)[R], int &bad)
 {
     constexpr int G = N / R;
+    constexpr bool PROT = (MODE == CK_MODE_EP || MODE == CK_MODE_P_REC);
+    constexpr bool ELEC = (MODE != CK_MODE_P_REC);
     const int base = gl * R;
     const CkParams &prm = S.prm;
     const int nsub = S.nsub[t];
@@ -800,7 +809,8 @@ __device__ __forceinline__ void snapshot_update(const CkGrid &g, const CkSlot &S
     const size_t roff = (size_t)t * N + base;
     const double mass = S.mass ? S.mass[t] : 0.0;
     const bool dis = S.dis ? (S.dis[t] != 0) : false;
-    const bool sec = PROT && S.fqe_t;
+    const bool sec = (MODE == CK_MODE_EP) && S.fqe_t;
+    const int row0 = (MODE == CK_MODE_P_REC || MODE == CK_MODE_E_SEC) ? S.seg_off[t] : 0;
 
     for (int iseg = 0; iseg < nseg; ++iseg) {
         const int fb = (int)(((long long)iseg * nsub) / nseg);
@@ -869,6 +879,11 @@ __device__ __forceinline__ void snapshot_update(const CkGrid &g, const CkSlot &S
 #pragma unroll
                 for (int k = 0; k < R; ++k) s_row[base + k] = xp[k];
             }
+            if (MODE == CK_MODE_P_REC) {
+                double *rec = S.seg_rec + (size_t)(row0 + iseg) * N + base;
+#pragma unroll
+                for (int k = 0; k < R; ++k) rec[k] = xp[k];
+            }
             double radp[R], src[R];
             seg_radp<false, R>(g, S, base, nseg, alpha, q, radp);
 #pragma unroll
@@ -878,10 +893,17 @@ __device__ __forceinline__ void snapshot_update(const CkGrid &g, const CkSlot &S
             species_segment<false, N, R, G>(gl, g, steps, windowed, ls, le, rp, dfac_on, dfac_off, mode_on, radp, src,
                                             xp, bad);
         }
-        {
+        if (ELEC) {
             double src[R];
 #pragma unroll
             for (int k = 0; k < R; ++k) src[k] = S.qepri ? S.qepri[roff + k] : 0.0;
+            if (MODE == CK_MODE_E_SEC) {
+                // the same source as below, with the proton contraction done
+                // for all tracers at once outside the kernel
+                const double *inj = S.seg_inje + (size_t)(row0 + iseg) * N + base;
+#pragma unroll
+                for (int k = 0; k < R; ++k) src[k] = inj[k] * K_GYR * q.n_gas + src[k];
+            }
             if (sec) {
                 // inje = Gyr n_gas sum_p N_p fqe[p, e] + qepri (tracer_fp_cpu.c:142-147)
                 double acc[R];
@@ -910,20 +932,23 @@ __device__ __forceinline__ void snapshot_update(const CkGrid &g, const CkSlot &S
     }
 }
 
-template <int N, int R, int WPB, bool PROT>
+template <int N, int R, int WPB, int MODE>
 __device__ __forceinline__ void k_fused_body(const CkGrid &g, const CkSlot *__restrict__ slots,
                                              const CkTask *__restrict__ tasks, int ntasks,
                                              int *__restrict__ counter, int *__restrict__ err)
 {
+    constexpr bool PROT = (MODE == CK_MODE_EP || MODE == CK_MODE_P_REC);
+    constexpr bool ELEC = (MODE != CK_MODE_P_REC);
+    constexpr bool SEC_LDS = (MODE == CK_MODE_EP);
     constexpr int G = N / R;
     constexpr int GPW = 64 / G;
-    __shared__ double s_crp[PROT ? WPB : 1][PROT ? GPW : 1][PROT ? N : 1];
+    __shared__ double s_crp[SEC_LDS ? WPB : 1][SEC_LDS ? GPW : 1][SEC_LDS ? N : 1];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int grp = lane / G;
     const int gl = lane % G;
     const int base = gl * R;
-    double *s_row = &s_crp[PROT ? wave : 0][PROT ? grp : 0][0];
+    double *s_row = &s_crp[SEC_LDS ? wave : 0][SEC_LDS ? grp : 0][0];
     int bad = 0;
 
     for (;;) {
@@ -941,14 +966,14 @@ __device__ __forceinline__ void k_fused_body(const CkGrid &g, const CkSlot *__re
             double xe[R], xp[R];
 #pragma unroll
             for (int k = 0; k < R; ++k) {
-                xe[k] = S0.cre[roff + k];
+                xe[k] = ELEC ? S0.cre[roff + k] : 0.0;
                 xp[k] = PROT ? S0.crp[roff + k] : 0.0;
             }
             // a task advances one tracer over nslot consecutive snapshots
             for (int sl = task.slot; sl < task.slot + task.nslot; ++sl) {
                 const CkSlot &S = slots[sl];
-                snapshot_update<N, R, PROT>(g, S, t, gl, s_row, xe, xp, bad);
-                if (S.out_e) {
+                snapshot_update<N, R, MODE>(g, S, t, gl, s_row, xe, xp, bad);
+                if (ELEC && S.out_e) {
 #pragma unroll
                     for (int k = 0; k < R; ++k) S.out_e[roff + k] = xe[k];
                 }
@@ -960,7 +985,7 @@ __device__ __forceinline__ void k_fused_body(const CkGrid &g, const CkSlot *__re
 #pragma unroll
             for (int k = 0; k < R; ++k) {
                 if (!isfinite(xe[k]) || !isfinite(xp[k])) bad |= ERR_NONFINITE;
-                S0.cre[roff + k] = xe[k];
+                if (ELEC) S0.cre[roff + k] = xe[k];
                 if (PROT) S0.crp[roff + k] = xp[k];
             }
         }
@@ -971,10 +996,10 @@ __device__ __forceinline__ void k_fused_body(const CkGrid &g, const CkSlot *__re
 // The kernel: WPB waves per block, register budget set for OCC waves per SIMD
 // (the solve is latency-bound, so the budget trades unrolled state against
 // waves available to hide cross-lane and memory latency).
-template <int N, int R, int WPB, bool PROT, int OCC>
+template <int N, int R, int WPB, int MODE, int OCC>
 __global__ __launch_bounds__(WPB * 64) __attribute__((amdgpu_waves_per_eu(OCC, 8))) void k_fused(
     CkGrid g, const CkSlot *__restrict__ slots, const CkTask *__restrict__ tasks, int ntasks,
     int *__restrict__ counter, int *__restrict__ err)
 {
-    k_fused_body<N, R, WPB, PROT>(g, slots, tasks, ntasks, counter, err);
+    k_fused_body<N, R, WPB, MODE>(g, slots, tasks, ntasks, counter, err);
 }

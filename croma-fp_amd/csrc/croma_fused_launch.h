@@ -6,27 +6,35 @@
 
 #define CK_FUSED_WPB 4
 
-template <int N, int R, bool PROT, int OCC>
+template <int N, int R, int MODE, int OCC>
 static int ck_launch_inst(int nblocks, const CkGrid &grid, const CkSlot *slots, const CkTask *tasks, int ntasks,
                           int *counter, int *err, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_fused<N, R, CK_FUSED_WPB, PROT, OCC>), dim3(nblocks), dim3(CK_FUSED_WPB * 64), 0, st,
+    hipLaunchKernelGGL((k_fused<N, R, CK_FUSED_WPB, MODE, OCC>), dim3(nblocks), dim3(CK_FUSED_WPB * 64), 0, st,
                        grid, slots, tasks, ntasks, counter, err);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 template <int N, int R>
-static int ck_launch_r(int occ, int protons, int nblocks, const CkGrid &grid, const CkSlot *slots,
+static int ck_launch_r(int occ, int mode, int nblocks, const CkGrid &grid, const CkSlot *slots,
                        const CkTask *tasks, int ntasks, int *counter, int *err, hipStream_t st)
 {
+#define CK_MODE_CASE(M, O)                                                                                             \
+    case M: return ck_launch_inst<N, R, M, O>(nblocks, grid, slots, tasks, ntasks, counter, err, st);
 #define CK_OCC_CASE(O)                                                                                                 \
     case O:                                                                                                            \
-        return protons ? ck_launch_inst<N, R, true, O>(nblocks, grid, slots, tasks, ntasks, counter, err, st)          \
-                       : ck_launch_inst<N, R, false, O>(nblocks, grid, slots, tasks, ntasks, counter, err, st);
+        switch (mode) {                                                                                                \
+            CK_MODE_CASE(0, O)                                                                                         \
+            CK_MODE_CASE(1, O)                                                                                         \
+            CK_MODE_CASE(2, O)                                                                                         \
+            CK_MODE_CASE(3, O)                                                                                         \
+        default: return -1;                                                                                            \
+        }
     switch (occ) {
         CK_OCC_CASE(1)
         CK_OCC_CASE(2)
     default: return -1;
     }
 #undef CK_OCC_CASE
+#undef CK_MODE_CASE
 }

@@ -1,16 +1,16 @@
 // croma_fused_n128.hip -- fused persistent kernel instances for n = 128
-// (rows per lane: 2 4 8 16; 1 or 2 waves per SIMD; electron-only and electron+proton).
+// (rows per lane: 2 4 8 16; 1 or 2 waves per SIMD; all launch modes).
 #include "croma_fused_launch.h"
 
-extern "C" int ck_launch_fused_n128(int r, int occ, int nblocks, int protons, CkGrid grid, const CkSlot *slots,
+extern "C" int ck_launch_fused_n128(int r, int occ, int nblocks, int mode, CkGrid grid, const CkSlot *slots,
                                    const CkTask *tasks, int ntasks, int *counter, int *err, void *stream)
 {
     hipStream_t st = (hipStream_t)stream;
     switch (r) {
-    case 2: return ck_launch_r<128, 2>(occ, protons, nblocks, grid, slots, tasks, ntasks, counter, err, st);
-    case 4: return ck_launch_r<128, 4>(occ, protons, nblocks, grid, slots, tasks, ntasks, counter, err, st);
-    case 8: return ck_launch_r<128, 8>(occ, protons, nblocks, grid, slots, tasks, ntasks, counter, err, st);
-    case 16: return ck_launch_r<128, 16>(occ, protons, nblocks, grid, slots, tasks, ntasks, counter, err, st);
+    case 2: return ck_launch_r<128, 2>(occ, mode, nblocks, grid, slots, tasks, ntasks, counter, err, st);
+    case 4: return ck_launch_r<128, 4>(occ, mode, nblocks, grid, slots, tasks, ntasks, counter, err, st);
+    case 8: return ck_launch_r<128, 8>(occ, mode, nblocks, grid, slots, tasks, ntasks, counter, err, st);
+    case 16: return ck_launch_r<128, 16>(occ, mode, nblocks, grid, slots, tasks, ntasks, counter, err, st);
     default: return -1;
     }
 }

@@ -74,6 +74,12 @@ typedef struct {
     double elapsed;
     double *out_e;                /* optional post-snapshot electron state [ntr * n] */
     double *out_p;                /* optional post-snapshot proton state [ntr * n] */
+    /* split launches (modes 2 and 3): segment rows, tracer t's segment s at
+     * row seg_off[t] + s; seg_rec [rows][n] segment-start protons (written
+     * by mode 2), seg_inje [rows][n] = sum_p seg_rec[row][p] fqe_t[p][e] */
+    double *seg_rec;
+    const double *seg_inje;
+    const int *seg_off;
     CkParams prm;
 } CkSlot;
 
@@ -119,6 +125,13 @@ void *ck_graph_end(void *stream);          /* returns executable graph or NULL *
 int ck_graph_launch(void *graph_exec, void *stream);
 int ck_graph_destroy(void *graph_exec);
 
+/* FP64 GEMM through the vendor library (plain GEMM of the split
+ * secondary-source contraction): C = A B, column-major. */
+void *ck_blas_create(void *stream);
+void ck_blas_destroy(void *handle);
+int ck_dgemm_nn(void *handle, int m, int n, int k, const double *A, int lda, const double *B, int ldb, double *C,
+                int ldc);
+
 /* Inverse-Compton cooling rows on the device: for each of nz redshifts,
  * zb[3 iz .. 3 iz + 2] = {z, e0min, e0max} (CMB bounds from the host),
  * rows[iz * n + j] for the n electron bins and ghosts[2 iz], ghosts[2 iz + 1]
@@ -135,9 +148,11 @@ int ck_fill_f64(double *dst, double value, size_t count, void *stream);
  * 2, 4, 8 or 16 (n / rows_per_lane lanes per tracer, 4..64); occupancy 1 or 2
  * waves per SIMD selects the register budget (launch cu_count * occupancy
  * blocks of 4 waves for one resident wave per SIMD and occupancy step).
- * protons selects the two-species variant; every slot of one launch must
- * agree.  The counter must be zero on entry. */
-int ck_launch_fused(int n, int rows_per_lane, int occupancy, int nblocks, int protons,
+ * mode: 0 electron_only, 1 both species (secondaries contracted in the
+ * kernel), 2 protons only recording segment-start rows, 3 electrons only
+ * with secondaries from seg_inje; every slot of one launch must agree.  The
+ * counter must be zero on entry. */
+int ck_launch_fused(int n, int rows_per_lane, int occupancy, int nblocks, int mode,
                     CkGrid grid, const CkSlot *slots_dev, const CkTask *tasks_dev, int ntasks,
                     int *counter_dev, int *err_dev, void *stream);
 int ck_fused_supported(int n, int rows_per_lane);

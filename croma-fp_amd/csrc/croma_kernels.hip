@@ -52,18 +52,18 @@ extern "C" int ck_fused_supported(int n, int r)
     return 1;
 }
 
-extern "C" int ck_launch_fused(int n, int r, int occ, int nblocks, int protons, CkGrid grid, const CkSlot *slots,
+extern "C" int ck_launch_fused(int n, int r, int occ, int nblocks, int mode, CkGrid grid, const CkSlot *slots,
                                const CkTask *tasks, int ntasks, int *counter, int *err, void *stream)
 {
     int rc = -1;
     if (!ck_fused_supported(n, r) || ntasks < 0 || nblocks <= 0) return -1;
-    if (occ < 1 || occ > 2) return -1;
+    if (occ < 1 || occ > 2 || mode < 0 || mode > 3) return -1;
     if (ntasks == 0) return 0;
     switch (n) {
-    case 64: rc = ck_launch_fused_n64(r, occ, nblocks, protons, grid, slots, tasks, ntasks, counter, err, stream); break;
-    case 128: rc = ck_launch_fused_n128(r, occ, nblocks, protons, grid, slots, tasks, ntasks, counter, err, stream); break;
-    case 256: rc = ck_launch_fused_n256(r, occ, nblocks, protons, grid, slots, tasks, ntasks, counter, err, stream); break;
-    case 512: rc = ck_launch_fused_n512(r, occ, nblocks, protons, grid, slots, tasks, ntasks, counter, err, stream); break;
+    case 64: rc = ck_launch_fused_n64(r, occ, nblocks, mode, grid, slots, tasks, ntasks, counter, err, stream); break;
+    case 128: rc = ck_launch_fused_n128(r, occ, nblocks, mode, grid, slots, tasks, ntasks, counter, err, stream); break;
+    case 256: rc = ck_launch_fused_n256(r, occ, nblocks, mode, grid, slots, tasks, ntasks, counter, err, stream); break;
+    case 512: rc = ck_launch_fused_n512(r, occ, nblocks, mode, grid, slots, tasks, ntasks, counter, err, stream); break;
     }
     if (rc) hip_ok(hipGetLastError(), "k_fused launch");
     return rc;
@@ -513,4 +513,42 @@ extern "C" int ck_graph_launch(void *ex, void *s)
 extern "C" int ck_graph_destroy(void *ex)
 {
     return ex ? hip_ok(hipGraphExecDestroy((hipGraphExec_t)ex), "hipGraphExecDestroy") : 0;
+}
+
+// ---------------------------------------------------------------------------
+// Library GEMM for the secondary-source contraction of the split launches.
+// ---------------------------------------------------------------------------
+#include <rocblas/rocblas.h>
+
+extern "C" void *ck_blas_create(void *stream)
+{
+    rocblas_handle h = 0;
+    if (rocblas_create_handle(&h) != rocblas_status_success) return 0;
+    if (rocblas_set_stream(h, (hipStream_t)stream) != rocblas_status_success ||
+        rocblas_set_pointer_mode(h, rocblas_pointer_mode_host) != rocblas_status_success) {
+        rocblas_destroy_handle(h);
+        return 0;
+    }
+    return (void *)h;
+}
+
+extern "C" void ck_blas_destroy(void *h)
+{
+    if (h) rocblas_destroy_handle((rocblas_handle)h);
+}
+
+// C (m x n, column-major, ldc) = A (m x k, lda) * B (k x n, ldb), FP64.
+extern "C" int ck_dgemm_nn(void *h, int m, int n, int k, const double *A, int lda, const double *B, int ldb,
+                           double *Cm, int ldc)
+{
+    const double one = 1.0, zero = 0.0;
+    if (!h || m <= 0 || k <= 0 || n < 0) return -1;
+    if (n == 0) return 0;
+    const rocblas_status st = rocblas_dgemm((rocblas_handle)h, rocblas_operation_none, rocblas_operation_none, m, n,
+                                            k, &one, A, lda, B, ldb, &zero, Cm, ldc);
+    if (st != rocblas_status_success) {
+        snprintf(g_errbuf, sizeof(g_errbuf), "rocblas_dgemm: status %d", (int)st);
+        return -1;
+    }
+    return 0;
 }

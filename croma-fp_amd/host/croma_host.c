@@ -165,6 +165,7 @@ struct CromaHipContext {
     double *h_fqe;
     int *h_npmin;
     double *d_fqe;
+    void *blas; /* vendor GEMM handle for the split secondary contraction */
     /* plain-path workspace */
     CkPlainWs pws;
     void *pws_mem;
@@ -248,18 +249,21 @@ static void default_tiles(CromaHipContext *c)
     const int n = c->g.n;
     /* rows per lane: the spectrum and one factor set stay in registers;
      * two waves per SIMD hide the cross-lane latency of the substep scans */
-    c->tile[0].rows_per_lane = (n >= 256) ? 8 : 4;
-    c->tile[0].waves_per_simd = 2;
+    /* measured defaults (the autotuner refines them per workload) */
+    c->tile[0].rows_per_lane = (n == 64) ? 4 : (n == 128) ? 8 : (n == 256) ? 8 : 16;
+    c->tile[0].waves_per_simd = (n <= 128) ? 1 : 2;
     c->tile[0].chain = 1;
-    c->tile[1].rows_per_lane = (n >= 256) ? 8 : 4;
-    c->tile[1].waves_per_simd = 2;
+    c->tile[0].secondary_gemm = 0;
+    c->tile[1].rows_per_lane = (n <= 256) ? 4 : 8;
+    c->tile[1].waves_per_simd = (n == 64) ? 1 : 2;
     c->tile[1].chain = 1;
+    c->tile[1].secondary_gemm = 1;
 }
 
 static int tile_ok(int n, const CromaHipTile *t)
 {
     return ck_fused_supported(n, t->rows_per_lane) && t->waves_per_simd >= 1 && t->waves_per_simd <= 2 &&
-           t->chain >= 1 && t->chain <= 4096;
+           t->chain >= 1 && t->chain <= 4096 && (t->secondary_gemm == 0 || t->secondary_gemm == 1);
 }
 
 CromaHipContext *croma_hip_context_create(const CromaHipGridSpec *gs, const CromaHipParams *prm)
@@ -321,6 +325,7 @@ void croma_hip_context_destroy(CromaHipContext *c)
     ck_free(c->d_err);
     ck_free(c->d_fqe);
     ck_free(c->pws_mem);
+    ck_blas_destroy(c->blas);
     free(c->h_fqe);
     free(c->h_npmin);
     if (c->stream) ck_stream_destroy(c->stream);
@@ -576,6 +581,9 @@ typedef struct {
     double *h_zb;            /* pinned [cap + 1][3] */
     int count;               /* snapshots prepared */
     int nz;                  /* IC rows to build */
+    int *d_segoff, *h_segoff; /* split launches: [cap][ntr] first segment row per tracer */
+    long segrows;            /* segment rows of the window */
+    int split;               /* the window runs as proton pass + GEMM + electron pass */
     void *ev_up;
 } WinBuf;
 
@@ -590,6 +598,8 @@ struct CromaHipBatch {
     double *d_src; /* optional [nsnap_res][2][ntr*n] per-snapshot sources */
     unsigned char *src_set;
     WinBuf win[2];
+    double *d_segrec, *d_seginje; /* split launches: [segrows_cap][n] */
+    long segrows_cap;
     int *hist, *keys;
     void *ev_a, *ev_b;
     void **ev_k; /* per-launch timing events, 2 per window */
@@ -607,6 +617,8 @@ static void win_free(WinBuf *w)
     ck_free(w->d_slot);
     ck_free(w->d_ric);
     ck_free(w->d_zb);
+    ck_free(w->d_segoff);
+    ck_host_free(w->h_segoff);
     ck_host_free(w->h_nsub);
     ck_host_free(w->h_non);
     ck_host_free(w->h_tasks);
@@ -633,9 +645,11 @@ static int win_reserve(WinBuf *w, int cap, int ntr, int n, void *stream)
     w->h_tasks = (CkTask *)ck_host_alloc((size_t)ntr * sizeof(CkTask));
     w->h_slot = (CkSlot *)ck_host_alloc((size_t)cap * sizeof(CkSlot));
     w->h_zb = (double *)ck_host_alloc((size_t)(cap + 1) * 3 * sizeof(double));
+    w->d_segoff = (int *)ck_malloc((size_t)cap * ntr * sizeof(int));
+    w->h_segoff = (int *)ck_host_alloc((size_t)cap * ntr * sizeof(int));
     w->ev_up = ck_event_create();
     if (!w->d_nsub || !w->d_non || !w->d_tasks || !w->d_slot || !w->d_ric || !w->d_zb || !w->h_nsub ||
-        !w->h_non || !w->h_tasks || !w->h_slot || !w->h_zb || !w->ev_up) {
+        !w->h_non || !w->h_tasks || !w->h_slot || !w->h_zb || !w->ev_up || !w->d_segoff || !w->h_segoff) {
         win_free(w);
         return dev_fail("window buffers");
     }
@@ -657,6 +671,8 @@ void croma_hip_batch_destroy(CromaHipBatch *b)
     ck_free(b->d_bg);
     ck_free(b->d_src);
     ck_free(b->d_idx);
+    ck_free(b->d_segrec);
+    ck_free(b->d_seginje);
     free(b->h_bg);
     free(b->src_set);
     free(b->hist);
@@ -834,7 +850,7 @@ typedef struct {
 } PrepStats;
 
 static int prepare_window(CromaHipBatch *b, WinBuf *w, int s_first, int count, int nsnap_total, const double *z,
-                          const double *dt_snap, double elapsed, PrepStats *ps)
+                          const double *dt_snap, double elapsed, int split, PrepStats *ps)
 {
     CromaHipContext *c = b->ctx;
     const int n = c->g.n, ntr = b->ntr;
@@ -847,6 +863,8 @@ static int prepare_window(CromaHipBatch *b, WinBuf *w, int s_first, int count, i
         allow_v = (int *)malloc((size_t)count * sizeof(int));
         if (!allow_v) return CROMA_HIP_ENOMEM;
     }
+    w->split = split;
+    w->segrows = 0;
     /* IC rows at z_first .. z_last (+ z_last+1 when the last snapshot has a
      * successor) are built on the device; the host only evaluates the two
      * end bins nsub needs (same arithmetic as the reference row). */
@@ -908,6 +926,16 @@ static int prepare_window(CromaHipBatch *b, WinBuf *w, int s_first, int count, i
         S->dt_snap = dt_snap[s];
         S->elapsed = elapsed;
         S->prm = c->prm;
+        if (split) {
+            int *off = w->h_segoff + (size_t)q * ntr;
+            for (i = 0; i < ntr; i++) {
+                off[i] = (int)w->segrows;
+                w->segrows += ch_choose_segments(&c->prm, nsub[i], al);
+            }
+            S->seg_off = w->d_segoff + (size_t)q * ntr;
+            S->seg_rec = b->d_segrec;
+            S->seg_inje = b->d_seginje;
+        }
         elapsed += dt_snap[s];
     }
     t0 = ch_now_s();
@@ -929,6 +957,7 @@ static int upload_window(CromaHipBatch *b, WinBuf *w)
         ck_h2d(w->d_tasks, w->h_tasks, (size_t)ntr * sizeof(CkTask), c->stream) ||
         ck_h2d(w->d_slot, w->h_slot, (size_t)q * sizeof(CkSlot), c->stream) ||
         ck_h2d(w->d_zb, w->h_zb, (size_t)w->nz * 3 * sizeof(double), c->stream) ||
+        (w->split && ck_h2d(w->d_segoff, w->h_segoff, (size_t)q * ntr * sizeof(int), c->stream)) ||
         ck_event_record(w->ev_up, c->stream))
         return dev_fail("window upload");
     if (ck_ic_rows(n, c->dgrid.ep, c->g.epm1, c->g.epp1, w->nz, w->d_zb, w->d_ric, 0, c->d_err, c->stream))
@@ -936,15 +965,40 @@ static int upload_window(CromaHipBatch *b, WinBuf *w)
     return 0;
 }
 
-static int launch_tasks(CromaHipContext *c, int protons, const CkSlot *d_slot, const CkTask *d_tasks, int ntasks)
+static int launch_mode(CromaHipContext *c, int protons, int mode, const CkSlot *d_slot, const CkTask *d_tasks,
+                       int ntasks)
 {
     const CromaHipTile *t = &c->tile[protons ? 1 : 0];
     const int nblocks = c->cu_count * t->waves_per_simd;
     if (ck_fill_i32(c->d_counter, 0, 1, c->stream)) return dev_fail("counter reset");
-    if (ck_launch_fused(c->g.n, t->rows_per_lane, t->waves_per_simd, nblocks, protons, c->dgrid, d_slot, d_tasks,
+    if (ck_launch_fused(c->g.n, t->rows_per_lane, t->waves_per_simd, nblocks, mode, c->dgrid, d_slot, d_tasks,
                         ntasks, c->d_counter, c->d_err, c->stream))
         return dev_fail("fused launch");
     return 0;
+}
+
+static int launch_tasks(CromaHipContext *c, int protons, const CkSlot *d_slot, const CkTask *d_tasks, int ntasks)
+{
+    return launch_mode(c, protons, protons ? 1 : 0, d_slot, d_tasks, ntasks);
+}
+
+/* Split two-species window: proton pass recording the segment-start rows,
+ * one GEMM of all rows against the secondary weights (rows jp0..n-1), then
+ * the electron pass. */
+static int launch_split(CromaHipBatch *b, WinBuf *w)
+{
+    CromaHipContext *c = b->ctx;
+    const int n = c->g.n, jp0 = c->jp0;
+    int rc = launch_mode(c, 1, 2, w->d_slot, w->d_tasks, b->ntr);
+    if (rc) return rc;
+    if (!c->blas && !(c->blas = ck_blas_create(c->stream))) return dev_fail("GEMM handle");
+    if (w->segrows > 0 && jp0 < n &&
+        ck_dgemm_nn(c->blas, n, (int)w->segrows, n - jp0, c->d_fqe + (size_t)jp0 * n, n, b->d_segrec + jp0, n,
+                    b->d_seginje, n))
+        return dev_fail("secondary GEMM");
+    if (w->segrows > 0 && jp0 >= n && ck_fill_f64(b->d_seginje, 0.0, (size_t)w->segrows * n, c->stream))
+        return dev_fail("secondary clear");
+    return launch_mode(c, 1, 3, w->d_slot, w->d_tasks, b->ntr);
 }
 
 /* Plain path: reference GPU semantics (per bucket of equal (nsub, n_on),
@@ -1156,7 +1210,7 @@ int croma_hip_batch_run(CromaHipBatch *b, int s0, int s1, int nsnap_total, const
     CromaHipContext *c;
     PrepStats ps;
     double t_start, el = elapsed;
-    int s, rc = 0, k = 0, wsz, nwin, iw;
+    int s, rc = 0, k = 0, wsz, nwin, iw, split;
     float dev_ms = 0.0f;
     double kern_ms = 0.0;
     if (!b || !z || !dt_snap || s0 < 0 || s1 < s0 || s1 > b->nsnap_res || nsnap_total < s1 ||
@@ -1166,6 +1220,22 @@ int croma_hip_batch_run(CromaHipBatch *b, int s0, int s1, int nsnap_total, const
     c = b->ctx;
     wsz = (path == CROMA_HIP_PATH_FUSED) ? c->tile[b->protons ? 1 : 0].chain : 1;
     if (wsz > s1 - s0) wsz = s1 - s0;
+    /* the split launches need the species decoupled within a segment: no eta cap */
+    split = path == CROMA_HIP_PATH_FUSED && b->protons && c->tile[1].secondary_gemm &&
+            !(c->prm.dpp_model != 3 && c->prm.eta_dpp_cap > 0.0);
+    if (split) {
+        const long segmax = c->prm.coeff_interp_max_segments > 1 ? c->prm.coeff_interp_max_segments : 1;
+        const long need = (long)wsz * b->ntr * segmax;
+        if (need > b->segrows_cap) {
+            ck_stream_sync(c->stream);
+            ck_free(b->d_segrec);
+            ck_free(b->d_seginje);
+            b->d_segrec = (double *)ck_malloc((size_t)need * c->g.n * sizeof(double));
+            b->d_seginje = (double *)ck_malloc((size_t)need * c->g.n * sizeof(double));
+            b->segrows_cap = (b->d_segrec && b->d_seginje) ? need : 0;
+            if (!b->segrows_cap) return dev_fail("segment rows");
+        }
+    }
     nwin = (s1 - s0 + wsz - 1) / wsz;
     for (k = 0; k < 2; k++)
         if ((rc = win_reserve(&b->win[k], wsz, b->ntr, c->g.n, c->stream)) != 0) return rc;
@@ -1186,7 +1256,8 @@ int croma_hip_batch_run(CromaHipBatch *b, int s0, int s1, int nsnap_total, const
     memset(&ps, 0, sizeof(ps));
     t_start = ch_now_s();
     if (ck_event_record(b->ev_a, c->stream)) return dev_fail("event");
-    rc = prepare_window(b, &b->win[0], s0, (s1 - s0 < wsz) ? s1 - s0 : wsz, nsnap_total, z, dt_snap, el, &ps);
+    rc = prepare_window(b, &b->win[0], s0, (s1 - s0 < wsz) ? s1 - s0 : wsz, nsnap_total, z, dt_snap, el, split,
+                        &ps);
     if (rc) return rc;
     for (iw = 0, s = s0; s < s1; iw++) {
         WinBuf *w = &b->win[iw & 1];
@@ -1194,7 +1265,7 @@ int croma_hip_batch_run(CromaHipBatch *b, int s0, int s1, int nsnap_total, const
         if ((rc = upload_window(b, w)) != 0) return rc;
         if (path == CROMA_HIP_PATH_FUSED) {
             if (ck_event_record(b->ev_k[2 * iw], c->stream)) return dev_fail("event");
-            rc = launch_tasks(c, b->protons, w->d_slot, w->d_tasks, b->ntr);
+            rc = w->split ? launch_split(b, w) : launch_tasks(c, b->protons, w->d_slot, w->d_tasks, b->ntr);
             if (!rc && ck_event_record(b->ev_k[2 * iw + 1], c->stream)) return dev_fail("event");
         } else {
             rc = run_plain_snapshot(b, w);
@@ -1206,7 +1277,7 @@ int croma_hip_batch_run(CromaHipBatch *b, int s0, int s1, int nsnap_total, const
             /* the other window's host buffers were consumed by its upload */
             WinBuf *o = &b->win[(iw + 1) & 1];
             if (ck_event_sync(o->ev_up)) return dev_fail("upload event");
-            rc = prepare_window(b, o, s, (s1 - s < wsz) ? s1 - s : wsz, nsnap_total, z, dt_snap, el, &ps);
+            rc = prepare_window(b, o, s, (s1 - s < wsz) ? s1 - s : wsz, nsnap_total, z, dt_snap, el, split, &ps);
             if (rc) return rc;
         }
     }
@@ -1510,14 +1581,15 @@ int croma_hip_autotune(CromaHipContext *c, int protons, int ntracer, CromaHipTil
         for (f = 0; f < CK_NBG; f++) fl[f] = bg + ((size_t)s * CK_NBG + f) * ntracer;
         if ((rc = croma_hip_batch_set_background(b, s, fl)) != 0) goto out;
     }
-    for (r = 0; r < 4; r++) {
-        if (!ck_fused_supported(n, rows[r])) continue;
+    for (r = 0; r < 4 * (protons ? 2 : 1); r++) {
+        if (!ck_fused_supported(n, rows[r % 4])) continue;
         for (w = 1; w <= 2; w++) {
             CromaHipTile t;
             CromaHipTimes tm;
-            t.rows_per_lane = rows[r];
+            t.rows_per_lane = rows[r % 4];
             t.waves_per_simd = w;
             t.chain = 1;
+            t.secondary_gemm = protons ? (r < 4) : 0;
             c->tile[protons ? 1 : 0] = t;
             memset(&tm, 0, sizeof(tm));
             if ((rc = croma_hip_batch_set_state(b, protons ? crp : 0, cre, 0, 0, 0, 0)) != 0) goto out;
@@ -1525,7 +1597,7 @@ int croma_hip_autotune(CromaHipContext *c, int protons, int ntracer, CromaHipTil
             if ((rc = croma_hip_batch_run(b, 0, 1, nsnap + 1, z, dt, 0.0, CROMA_HIP_PATH_FUSED, 0)) != 0) goto out;
             if ((rc = croma_hip_batch_run(b, 1, nsnap, nsnap + 1, z, dt, dt[0], CROMA_HIP_PATH_FUSED, &tm)) != 0)
                 goto out;
-            if (tm.kernel_ms < best_ms * 0.98 || (tm.kernel_ms < best_ms * 1.02 && rows[r] > bt.rows_per_lane)) {
+            if (tm.kernel_ms < best_ms * 0.98 || (tm.kernel_ms < best_ms * 1.02 && rows[r % 4] > bt.rows_per_lane)) {
                 if (tm.kernel_ms < best_ms) best_ms = tm.kernel_ms;
                 bt = t;
             }

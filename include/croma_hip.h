@@ -90,6 +90,9 @@ typedef struct {
     int rows_per_lane;    /* 2, 4, 8 or 16 spectrum rows per lane: n / rows_per_lane lanes (4..64) per tracer */
     int waves_per_simd;   /* 1 or 2: register budget of the kernel and resident persistent waves per SIMD */
     int chain;            /* batch runs: snapshots advanced per queue task (1 = one launch per snapshot) */
+    int secondary_gemm;   /* two species: 1 = proton pass, one GEMM of all segment-start proton rows
+                           * against the secondary weights, electron pass; 0 = contraction inside
+                           * the fused kernel (always used when the eta cap couples the species) */
 } CromaHipTile;
 
 /* Stage timers, the subset of TracerFpGpuTimes (include/tracer_fp_types.h:10-57)

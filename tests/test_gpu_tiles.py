@@ -1,5 +1,6 @@
 """Every launch shape of the fused kernel (rows per lane x waves per SIMD x
-snapshots per task) against the CPU oracle, on small seeded workloads of both
+snapshots per task x secondary contraction in the kernel or as one GEMM
+between a proton and an electron pass) against the CPU oracle, on small seeded workloads of both
 species sets.  Same tolerance as the main parity test: |hip - cpu| <= 1e-30 +
 1e-10 |cpu| (analysis/compare_tracer_outputs.py:126-127)."""
 import numpy as np
@@ -36,9 +37,11 @@ def test_all_tiles_match_oracle(refs, case):
     try:
         for r in ROWS[n]:
             for occ in (1, 2):
-                for chain in (1, ns):
-                    print(f"[tiles] n={n} species={sp} rows={r} waves={occ} chain={chain}", flush=True)
-                    ctx.set_tile(r, occ, chain)
+                for chain, gemm in ((1, 0), (ns, 0), (1, 1), (ns, 1)):
+                    if gemm and sp == 1:
+                        continue  # the split launch exists for two species only
+                    print(f"[tiles] n={n} species={sp} rows={r} waves={occ} chain={chain} gemm={gemm}", flush=True)
+                    ctx.set_tile(r, occ, chain, secondary_gemm=gemm)
                     crp, cre, _ = run_hip(w, ctx, path=ch.PATH_FUSED)
                     e_err, e_ok = rel_err(cre, cre_ref)
                     assert e_ok, f"rows={r} waves={occ} chain={chain}: electron max rel err {e_err:.3e}"
