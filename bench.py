@@ -87,7 +87,7 @@ def cpu_baseline(w, warmup, seconds_target=15.0):
 
     nthreads = os.cpu_count() or 1
     rng = np.random.default_rng(5)
-    nsample = max(nthreads * 8, 64)
+    nsample = max(nthreads * 48, 64)
     idx = np.sort(rng.choice(w.ntracer, size=min(nsample, w.ntracer), replace=False))
     sub = w.subset(idx)
     prm = run_oracle.oracle_params(sub)
@@ -130,6 +130,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--no-tune", action="store_true", help="keep the default tile instead of autotuning it")
+    ap.add_argument("--tile", default="", help="rows_per_lane,waves_per_simd,chain,secondary_gemm (skips the autotuner)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -150,7 +151,14 @@ def main():
     steps = min(args.steps, w.nsnap - args.warmup)
     ctx = ch.Context(w.n, hip_params(w))
     t_tune = time.perf_counter()
-    tile = ctx.autotune(w.protons, ntracer=min(w.ntracer, 16384)) if not args.no_tune else ctx.get_tile(w.protons)
+    if args.tile:
+        r_, w_, c_, g_ = (int(v) for v in args.tile.split(","))
+        ctx.set_tile(r_, w_, c_, protons=int(w.protons), secondary_gemm=g_)
+        tile = ctx.get_tile(w.protons)
+    elif args.no_tune:
+        tile = ctx.get_tile(w.protons)
+    else:
+        tile = ctx.autotune(w.protons, ntracer=min(w.ntracer, 16384))
     t_tune = time.perf_counter() - t_tune
 
     wall, t = run_path(ch, ctx, w, ch.PATH_FUSED, args.warmup, steps)
@@ -175,7 +183,7 @@ def main():
                                f"dpp={['', 'asa', 'ttd', 'off'][w.dpp_model]}, {w.nsnap} snapshots z=1->0",
                    "ntracer": w.ntracer, "n": w.n, "species": species, "snapshots_timed": steps,
                    "tile": {"rows_per_lane": tile[0], "waves_per_simd": tile[1], "snapshots_per_task": tile[2],
-                            "autotune_s": round(t_tune, 2)}},
+                            "secondary_gemm": tile[3], "autotune_s": round(t_tune, 2)}},
         "tracer_substeps": int(t.cell_steps),
         "species_substeps": int(t.species_steps),
         "host_ms": {"ic_rows": t.ic_rows_ms, "nsub": t.nsub_estimate_ms, "order": t.bucket_build_ms},
@@ -187,9 +195,12 @@ def main():
     alg_bytes = 48.0 * w.n * t.species_steps / max(t.kernel_launches, 1)
     achieved = alg_bytes / (kern_ms * 1e-3) / 1e9
     flops = FP64_FLOP_PER_ROW * w.n * t.species_steps / max(t.kernel_launches, 1)
+    split = bool(w.protons and tile[3])
     out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(f"k_fused_{args.config}_traffic"),
-                       "kernel": "k_fused", "avg_launch_ms": kern_ms, "alg_bytes_per_launch": alg_bytes,
+                       "kernel": ("k_fused proton pass + secondary GEMM + k_fused electron pass (one snapshot)"
+                                  if split else "k_fused (one snapshot)"),
+                       "avg_launch_ms": kern_ms, "alg_bytes_per_launch": alg_bytes,
                        "fp64_valu": {"achieved_tflops": flops / (kern_ms * 1e-3) / 1e12, "peak_tflops": FP64_PEAK_TFLOPS,
                                      "frac": flops / (kern_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}}
 
