@@ -131,7 +131,22 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--no-tune", action="store_true", help="keep the default tile instead of autotuning it")
     ap.add_argument("--tile", default="", help="rows_per_lane,waves_per_simd,chain,secondary_gemm (skips the autotuner)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 256 MiB device copy (scale of the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+
+    if args.copy_calibration:
+        import torch
+
+        n = 256 * 1024 * 1024 // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_per_dispatch": n * 8, "dispatches": 3}))
+        return
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on exactly one GPU; the tracer update shards across GPUs "
