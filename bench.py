@@ -111,6 +111,75 @@ def cpu_baseline(w, warmup, seconds_target=15.0):
                       f"tracer-substeps in {t_total:.1f} s, OpenMP {nthreads} threads)"}
 
 
+def emission_bench(ch, ntracer=1_000_000, n=256, nbins=64, seed=11):
+    """Emission contractions on the spectra workload of 1e6 tracers x 256
+    bins (power laws q ~ U(2, 3.5), log-uniform cutoffs) onto 64 bins: IC
+    and pp gamma/neutrino (one GEMM-shaped kernel each, per precision mode)
+    and synchrotron (log-B table interpolation).  Kernel tables are
+    synthetic positive tables of the reference's shapes (the contraction's
+    cost does not depend on their values).  Device time per kernel from HIP
+    events on the library's stream; bytes = FP64 spectrum row + FP64 output
+    row per tracer; flops = 2 K nbins per tracer (the split-BF16 mode issues
+    three MFMA passes of them)."""
+    rng = np.random.default_rng(seed)
+    ctx = ch.Context(n, ch.default_params())
+    out = {"workload": f"{ntracer} tracers x {n} bins -> {nbins} bins", "data": "synthetic"}
+    try:
+        pe = ctx.grid(1)[0]
+        pp = ctx.grid(0)[0]
+        q = rng.uniform(2.0, 3.5, ntracer)
+        cut = 10.0 ** rng.uniform(2.0, 6.0, ntracer)
+        spec = np.exp(-q[:, None] * np.log(pe)[None, :] - pe[None, :] / cut[:, None])
+        centres = np.linspace(np.log10(pe[0]), np.log10(pe[-1]), nbins)
+        lp = np.log10(pe)
+        fic = np.exp(-0.5 * ((lp[None, :] - centres[:, None]) / 0.7) ** 2).ravel()
+        lpp = np.log10(pp)
+        fk = np.exp(-0.5 * ((lpp[None, :] - centres[:, None] - 1.0) / 0.9) ** 2).ravel()
+        n_gas = 10.0 ** rng.uniform(-5.0, -2.0, ntracer)
+        modes = {"fp64": ch.EMIS_FP64, "fp32": ch.EMIS_FP32, "mfma_split_bf16x3": ch.EMIS_MFMA_XF32,
+                 "mfma_bf16": ch.EMIS_MFMA_BF16}
+        byts = ntracer * (8.0 * n + 8.0 * nbins)
+        flops = 2.0 * n * nbins * ntracer
+        peaks = {"fp64": ("fp64 valu", 78.6), "fp32": ("fp32 valu", 157.3),
+                 "mfma_split_bf16x3": ("bf16 mfma", 2500.0), "mfma_bf16": ("bf16 mfma", 2500.0)}
+        res = {}
+        for kname in ("ic", "hadronic"):
+            res[kname] = {}
+            for mname, mode in modes.items():
+                ms = []
+                for _ in range(2):
+                    if kname == "ic":
+                        _, t = ctx.emission_ic(spec, fic, nbins, mode)
+                    else:
+                        _, t = ctx.emission_hadronic(spec, n_gas, fk, nbins, -1.0, 5.0, mode)
+                    ms.append(t)
+                t = ms[-1]
+                passes = 3.0 if mode == ch.EMIS_MFMA_XF32 else 1.0
+                unit, peak = peaks[mname]
+                res[kname][mname] = {"ms": t, "hbm_gbs": byts / (t * 1e-3) / 1e9,
+                                     "hbm_frac": byts / (t * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                     "tflops_issued": passes * flops / (t * 1e-3) / 1e12,
+                                     "peak_tflops": peak, "peak_unit": unit,
+                                     "compute_frac": passes * flops / (t * 1e-3) / 1e12 / peak}
+        nlogb, logb_min, inv_dlogb = 512, -10.0, 512 / 6.0
+        table = np.exp(-0.5 * ((lp[None, None, :] - centres[None, :, None]) / 0.8) ** 2)
+        table = np.ascontiguousarray(np.broadcast_to(table, (nlogb, nbins, n))).ravel()
+        bfield = 10.0 ** rng.uniform(-8.0, -5.0, ntracer)
+        ms = []
+        for _ in range(2):
+            _, t = ctx.emission_synch(spec, bfield, table, nbins, nlogb, logb_min, inv_dlogb)
+            ms.append(t)
+        t = ms[-1]
+        res["synch"] = {"fp64": {"ms": t, "hbm_gbs": (byts + 8.0 * ntracer) / (t * 1e-3) / 1e9,
+                                 "hbm_frac": (byts + 8.0 * ntracer) / (t * 1e-3) / 1e9 / HBM_PEAK_GBS}}
+        out["kernels"] = res
+        out["bytes_per_call"] = byts
+        out["flops_per_call"] = flops
+    finally:
+        ctx.close()
+    return out
+
+
 def served_latency(ch, seconds=20.0, seed=3):
     """Short Poisson stream of mixed chunks through the serving loop."""
     import serve_stream
@@ -126,9 +195,13 @@ def main():
     ap.add_argument("--config", default="c1", choices=["c0", "c1", "c2"])
     ap.add_argument("--ntracer", type=int, default=0, help="override the configuration's tracer count")
     ap.add_argument("--plain-steps", type=int, default=3, help="snapshots timed on the plain path (0 skips)")
+    ap.add_argument("--plain-ntracer", type=int, default=0,
+                    help="run the plain-vs-fused comparison on the first N tracers (rates are per tracer-substep)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--emission-ntracer", type=int, default=1_000_000,
+                    help="tracers of the emission contraction workload (0 skips it)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default tile instead of autotuning it")
     ap.add_argument("--tile", default="", help="rows_per_lane,waves_per_simd,chain,secondary_gemm (skips the autotuner)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -150,7 +223,7 @@ def main():
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on exactly one GPU; the tracer update shards across GPUs "
-                                   "without communication (see DESIGN.md), but --gpus > 1 is not run here"}))
+                                   "without communication, but --gpus > 1 is not run here"}))
         sys.exit(2)
 
     import croma_hip as ch
@@ -220,12 +293,19 @@ def main():
                                      "frac": flops / (kern_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}}
 
     if args.plain_steps > 0:
-        pw, pt = run_path(ch, ctx, w, ch.PATH_PLAIN, args.warmup, min(args.plain_steps, steps))
-        fw, ft = run_path(ch, ctx, w, ch.PATH_FUSED, args.warmup, min(args.plain_steps, steps))
-        out["plain"] = {"value": pt.cell_steps / pw, "snapshots": min(args.plain_steps, steps),
+        wp = w.subset(np.arange(min(args.plain_ntracer, w.ntracer))) if args.plain_ntracer else w
+        pw, pt = run_path(ch, ctx, wp, ch.PATH_PLAIN, args.warmup, min(args.plain_steps, steps))
+        fw, ft = run_path(ch, ctx, wp, ch.PATH_FUSED, args.warmup, min(args.plain_steps, steps))
+        out["plain"] = {"value": pt.cell_steps / pw, "snapshots": min(args.plain_steps, steps), "ntracer": wp.ntracer,
                         "fused_same_snapshots": ft.cell_steps / fw,
                         "speedup_fused_vs_plain": (ft.cell_steps / fw) / (pt.cell_steps / pw)}
     ctx.close()
+
+    if args.emission_ntracer > 0:
+        try:
+            out["emission"] = emission_bench(ch, ntracer=args.emission_ntracer)
+        except (ch.CromaHipError, MemoryError) as exc:
+            out["emission"] = {"error": str(exc)}
 
     if not args.no_serve:
         try:

@@ -59,7 +59,8 @@ class Times(C.Structure):
         ("nsub_estimate_ms", C.c_double), ("bucket_build_ms", C.c_double), ("ic_rows_ms", C.c_double),
         ("h2d_ms", C.c_double), ("solve_ms", C.c_double), ("d2h_ms", C.c_double), ("total_ms", C.c_double),
         ("cell_steps", C.c_longlong), ("species_steps", C.c_longlong), ("kernel_ms", C.c_double),
-        ("kernel_launches", C.c_longlong),
+        ("kernel_launches", C.c_longlong), ("graph_launches", C.c_longlong),
+        ("graph_captures", C.c_longlong), ("graph_failures", C.c_longlong),
     ]
 
 
@@ -182,6 +183,10 @@ class Context:
         """protons: 0 electron_only tile, 1 electron_proton tile, -1 both."""
         t = Tile(rows_per_lane, waves_per_simd, chain, secondary_gemm)
         _chk(lib().croma_hip_context_set_tile(C.c_void_p(self.h), int(protons), C.byref(t)), "set_tile")
+
+    def set_graphs(self, enable):
+        """True: repeated snapshot windows replay a captured hipGraph of their launches."""
+        _chk(lib().croma_hip_context_set_graphs(C.c_void_p(self.h), int(bool(enable))), "set_graphs")
 
     def get_tile(self, protons=False):
         t = Tile()

@@ -32,6 +32,10 @@
 __device__ __forceinline__ double d_coulomb_kernel(double x)
 {
 #pragma clang fp contract(off)
+    // For x >= 40, erf(sqrt x) rounds to 1 and sqrt(x) exp(-x) < 3e-17 is
+    // below half an ulp of 0.5 sqrt(pi): the expression equals that constant
+    // exactly (the thermal proton term of every bin well above kT).
+    if (x >= 40.0) return 0.5 * sqrt(K_PI);
     return 0.5 * sqrt(K_PI) * erf(sqrt(x)) - sqrt(x) * exp(-x);
 }
 

@@ -134,6 +134,15 @@ static int valid_params(const CromaHipParams *p)
 #define CH_IC_CACHE 8
 #define CH_IC_ENDS 256
 
+/* A launch graph slot: the executable of the last launch key seen. */
+#define CH_GKEY 8
+
+typedef struct {
+    void *exec;
+    long key[CH_GKEY];
+    int state; /* 0 empty, 1 key seen once, 2 captured, -1 capture refused */
+} ChGraph;
+
 struct CromaHipContext {
     int device;
     int cu_count;
@@ -170,6 +179,15 @@ struct CromaHipContext {
     CkPlainWs pws;
     void *pws_mem;
     int pws_cap;
+    /* window launches replayed as hipGraphs */
+    int use_graphs;
+    void **retired; /* graph executables replaced while their last launch may still run */
+    int nretired, retired_cap;
+    /* croma_hip_execute_snapshot state kept between calls */
+    CromaHipBatch *xb;
+    void *x_ws;
+    size_t x_ws_bytes;
+    ChGraph x_graph;
 };
 
 /* Derived tables of the Chang-Cooper build for one species: face width
@@ -313,13 +331,45 @@ CromaHipContext *croma_hip_context_create(const CromaHipGridSpec *gs, const Crom
         return 0;
     }
     default_tiles(c);
+    c->use_graphs = 1;
     return c;
+}
+
+/* Graph executables are destroyed only after the stream has drained. */
+static void retire_graph(CromaHipContext *c, void *ex)
+{
+    if (!ex) return;
+    if (c->nretired == c->retired_cap) {
+        const int cap = c->retired_cap ? 2 * c->retired_cap : 8;
+        void **nr = (void **)realloc(c->retired, (size_t)cap * sizeof(void *));
+        if (!nr) {
+            ck_stream_sync(c->stream);
+            ck_graph_destroy(ex);
+            return;
+        }
+        c->retired = nr;
+        c->retired_cap = cap;
+    }
+    c->retired[c->nretired++] = ex;
+}
+
+/* Call with the stream drained. */
+static void flush_retired(CromaHipContext *c)
+{
+    int i;
+    for (i = 0; i < c->nretired; i++) ck_graph_destroy(c->retired[i]);
+    c->nretired = 0;
 }
 
 void croma_hip_context_destroy(CromaHipContext *c)
 {
     if (!c) return;
     if (c->stream) ck_stream_sync(c->stream);
+    croma_hip_batch_destroy(c->xb);
+    ck_free(c->x_ws);
+    ck_graph_destroy(c->x_graph.exec);
+    flush_retired(c);
+    free(c->retired);
     ck_free(c->d_grid_mem);
     ck_free(c->d_counter);
     ck_free(c->d_err);
@@ -354,6 +404,14 @@ int croma_hip_context_get_tile(const CromaHipContext *c, int protons, CromaHipTi
 {
     if (!c || !t) return CROMA_HIP_EINVAL;
     *t = c->tile[protons ? 1 : 0];
+    return 0;
+}
+
+int croma_hip_context_set_graphs(CromaHipContext *c, int enable)
+{
+    if (!c || (enable != 0 && enable != 1)) return CROMA_HIP_EINVAL;
+    ck_stream_sync(c->stream);
+    c->use_graphs = enable;
     return 0;
 }
 
@@ -565,6 +623,64 @@ static void order_tasks(const CkParams *prm, const int *allow, int nslot, int nt
 }
 
 /* ------------------------------------------------------------------ */
+/* Launch graphs.  The launches of a window (counter fill, fused kernel,
+ * GEMM, fused kernel) depend only on the key: kernel shape, buffer
+ * addresses and GEMM rows.  The per-window data travel in device buffers
+ * uploaded before the launch, so a window whose key equals the previous
+ * window's of the same buffer set replays the captured graph.  A new key
+ * runs as plain launches once; its first repeat is captured. */
+typedef struct {
+    long long launches, captures, failures;
+} GraphStats;
+
+typedef int (*ch_issue_fn)(void *arg);
+
+static int graph_issue(CromaHipContext *c, ChGraph *g, const long key[CH_GKEY], ch_issue_fn fn, void *arg,
+                       GraphStats *gs)
+{
+    void *ex;
+    int rc;
+    const int same = g->state != 0 && !memcmp(g->key, key, sizeof(g->key));
+    if (!c->use_graphs) return fn(arg);
+    if (!same) {
+        retire_graph(c, g->exec);
+        g->exec = 0;
+        memcpy(g->key, key, sizeof(g->key));
+        g->state = 1;
+        return fn(arg);
+    }
+    if (g->state == -1) return fn(arg);
+    if (g->state == 1) {
+        if (ck_graph_begin(c->stream)) {
+            g->state = -1;
+            gs->failures++;
+            return fn(arg);
+        }
+        rc = fn(arg);
+        ex = ck_graph_end(c->stream);
+        if (rc || !ex) {
+            /* nothing captured ran: issue the window as plain launches */
+            ck_graph_destroy(ex);
+            g->state = -1;
+            gs->failures++;
+            return fn(arg);
+        }
+        g->exec = ex;
+        g->state = 2;
+        gs->captures++;
+    }
+    if (ck_graph_launch(g->exec, c->stream)) return dev_fail("graph launch");
+    gs->launches++;
+    return 0;
+}
+
+static void graph_reset(CromaHipContext *c, ChGraph *g)
+{
+    retire_graph(c, g->exec);
+    memset(g, 0, sizeof(*g));
+}
+
+/* ------------------------------------------------------------------ */
 /* A window is the unit of one fused launch: W consecutive snapshots, one
  * queue task per tracer advancing it over all of them.  Two window buffer
  * sets alternate so the host prepares window w+1 while the GPU runs w. */
@@ -583,8 +699,10 @@ typedef struct {
     int nz;                  /* IC rows to build */
     int *d_segoff, *h_segoff; /* split launches: [cap][ntr] first segment row per tracer */
     long segrows;            /* segment rows of the window */
+    long segq;               /* segrows rounded up to the batch's GEMM row quantum */
     int split;               /* the window runs as proton pass + GEMM + electron pass */
     void *ev_up;
+    ChGraph graph;           /* launches of the last window shape run from this buffer set */
 } WinBuf;
 
 struct CromaHipBatch {
@@ -600,6 +718,7 @@ struct CromaHipBatch {
     WinBuf win[2];
     double *d_segrec, *d_seginje; /* split launches: [segrows_cap][n] */
     long segrows_cap;
+    long seg_quantum;             /* GEMM rows are rounded up to a multiple of this */
     int *hist, *keys;
     void *ev_a, *ev_b;
     void **ev_k; /* per-launch timing events, 2 per window */
@@ -609,8 +728,9 @@ struct CromaHipBatch {
     int *h_idx;
 };
 
-static void win_free(WinBuf *w)
+static void win_free(CromaHipContext *c, WinBuf *w)
 {
+    graph_reset(c, &w->graph);
     ck_free(w->d_nsub);
     ck_free(w->d_non);
     ck_free(w->d_tasks);
@@ -628,12 +748,12 @@ static void win_free(WinBuf *w)
     memset(w, 0, sizeof(*w));
 }
 
-static int win_reserve(WinBuf *w, int cap, int ntr, int n, void *stream)
+static int win_reserve(CromaHipContext *c, WinBuf *w, int cap, int ntr, int n)
 {
     if (w->cap >= cap) return 0;
     if (w->ev_up) ck_event_sync(w->ev_up);
-    ck_stream_sync(stream);
-    win_free(w);
+    ck_stream_sync(c->stream);
+    win_free(c, w);
     w->d_nsub = (int *)ck_malloc((size_t)cap * ntr * sizeof(int));
     w->d_non = (int *)ck_malloc((size_t)cap * ntr * sizeof(int));
     w->d_tasks = (CkTask *)ck_malloc((size_t)ntr * sizeof(CkTask));
@@ -650,7 +770,7 @@ static int win_reserve(WinBuf *w, int cap, int ntr, int n, void *stream)
     w->ev_up = ck_event_create();
     if (!w->d_nsub || !w->d_non || !w->d_tasks || !w->d_slot || !w->d_ric || !w->d_zb || !w->h_nsub ||
         !w->h_non || !w->h_tasks || !w->h_slot || !w->h_zb || !w->ev_up || !w->d_segoff || !w->h_segoff) {
-        win_free(w);
+        win_free(c, w);
         return dev_fail("window buffers");
     }
     w->cap = cap;
@@ -662,6 +782,8 @@ void croma_hip_batch_destroy(CromaHipBatch *b)
     int k;
     if (!b) return;
     if (b->ctx && b->ctx->stream) ck_stream_sync(b->ctx->stream);
+    for (k = 0; k < 2; k++) win_free(b->ctx, &b->win[k]);
+    if (b->ctx) flush_retired(b->ctx);
     ck_free(b->d_crp);
     ck_free(b->d_cre);
     ck_free(b->d_qpi);
@@ -678,7 +800,6 @@ void croma_hip_batch_destroy(CromaHipBatch *b)
     free(b->hist);
     free(b->keys);
     ck_host_free(b->h_idx);
-    for (k = 0; k < 2; k++) win_free(&b->win[k]);
     ck_event_destroy(b->ev_a);
     ck_event_destroy(b->ev_b);
     for (k = 0; k < b->ev_k_cap; k++) ck_event_destroy(b->ev_k[k]);
@@ -718,7 +839,7 @@ CromaHipBatch *croma_hip_batch_create(CromaHipContext *c, int ntr, int species, 
         !b->ev_a || !b->ev_b)
         goto fail;
     for (k = 0; k < 2; k++)
-        if (win_reserve(&b->win[k], 1, ntr, n, c->stream)) goto fail;
+        if (win_reserve(c, &b->win[k], 1, ntr, n)) goto fail;
     if (b->protons && ch_ctx_fqe(c) != 0) goto fail;
     return b;
 fail:
@@ -938,6 +1059,7 @@ static int prepare_window(CromaHipBatch *b, WinBuf *w, int s_first, int count, i
         }
         elapsed += dt_snap[s];
     }
+    w->segq = split ? (w->segrows + b->seg_quantum - 1) / b->seg_quantum * b->seg_quantum : 0;
     t0 = ch_now_s();
     order_tasks(&c->prm, allow_v, count, ntr, w->h_nsub, b->protons, w->h_tasks, b->hist, b->keys);
     ps->order_ms += 1e3 * (ch_now_s() - t0);
@@ -991,14 +1113,50 @@ static int launch_split(CromaHipBatch *b, WinBuf *w)
     const int n = c->g.n, jp0 = c->jp0;
     int rc = launch_mode(c, 1, 2, w->d_slot, w->d_tasks, b->ntr);
     if (rc) return rc;
-    if (!c->blas && !(c->blas = ck_blas_create(c->stream))) return dev_fail("GEMM handle");
-    if (w->segrows > 0 && jp0 < n &&
-        ck_dgemm_nn(c->blas, n, (int)w->segrows, n - jp0, c->d_fqe + (size_t)jp0 * n, n, b->d_segrec + jp0, n,
+    /* rows segrows..segq-1 (the rounding to the row quantum, which keeps the
+     * GEMM shape and so the launch graph fixed while segrows drifts) hold
+     * stale rows whose products are never read */
+    if (!c->blas) return dev_fail("GEMM handle");
+    if (w->segq > 0 && jp0 < n &&
+        ck_dgemm_nn(c->blas, n, (int)w->segq, n - jp0, c->d_fqe + (size_t)jp0 * n, n, b->d_segrec + jp0, n,
                     b->d_seginje, n))
         return dev_fail("secondary GEMM");
-    if (w->segrows > 0 && jp0 >= n && ck_fill_f64(b->d_seginje, 0.0, (size_t)w->segrows * n, c->stream))
+    if (w->segq > 0 && jp0 >= n && ck_fill_f64(b->d_seginje, 0.0, (size_t)w->segq * n, c->stream))
         return dev_fail("secondary clear");
     return launch_mode(c, 1, 3, w->d_slot, w->d_tasks, b->ntr);
+}
+
+typedef struct {
+    CromaHipBatch *b;
+    WinBuf *w;
+} WinIssue;
+
+static int issue_window(void *arg)
+{
+    WinIssue *wi = (WinIssue *)arg;
+    CromaHipBatch *b = wi->b;
+    if (wi->w->split) return launch_split(b, wi->w);
+    return launch_tasks(b->ctx, b->protons, wi->w->d_slot, wi->w->d_tasks, b->ntr);
+}
+
+static int launch_window(CromaHipBatch *b, WinBuf *w, GraphStats *gs)
+{
+    CromaHipContext *c = b->ctx;
+    const CromaHipTile *t = &c->tile[b->protons ? 1 : 0];
+    WinIssue wi;
+    long key[CH_GKEY];
+    memset(key, 0, sizeof(key));
+    key[0] = w->split;
+    key[1] = w->segq;
+    key[2] = t->rows_per_lane;
+    key[3] = t->waves_per_simd;
+    key[4] = (long)(size_t)b->d_segrec;
+    key[5] = b->protons;
+    key[6] = c->jp0;
+    key[7] = b->ntr;
+    wi.b = b;
+    wi.w = w;
+    return graph_issue(c, &w->graph, key, issue_window, &wi, gs);
 }
 
 /* Plain path: reference GPU semantics (per bucket of equal (nsub, n_on),
@@ -1187,6 +1345,7 @@ int croma_hip_batch_sync(CromaHipBatch *b)
     if (!b) return CROMA_HIP_EINVAL;
     if (ck_d2h(&err, b->ctx->d_err, sizeof(int), b->ctx->stream) || ck_stream_sync(b->ctx->stream))
         return dev_fail("batch sync");
+    flush_retired(b->ctx);
     if (err) {
         ck_fill_i32(b->ctx->d_err, 0, 1, b->ctx->stream);
         ck_stream_sync(b->ctx->stream);
@@ -1209,6 +1368,7 @@ int croma_hip_batch_run(CromaHipBatch *b, int s0, int s1, int nsnap_total, const
 {
     CromaHipContext *c;
     PrepStats ps;
+    GraphStats gst;
     double t_start, el = elapsed;
     int s, rc = 0, k = 0, wsz, nwin, iw, split;
     float dev_ms = 0.0f;
@@ -1225,20 +1385,28 @@ int croma_hip_batch_run(CromaHipBatch *b, int s0, int s1, int nsnap_total, const
             !(c->prm.dpp_model != 3 && c->prm.eta_dpp_cap > 0.0);
     if (split) {
         const long segmax = c->prm.coeff_interp_max_segments > 1 ? c->prm.coeff_interp_max_segments : 1;
-        const long need = (long)wsz * b->ntr * segmax;
+        long need = (long)wsz * b->ntr * segmax, q = 256;
+        while (q < need / 64) q <<= 1;
+        need = (need + q - 1) / q * q;
         if (need > b->segrows_cap) {
+            const size_t cnt = (size_t)need * c->g.n;
             ck_stream_sync(c->stream);
             ck_free(b->d_segrec);
             ck_free(b->d_seginje);
-            b->d_segrec = (double *)ck_malloc((size_t)need * c->g.n * sizeof(double));
-            b->d_seginje = (double *)ck_malloc((size_t)need * c->g.n * sizeof(double));
+            b->d_segrec = (double *)ck_malloc(cnt * sizeof(double));
+            b->d_seginje = (double *)ck_malloc(cnt * sizeof(double));
             b->segrows_cap = (b->d_segrec && b->d_seginje) ? need : 0;
             if (!b->segrows_cap) return dev_fail("segment rows");
+            /* rows past a window's segment count enter the rounded GEMM: keep them finite */
+            if (ck_fill_f64(b->d_segrec, 0.0, cnt, c->stream) || ck_fill_f64(b->d_seginje, 0.0, cnt, c->stream))
+                return dev_fail("segment rows");
         }
+        b->seg_quantum = q;
+        if (!c->blas && !(c->blas = ck_blas_create(c->stream))) return dev_fail("GEMM handle");
     }
     nwin = (s1 - s0 + wsz - 1) / wsz;
     for (k = 0; k < 2; k++)
-        if ((rc = win_reserve(&b->win[k], wsz, b->ntr, c->g.n, c->stream)) != 0) return rc;
+        if ((rc = win_reserve(c, &b->win[k], wsz, b->ntr, c->g.n)) != 0) return rc;
     if (path == CROMA_HIP_PATH_FUSED && b->ev_k_cap < 2 * nwin) {
         int need = 2 * nwin, j;
         void **ne = (void **)realloc(b->ev_k, (size_t)need * sizeof(void *));
@@ -1254,6 +1422,7 @@ int croma_hip_batch_run(CromaHipBatch *b, int s0, int s1, int nsnap_total, const
         b->ev_k_cap = need;
     }
     memset(&ps, 0, sizeof(ps));
+    memset(&gst, 0, sizeof(gst));
     t_start = ch_now_s();
     if (ck_event_record(b->ev_a, c->stream)) return dev_fail("event");
     rc = prepare_window(b, &b->win[0], s0, (s1 - s0 < wsz) ? s1 - s0 : wsz, nsnap_total, z, dt_snap, el, split,
@@ -1265,7 +1434,7 @@ int croma_hip_batch_run(CromaHipBatch *b, int s0, int s1, int nsnap_total, const
         if ((rc = upload_window(b, w)) != 0) return rc;
         if (path == CROMA_HIP_PATH_FUSED) {
             if (ck_event_record(b->ev_k[2 * iw], c->stream)) return dev_fail("event");
-            rc = w->split ? launch_split(b, w) : launch_tasks(c, b->protons, w->d_slot, w->d_tasks, b->ntr);
+            rc = launch_window(b, w, &gst);
             if (!rc && ck_event_record(b->ev_k[2 * iw + 1], c->stream)) return dev_fail("event");
         } else {
             rc = run_plain_snapshot(b, w);
@@ -1296,8 +1465,24 @@ int croma_hip_batch_run(CromaHipBatch *b, int s0, int s1, int nsnap_total, const
         times->total_ms += 1e3 * (ch_now_s() - t_start);
         times->cell_steps += ps.cell_steps;
         times->species_steps += ps.species_steps;
+        times->graph_launches += gst.launches;
+        times->graph_captures += gst.captures;
+        times->graph_failures += gst.failures;
     }
     return rc;
+}
+
+typedef struct {
+    CromaHipContext *c;
+    int protons, ntr;
+    const CkSlot *d_slot;
+    const CkTask *d_tasks;
+} XIssue;
+
+static int issue_tasks(void *arg)
+{
+    XIssue *x = (XIssue *)arg;
+    return launch_tasks(x->c, x->protons, x->d_slot, x->d_tasks, x->ntr);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1362,16 +1547,36 @@ int croma_hip_execute_snapshot(
     for (i = 0; i < (int)m && !protons; i++)
         if (crp[i] != 0.0 || qpi[i] != 0.0) protons = 1;
 
-    b = croma_hip_batch_create(c, ntr, protons ? CROMA_HIP_SPECIES_ELECTRON_PROTON : CROMA_HIP_SPECIES_ELECTRON, 2);
-    if (!b) return CROMA_HIP_ENOMEM;
+    /* the batch and the workspace persist between calls of equal shape, as
+     * the reference's grow-only pipeline workspace does */
+    if (c->xb && (c->xb->ntr != ntr || c->xb->protons != protons)) {
+        croma_hip_batch_destroy(c->xb);
+        c->xb = 0;
+    }
+    if (!c->xb &&
+        !(c->xb = croma_hip_batch_create(c, ntr, protons ? CROMA_HIP_SPECIES_ELECTRON_PROTON : CROMA_HIP_SPECIES_ELECTRON,
+                                         2)))
+        return CROMA_HIP_ENOMEM;
+    b = c->xb;
     rc = croma_hip_batch_set_state(b, protons ? crp : 0, cre, protons ? qpi : 0, qepri, mass, dis);
     if (!rc) rc = croma_hip_batch_set_background(b, 0, cur);
     if (!rc) rc = croma_hip_batch_set_background(b, 1, nxt);
     if (rc) goto done;
 
     /* caller-provided schedule, IC rows and secondary weights */
-    d_ws = ck_malloc((size_t)ntr * (2 * sizeof(int) + sizeof(CkTask)) + sizeof(CkSlot) + 2 * (size_t)n * sizeof(double) +
-                     (size_t)n * n * sizeof(double) + 8192);
+    {
+        const size_t need = (size_t)ntr * (2 * sizeof(int) + sizeof(CkTask)) + sizeof(CkSlot) +
+                            2 * (size_t)n * sizeof(double) + (size_t)n * n * sizeof(double) + 8192;
+        if (need > c->x_ws_bytes) {
+            ck_stream_sync(c->stream);
+            ck_graph_destroy(c->x_graph.exec);
+            memset(&c->x_graph, 0, sizeof(c->x_graph));
+            ck_free(c->x_ws);
+            c->x_ws = ck_malloc(need);
+            c->x_ws_bytes = c->x_ws ? need : 0;
+        }
+        d_ws = c->x_ws;
+    }
     if (!d_ws) {
         rc = dev_fail("execute workspace");
         goto done;
@@ -1438,9 +1643,29 @@ int croma_hip_execute_snapshot(
             rc = dev_fail("execute upload");
             goto done;
         }
-        if (launch_tasks(c, protons, d_slot, d_tasks, ntr)) {
-            rc = dev_fail("execute launch");
-            goto done;
+        {
+            const CromaHipTile *t = &c->tile[protons ? 1 : 0];
+            XIssue xi;
+            GraphStats gst;
+            long key[CH_GKEY];
+            memset(key, 0, sizeof(key));
+            memset(&gst, 0, sizeof(gst));
+            key[0] = protons;
+            key[1] = ntr;
+            key[2] = t->rows_per_lane;
+            key[3] = t->waves_per_simd;
+            key[4] = (long)(size_t)d_ws;
+            xi.c = c;
+            xi.protons = protons;
+            xi.d_slot = d_slot;
+            xi.d_tasks = d_tasks;
+            xi.ntr = ntr;
+            if ((rc = graph_issue(c, &c->x_graph, key, issue_tasks, &xi, &gst)) != 0) goto done;
+            if (times) {
+                times->graph_launches += gst.launches;
+                times->graph_captures += gst.captures;
+                times->graph_failures += gst.failures;
+            }
         }
     }
     rc = croma_hip_batch_sync(b);
@@ -1457,8 +1682,7 @@ int croma_hip_execute_snapshot(
     if (times) times->total_ms += 1e3 * (ch_now_s() - t0);
 done:
     ck_stream_sync(c->stream);
-    ck_free(d_ws);
-    croma_hip_batch_destroy(b);
+    flush_retired(c);
     return rc;
 }
 

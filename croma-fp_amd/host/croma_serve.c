@@ -71,6 +71,13 @@ typedef struct {
     int slot_cap, task_cap;
     SvReq **launched;
     int nlaunched, launched_cap;
+    /* split launches (two species, no eta cap): segment offsets per launched
+     * (chunk, snapshot) and the class's segment-row buffers */
+    int *h_segoff, *d_segoff;
+    long segoff_cap;
+    double *d_segrec, *d_seginje;
+    long segrows_cap;
+    void *blas;
 } SvClass;
 
 struct CromaHipServer {
@@ -430,8 +437,8 @@ static int sv_launch_class(CromaHipServer *s, int ci)
     CromaHipContext *c = s->ctx[ci >> 1];
     const int protons = ci & 1;
     CromaHipTile tile;
-    int i, nslots = 0, ntasks = 0, nl = 0, q;
-    long pos;
+    int i, nslots = 0, ntasks = 0, nl = 0, q, split;
+    long pos, segrows = 0, nsegoff = 0;
     if (!c) return 0;
     tile = ch_ctx_tile(c, protons);
     for (i = 0; i < s->nactive; i++) {
@@ -450,6 +457,40 @@ static int sv_launch_class(CromaHipServer *s, int ci)
     }
     if (nl == 0) return 0;
     if (sv_grow_class(k, nslots, ntasks, nl)) return CROMA_HIP_ENOMEM;
+    /* two species without the eta cap: proton pass, one GEMM, electron pass */
+    split = protons && tile.secondary_gemm && !(ch_ctx_prm(c)->eta_dpp_cap > 0.0);
+    if (split) {
+        long need_off = 0, need_rows = 0;
+        for (i = 0; i < s->nactive; i++) {
+            SvReq *r = s->active[i];
+            if (r->cls != ci || r->state != SV_ACTIVE || r->snap_next >= r->nsnap || sv_cancelled(r) || !r->ready)
+                continue;
+            need_off += (long)r->inflight * r->ntr;
+            for (q = 0; q < r->inflight; q++) {
+                const int *ns = r->h_sched + (size_t)(r->snap_next + q) * r->ntr;
+                int t;
+                for (t = 0; t < r->ntr; t++) need_rows += ch_choose_segments(&r->prm, ns[t], 1);
+            }
+        }
+        if (need_off > k->segoff_cap) {
+            ck_host_free(k->h_segoff);
+            ck_free(k->d_segoff);
+            k->h_segoff = (int *)ck_host_alloc((size_t)need_off * 2 * sizeof(int));
+            k->d_segoff = (int *)ck_malloc((size_t)need_off * 2 * sizeof(int));
+            k->segoff_cap = (k->h_segoff && k->d_segoff) ? need_off * 2 : 0;
+            if (!k->segoff_cap) return CROMA_HIP_ENOMEM;
+        }
+        if (need_rows > k->segrows_cap) {
+            const long cap = need_rows + need_rows / 2;
+            ck_free(k->d_segrec);
+            ck_free(k->d_seginje);
+            k->d_segrec = (double *)ck_malloc((size_t)cap * SV_NS[ci >> 1] * sizeof(double));
+            k->d_seginje = (double *)ck_malloc((size_t)cap * SV_NS[ci >> 1] * sizeof(double));
+            k->segrows_cap = (k->d_segrec && k->d_seginje) ? cap : 0;
+            if (!k->segrows_cap) return CROMA_HIP_ENOMEM;
+        }
+        if (!k->blas && !(k->blas = ck_blas_create(k->stream))) return CROMA_HIP_EDEVICE;
+    }
     if (ntasks > s->tmp_cap) {
         free(s->tmp_tasks);
         free(s->keys);
@@ -495,6 +536,19 @@ static int sv_launch_class(CromaHipServer *s, int ci)
             S->dt_snap = r->dt[sn];
             S->elapsed = r->elapsed;
             S->prm = r->prm;
+            if (split) {
+                int *off = k->h_segoff + nsegoff;
+                const int *ns = r->h_sched + (size_t)sn * r->ntr;
+                int t;
+                for (t = 0; t < r->ntr; t++) {
+                    off[t] = (int)segrows;
+                    segrows += ch_choose_segments(&r->prm, ns[t], 1);
+                }
+                S->seg_off = k->d_segoff + nsegoff;
+                S->seg_rec = k->d_segrec;
+                S->seg_inje = k->d_seginje;
+                nsegoff += r->ntr;
+            }
             r->elapsed += r->dt[sn];
         }
         for (t = 0; t < r->ntr; t++) {
@@ -526,12 +580,27 @@ static int sv_launch_class(CromaHipServer *s, int ci)
         const int nblocks = ch_ctx_cu_count(c) * tile.waves_per_simd;
         for (i = 0; i < nl; i++)
             if (ck_stream_wait_event(k->stream, k->launched[i]->ev_ready)) return CROMA_HIP_EDEVICE;
-        if (ck_h2d(k->d_slots, k->h_slots, (size_t)nslots * sizeof(CkSlot), k->stream) ||
-            ck_h2d(k->d_tasks, k->h_tasks, (size_t)ntasks * sizeof(CkTask), k->stream) ||
-            ck_fill_i32(k->d_ctrl, 0, 1, k->stream) ||
-            ck_launch_fused(SV_NS[ci >> 1], tile.rows_per_lane, tile.waves_per_simd, nblocks, protons, *dg,
-                            k->d_slots, k->d_tasks, ntasks, k->d_ctrl, k->d_ctrl + 1, k->stream) ||
-            ck_d2h(k->h_err, k->d_ctrl + 1, sizeof(int), k->stream) || ck_event_record(k->ev, k->stream)) {
+        const int n = SV_NS[ci >> 1], jp0 = ch_ctx_jp0(c);
+        int rc = ck_h2d(k->d_slots, k->h_slots, (size_t)nslots * sizeof(CkSlot), k->stream) ||
+                 ck_h2d(k->d_tasks, k->h_tasks, (size_t)ntasks * sizeof(CkTask), k->stream) ||
+                 (split && ck_h2d(k->d_segoff, k->h_segoff, (size_t)nsegoff * sizeof(int), k->stream)) ||
+                 ck_fill_i32(k->d_ctrl, 0, 1, k->stream);
+        if (!rc && !split)
+            rc = ck_launch_fused(n, tile.rows_per_lane, tile.waves_per_simd, nblocks, protons, *dg, k->d_slots,
+                                 k->d_tasks, ntasks, k->d_ctrl, k->d_ctrl + 1, k->stream);
+        if (!rc && split) {
+            rc = ck_launch_fused(n, tile.rows_per_lane, tile.waves_per_simd, nblocks, 2, *dg, k->d_slots, k->d_tasks,
+                                 ntasks, k->d_ctrl, k->d_ctrl + 1, k->stream);
+            if (!rc && segrows > 0 && jp0 < n)
+                rc = ck_dgemm_nn(k->blas, n, (int)segrows, n - jp0, ch_ctx_d_fqe(c) + (size_t)jp0 * n, n,
+                                 k->d_segrec + jp0, n, k->d_seginje, n);
+            if (!rc && segrows > 0 && jp0 >= n) rc = ck_fill_f64(k->d_seginje, 0.0, (size_t)segrows * n, k->stream);
+            if (!rc) rc = ck_fill_i32(k->d_ctrl, 0, 1, k->stream);
+            if (!rc)
+                rc = ck_launch_fused(n, tile.rows_per_lane, tile.waves_per_simd, nblocks, 3, *dg, k->d_slots,
+                                     k->d_tasks, ntasks, k->d_ctrl, k->d_ctrl + 1, k->stream);
+        }
+        if (rc || ck_d2h(k->h_err, k->d_ctrl + 1, sizeof(int), k->stream) || ck_event_record(k->ev, k->stream)) {
             ch_set_error("serve: launch: %s", ck_last_error());
             return CROMA_HIP_EDEVICE;
         }
@@ -756,6 +825,11 @@ void croma_hip_serve_destroy(CromaHipServer *s)
         ck_free(k->d_slots);
         ck_host_free(k->h_tasks);
         ck_free(k->d_tasks);
+        ck_host_free(k->h_segoff);
+        ck_free(k->d_segoff);
+        ck_free(k->d_segrec);
+        ck_free(k->d_seginje);
+        ck_blas_destroy(k->blas);
         free(k->launched);
         if (k->stream) ck_stream_destroy(k->stream);
     }

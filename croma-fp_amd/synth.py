@@ -153,10 +153,10 @@ def make_workload(n=128, ntracer=1024, nsnap=100, species=1, dpp_model=3, seed=1
 
 
 CONFIGS = {
-    # 1,024 electron tracers, 128 bins, 100 snapshots, Dpp off
+    # plain CPU-runnable case: 1,024 electron tracers, 128 bins, 100 snapshots, D_pp off
     "c0": dict(n=128, ntracer=1024, nsnap=100, species=1, dpp_model=3),
-    # 10,000 tracers, electron_proton, 256 bins, ASA, DSA sources on 5%
+    # batched uneven work: 10,000 tracers, electron_proton, 256 bins, ASA, DSA sources on 5%
     "c1": dict(n=256, ntracer=10000, nsnap=100, species=3, dpp_model=1, dsa_frac=0.05),
-    # 1,000,000 electron tracers, 512 bins, TTD, 20 snapshots
+    # bulk throughput: 1,000,000 electron tracers, 512 bins, TTD, 20 snapshots
     "c2": dict(n=512, ntracer=1000000, nsnap=20, species=1, dpp_model=2),
 }

@@ -109,6 +109,9 @@ typedef struct {
     long long species_steps;  /* sum of nsub x active species */
     double kernel_ms;         /* device time inside the fused snapshot kernels */
     long long kernel_launches;/* fused snapshot kernel launches timed in kernel_ms */
+    long long graph_launches; /* windows replayed from a captured hipGraph */
+    long long graph_captures; /* hipGraphs captured */
+    long long graph_failures; /* captures the runtime refused (those windows ran as plain launches) */
 } CromaHipTimes;
 
 /* ---- context: device, grid tables, streams (cuda_pipeline_workspace_*,
@@ -121,6 +124,11 @@ int croma_hip_context_set_params(CromaHipContext *ctx, const CromaHipParams *prm
 /* protons: 0 electron_only tile, 1 electron_proton tile, -1 both. */
 int croma_hip_context_set_tile(CromaHipContext *ctx, int protons, const CromaHipTile *tile);
 int croma_hip_context_get_tile(const CromaHipContext *ctx, int protons, CromaHipTile *tile);
+/* 1 (default): a snapshot window whose launch shape repeats the previous
+ * window of its buffer set replays a captured hipGraph of its launches
+ * (fill, fused kernel, GEMM, fused kernel); 0: plain stream launches. No
+ * reference counterpart (the reference launches per bucket and segment). */
+int croma_hip_context_set_graphs(CromaHipContext *ctx, int enable);
 /* Grid arrays (momentum_bin, FP_Coef.c:291-330); species 0 proton, 1 electron. */
 int croma_hip_context_grid(const CromaHipContext *ctx, int species, double *p, double *dp, double *E,
                            double *pm1, double *pp1);

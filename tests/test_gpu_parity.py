@@ -1,6 +1,6 @@
 """HIP fused and plain paths vs the CPU oracle on the same seeded inputs.
 
-Tolerance (north star): spectra within 1e-10 relative above a 1e-30 floor,
+Tolerance: spectra within 1e-10 relative above a 1e-30 floor,
 i.e. the reference comparator |hip - cpu| <= 1e-30 + 1e-10 |cpu|
 (analysis/compare_tracer_outputs.py:126-127, 356-366)."""
 import numpy as np
