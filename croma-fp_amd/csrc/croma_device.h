@@ -56,6 +56,20 @@ __device__ double d_b_coulomb_p(double nicm, double p, double kBT)
     return K_GYR * K_C / (K_MP * K_GEV) * sum * nicm * (1.0 / betap2) * thermal * 3.5e-29;
 }
 
+// The same rate from per-bin factors: Ep = gamma m_p - m_p, two_mk =
+// 2 m_p (gamma - 1), ib2 = 1 / beta^2 (ck_grid_loss_tables evaluates them with
+// the expressions above, so the result is bit-identical).
+__device__ __forceinline__ double d_b_coulomb_p_tab(double nicm, double kBT, double Ep, double two_mk, double ib2)
+{
+#pragma clang fp contract(off)
+    const double xp = Ep / kBT;
+    const double xe = xp * (K_ME / K_MP);
+    const double sum = d_coulomb_kernel(xe) + d_coulomb_kernel(xp) * (K_ME / K_MP);
+    const double thermal = 1.0 - kBT / two_mk;
+    if (thermal < 0.0) return 0.0;
+    return K_GYR * K_C / (K_MP * K_GEV) * sum * nicm * ib2 * thermal * 3.5e-29;
+}
+
 __device__ double d_I_theta(double x)
 {
 #pragma clang fp contract(off)
@@ -740,7 +754,7 @@ __device__ __forceinline__ void seg_radp(const CkGrid &g, const CkSlot &S, int b
             const int j = base + k;
             const double pe = g.ep[j];
             const double pe2 = g.e_p2[j];
-            const double beta2 = pe2 / (1.0 + pe2);
+            const double beta2 = g.e_b2[j];
             const double ric = (nseg > 1) ? d_lerp(S.ric_c[j], S.ric_n[j], alpha) : S.ric_c[j];
             const double coul = coul_coeff * (1.0 + log(pe / q.n_gas) / 74.8) / beta2;
             radp[k] = synch_coeff * pe2 + ric + coul + adia * pe;
@@ -753,6 +767,33 @@ __device__ __forceinline__ void seg_radp(const CkGrid &g, const CkSlot &S, int b
             radp[k] = d_b_coulomb_p(q.n_gas, p, q.kbt) + adia * p;
         }
     }
+}
+
+// Proton loss rates (prepare_crp_losses_1d_with_threshold,
+// fp_shared_core.c:729-778) evaluated in the strided row order m G + gl and
+// moved to the lane's own block of rows through LDS.  The thermal (erf)
+// branch of the Coulomb kernel is taken only by the lowest bins; in the
+// block order the lane holding them would make the whole wave take the
+// branch for each of its R rows, in the strided order the wave takes it
+// ceil(rows / G) times.  s_tr holds N + N / R doubles (row j at j + j / R:
+// one pad slot per block of R rows against LDS bank conflicts on the reads).
+template <int N, int R, int G>
+__device__ __forceinline__ void seg_radp_p(const CkGrid &g, int gl, const SegScalars &q, double *s_tr,
+                                           double (&radp)[R])
+{
+#pragma clang fp contract(off)
+    const double adia = 1.0 / 3.0 * q.divv;
+#pragma unroll
+    for (int m = 0; m < R; ++m) {
+        const int j = m * G + gl;
+        s_tr[j + j / R] = d_b_coulomb_p_tab(q.n_gas, q.kbt, g.p_Ep[j], g.p_2mk[j], g.p_ib2[j]) + adia * g.pp[j];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+    for (int k = 0; k < R; ++k) radp[k] = s_tr[gl * (R + 1) + k];
+    __builtin_amdgcn_wave_barrier();
 }
 
 // All runs of one species over one segment: up to three runs of constant
@@ -796,7 +837,7 @@ enum { CK_MODE_E = 0,      // electron_only: protons are exactly zero
 
 template <int N, int R, int MODE>
 __device__ __forceinline__ void snapshot_update(const CkGrid &g, const CkSlot &S, int t, int gl, double *s_row,
-                                                double (&xe)[R], double (&xp)[R], int &bad)
+                                                double *s_tr, double (&xe)[R], double (&xp)[R], int &bad)
 {
     constexpr int G = N / R;
     constexpr bool PROT = (MODE == CK_MODE_EP || MODE == CK_MODE_P_REC);
@@ -889,7 +930,7 @@ __device__ __forceinline__ void snapshot_update(const CkGrid &g, const CkSlot &S
                 for (int k = 0; k < R; ++k) rec[k] = xp[k];
             }
             double radp[R], src[R];
-            seg_radp<false, R>(g, S, base, nseg, alpha, q, radp);
+            seg_radp_p<N, R, G>(g, gl, q, s_tr, radp);
 #pragma unroll
             for (int k = 0; k < R; ++k) src[k] = S.qpi ? dt * S.qpi[roff + k] : 0.0;
             RunCoef rp = rc;
@@ -947,12 +988,14 @@ __device__ __forceinline__ void k_fused_body(const CkGrid &g, const CkSlot *__re
     constexpr int G = N / R;
     constexpr int GPW = 64 / G;
     __shared__ double s_crp[SEC_LDS ? WPB : 1][SEC_LDS ? GPW : 1][SEC_LDS ? N : 1];
+    __shared__ double s_trp[PROT ? WPB : 1][PROT ? GPW : 1][PROT ? N + N / R + 1 : 1];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int grp = lane / G;
     const int gl = lane % G;
     const int base = gl * R;
     double *s_row = &s_crp[SEC_LDS ? wave : 0][SEC_LDS ? grp : 0][0];
+    double *s_tr = &s_trp[PROT ? wave : 0][PROT ? grp : 0][0];
     int bad = 0;
 
     for (;;) {
@@ -976,7 +1019,7 @@ __device__ __forceinline__ void k_fused_body(const CkGrid &g, const CkSlot *__re
             // a task advances one tracer over nslot consecutive snapshots
             for (int sl = task.slot; sl < task.slot + task.nslot; ++sl) {
                 const CkSlot &S = slots[sl];
-                snapshot_update<N, R, MODE>(g, S, t, gl, s_row, xe, xp, bad);
+                snapshot_update<N, R, MODE>(g, S, t, gl, s_row, s_tr, xe, xp, bad);
                 if (ELEC && S.out_e) {
 #pragma unroll
                     for (int k = 0; k < R; ++k) S.out_e[roff + k] = xe[k];

@@ -48,6 +48,10 @@ typedef struct {
      * (0 for j = n-1), 1/h_j, 1/dx_j and 2/p_j */
     const double *p_h, *p_inv_h, *p_inv_dx, *p_2op;
     const double *e_h, *e_inv_h, *e_inv_dx, *e_2op;
+    /* per-bin factors of the loss rates, built on the device with the loss
+     * code's own arithmetic: proton E_kin = gamma m_p - m_p, 2 m_p (gamma - 1),
+     * 1 / beta^2; electron beta^2 */
+    const double *p_Ep, *p_2mk, *p_ib2, *e_b2;
     double ppm1, ppp1, epm1, epp1;
 } CkGrid;
 
@@ -139,6 +143,10 @@ int ck_dgemm_nn(void *handle, int m, int n, int k, const double *A, int lda, con
  * CMB bounds span more nodes than the kernel holds. */
 int ck_ic_rows(int n, const double *ep_dev, double epm1, double epp1, int nz, const double *zb_dev,
                double *rows_dev, double *ghosts_dev, int *err_dev, void *stream);
+
+/* Fills the per-bin loss factors of g (p_Ep, p_2mk, p_ib2, e_b2; writable
+ * device arrays of g.n). */
+int ck_grid_loss_tables(CkGrid g, void *stream);
 
 /* Fill kernel (used instead of memset so it is safe under graph capture). */
 int ck_fill_i32(int *dst, int value, size_t count, void *stream);

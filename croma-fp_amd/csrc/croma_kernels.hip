@@ -350,6 +350,32 @@ __global__ void k_fill(T *dst, T v, size_t count)
         dst[i] = v;
 }
 
+// Per-bin loss factors, with the expressions of d_b_coulomb_p and of the
+// electron beta^2 in seg_radp (so the tabulated forms are bit-identical).
+__global__ void k_grid_loss_tables(CkGrid g)
+{
+#pragma clang fp contract(off)
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= g.n) return;
+    const double p = g.pp[j];
+    const double p2 = p * p;
+    const double gamma = sqrt(1.0 + p2);
+    const double betap2 = p2 / (1.0 + p2);
+    const double kinetic = gamma - 1.0;
+    const_cast<double *>(g.p_Ep)[j] = gamma * K_MP - K_MP;
+    const_cast<double *>(g.p_2mk)[j] = 2.0 * K_MP * kinetic;
+    const_cast<double *>(g.p_ib2)[j] = 1.0 / betap2;
+    const double pe2 = g.e_p2[j];
+    const_cast<double *>(g.e_b2)[j] = pe2 / (1.0 + pe2);
+}
+
+extern "C" int ck_grid_loss_tables(CkGrid g, void *stream)
+{
+    if (g.n <= 0) return 0;
+    hipLaunchKernelGGL(k_grid_loss_tables, dim3((g.n + 255) / 256), dim3(256), 0, (hipStream_t)stream, g);
+    return hip_ok(hipGetLastError(), "k_grid_loss_tables");
+}
+
 extern "C" int ck_fill_i32(int *dst, int v, size_t count, void *stream)
 {
     if (count == 0) return 0;

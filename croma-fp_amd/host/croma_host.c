@@ -204,7 +204,7 @@ static void derived_tables(int n, const double *x, const double *dx, double *h, 
     }
 }
 
-#define CH_NTAB 21
+#define CH_NTAB 25 /* 21 uploaded, 4 loss-factor tables built on the device */
 
 static int ctx_upload_grid(CromaHipContext *c)
 {
@@ -226,7 +226,7 @@ static int ctx_upload_grid(CromaHipContext *c)
         free(tab);
         return dev_fail("grid alloc");
     }
-    if (ck_h2d(c->d_grid_mem, tab, CH_NTAB * nb, c->stream) || ck_stream_sync(c->stream)) {
+    if (ck_h2d(c->d_grid_mem, tab, 21 * nb, c->stream) || ck_stream_sync(c->stream)) {
         free(tab);
         return dev_fail("grid upload");
     }
@@ -255,10 +255,15 @@ static int ctx_upload_grid(CromaHipContext *c)
     c->dgrid.e_inv_h = d + 18 * (size_t)n;
     c->dgrid.e_inv_dx = d + 19 * (size_t)n;
     c->dgrid.e_2op = d + 20 * (size_t)n;
+    c->dgrid.p_Ep = d + 21 * (size_t)n;
+    c->dgrid.p_2mk = d + 22 * (size_t)n;
+    c->dgrid.p_ib2 = d + 23 * (size_t)n;
+    c->dgrid.e_b2 = d + 24 * (size_t)n;
     c->dgrid.ppm1 = c->g.ppm1;
     c->dgrid.ppp1 = c->g.ppp1;
     c->dgrid.epm1 = c->g.epm1;
     c->dgrid.epp1 = c->g.epp1;
+    if (ck_grid_loss_tables(c->dgrid, c->stream) || ck_stream_sync(c->stream)) return dev_fail("grid loss tables");
     return 0;
 }
 
