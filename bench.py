@@ -36,6 +36,11 @@ FP64_PEAK_TFLOPS = 78.6
 FP64_FLOP_PER_ROW = 10
 
 
+def progress(msg):
+    """Stage line on stderr (the JSON result is the only stdout line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def load_traffic(name):
     """HBM bytes per launch measured with rocprofv3 FETCH_SIZE/WRITE_SIZE, from
     the committed summary profiles/<name>.txt ("bytes_per_launch = ...")."""
@@ -61,8 +66,8 @@ def make_batch(ch, ctx, w):
     return b
 
 
-def run_path(ch, ctx, w, path, warmup, steps):
-    b = make_batch(ch, ctx, w)
+def run_path(ch, ctx, w, path, warmup, steps, batch=None):
+    b = batch if batch is not None else make_batch(ch, ctx, w)
     try:
         if warmup:
             b.run(0, warmup, w.nsnap, w.z, w.dt_snap, path=path)
@@ -203,6 +208,7 @@ def main():
     ap.add_argument("--emission-ntracer", type=int, default=1_000_000,
                     help="tracers of the emission contraction workload (0 skips it)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default tile instead of autotuning it")
+    ap.add_argument("--no-graphs", action="store_true", help="plain stream launches instead of replayed hipGraphs")
     ap.add_argument("--tile", default="", help="rows_per_lane,waves_per_simd,chain,secondary_gemm (skips the autotuner)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy (scale of the FETCH_SIZE / WRITE_SIZE counters)")
@@ -236,8 +242,11 @@ def main():
     if args.ntracer:
         cfg["ntracer"] = args.ntracer
     w = synth.make_workload(seed=2026, **cfg)
+    progress(f"workload {args.config}: {w.ntracer} tracers x {w.n} bins, {w.nsnap} snapshots")
     steps = min(args.steps, w.nsnap - args.warmup)
     ctx = ch.Context(w.n, hip_params(w))
+    if args.no_graphs:
+        ctx.set_graphs(False)
     t_tune = time.perf_counter()
     if args.tile:
         r_, w_, c_, g_ = (int(v) for v in args.tile.split(","))
@@ -245,12 +254,18 @@ def main():
         tile = ctx.get_tile(w.protons)
     elif args.no_tune:
         tile = ctx.get_tile(w.protons)
-    else:
-        tile = ctx.autotune(w.protons, ntracer=min(w.ntracer, 16384))
     t_tune = time.perf_counter() - t_tune
+    batch = make_batch(ch, ctx, w)
+    if not args.tile and not args.no_tune:
+        # trials of every tile on the workload's first snapshot (state restored)
+        t_tune = time.perf_counter()
+        tile = batch.autotune(0, w.nsnap, w.z, w.dt_snap)
+        t_tune = time.perf_counter() - t_tune
 
-    wall, t = run_path(ch, ctx, w, ch.PATH_FUSED, args.warmup, steps)
+    progress(f"tile {tile}")
+    wall, t = run_path(ch, ctx, w, ch.PATH_FUSED, args.warmup, steps, batch=batch)
     value = t.cell_steps / wall
+    progress(f"fused: {steps} snapshots in {wall:.2f} s")
     species = 2 if w.protons else 1
 
     out = {
@@ -275,6 +290,8 @@ def main():
         "tracer_substeps": int(t.cell_steps),
         "species_substeps": int(t.species_steps),
         "host_ms": {"ic_rows": t.ic_rows_ms, "nsub": t.nsub_estimate_ms, "order": t.bucket_build_ms},
+        "graphs": {"windows_replayed": int(t.graph_launches), "captured": int(t.graph_captures),
+                   "refused": int(t.graph_failures)},
     }
 
     # roofline of the dominant kernel (k_fused), from per-launch HIP events on
@@ -295,6 +312,7 @@ def main():
     if args.plain_steps > 0:
         wp = w.subset(np.arange(min(args.plain_ntracer, w.ntracer))) if args.plain_ntracer else w
         pw, pt = run_path(ch, ctx, wp, ch.PATH_PLAIN, args.warmup, min(args.plain_steps, steps))
+        progress(f"plain: {pt.cell_steps / pw:.4g} tracer-substeps/s")
         fw, ft = run_path(ch, ctx, wp, ch.PATH_FUSED, args.warmup, min(args.plain_steps, steps))
         out["plain"] = {"value": pt.cell_steps / pw, "snapshots": min(args.plain_steps, steps), "ntracer": wp.ntracer,
                         "fused_same_snapshots": ft.cell_steps / fw,
@@ -303,17 +321,20 @@ def main():
 
     if args.emission_ntracer > 0:
         try:
+            progress("emission")
             out["emission"] = emission_bench(ch, ntracer=args.emission_ntracer)
         except (ch.CromaHipError, MemoryError) as exc:
             out["emission"] = {"error": str(exc)}
 
     if not args.no_serve:
+        progress("served stream")
         try:
             out["served"] = served_latency(ch, seconds=args.serve_seconds)
         except (ImportError, AttributeError, OSError) as exc:  # serving loop not built
             out["served"] = {"error": str(exc)}
 
     if not args.no_cpu:
+        progress("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(w, args.warmup)
     print(json.dumps(out))
 

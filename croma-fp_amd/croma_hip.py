@@ -345,6 +345,13 @@ class Batch:
     def sync(self):
         _chk(lib().croma_hip_batch_sync(C.c_void_p(self.h)), "batch_sync")
 
+    def autotune(self, isnap, nsnap_total, z, dt_snap, elapsed=0.0):
+        """Tile autotuned on snapshot isnap of this batch (state restored); returns the tile tuple."""
+        t = Tile()
+        _chk(lib().croma_hip_batch_autotune(C.c_void_p(self.h), int(isnap), int(nsnap_total), _p(f64(z)),
+                                            _p(f64(dt_snap)), C.c_double(elapsed), C.byref(t)), "batch_autotune")
+        return (t.rows_per_lane, t.waves_per_simd, t.chain, t.secondary_gemm)
+
 
 def solve_cc_batch(a, b, c, source, state, dt):
     nsys, nrow = a.shape

@@ -137,6 +137,117 @@ __global__ __launch_bounds__(256) void k_emis_mfma(int ntr, int K, int nbins, in
     }
 }
 
+// Streaming MFMA contraction for K = 32 KS <= 256 (reduced-precision modes).
+// The kernel columns c0..c0+63 of the block stay in LDS for the block's
+// life; each wave streams tiles of 16 tracers straight from HBM into the
+// registers of the MFMA A operand (no LDS staging of spectra), scales each
+// row by its largest power of two, rounds to bf16 hi (and lo) and runs
+// v_mfma_f32_16x16x32_bf16 over 4 column tiles.
+//
+// Lane (r = lane % 16, q = lane / 16) loads, for k-step ks and m = 0..3, the
+// double pair at k = 32 ks + 8 m + 2 q (+1), so the four q-lanes of a row
+// read 64 contiguous bytes per load.  Its operand slot s = 2 m + (0|1) thus
+// holds k(q, s) = 32 ks + 8 (s / 2) + 2 q + (s & 1); the B operand in LDS is
+// stored with the same k order (a contraction does not depend on the order
+// of k, only on A and B agreeing).
+template <int SPLIT, int KS>
+__global__ __launch_bounds__(256) void k_emis_stream(int ntr, int nbins, int ld_out,
+                                                     const double *__restrict__ spectra,
+                                                     const uint16_t *__restrict__ kt_hi,
+                                                     const uint16_t *__restrict__ kt_lo,
+                                                     const int *__restrict__ col_shift,
+                                                     const double *__restrict__ row_scale,
+                                                     double *__restrict__ out)
+{
+    constexpr int K = 32 * KS;
+    // [ks][q][column][slot]: one 16-byte operand per (ks, q, column)
+    __shared__ __attribute__((aligned(16))) uint16_t b_hi[KS][4][EM_TB][8];
+    __shared__ __attribute__((aligned(16))) uint16_t b_lo[SPLIT ? KS : 1][4][EM_TB][8];
+    const int t = threadIdx.x;
+    const int lane = t & 63, w = t >> 6;
+    const int c0 = blockIdx.y * EM_TB;
+    for (int e = t; e < KS * 4 * EM_TB * 8; e += 256) {
+        const int sl = e & 7, c = (e >> 3) % EM_TB, q = (e / (8 * EM_TB)) & 3, ks = e / (32 * EM_TB);
+        const int k = 32 * ks + 8 * (sl >> 1) + 2 * q + (sl & 1);
+        const size_t g = (size_t)(c0 + c) * K + k;
+        b_hi[ks][q][c][sl] = kt_hi[g];
+        if (SPLIT) b_lo[ks][q][c][sl] = kt_lo[g];
+    }
+    __syncthreads();
+    const int r = lane & 15, q = lane >> 4;
+    const int ntiles = (ntr + 15) / 16;
+    for (int tile = blockIdx.x * 4 + w; tile < ntiles; tile += gridDim.x * 4) {
+        const int i = tile * 16 + r;
+        const bool live = i < ntr;
+        const double *row = spectra + (size_t)(live ? i : 0) * K + 2 * q;
+        double2 a[KS][4];
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+                a[ks][m] = live ? *reinterpret_cast<const double2 *>(row + 32 * ks + 8 * m) : make_double2(0.0, 0.0);
+        // largest binary exponent of the row (frexp convention): the four
+        // lanes of row r are r, r + 16, r + 32, r + 48
+        int e = -100000;
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const double v0 = fabs(a[ks][m].x), v1 = fabs(a[ks][m].y);
+                if (v0 > 0.0 && isfinite(v0)) e = max(e, __builtin_amdgcn_frexp_exp(v0));
+                if (v1 > 0.0 && isfinite(v1)) e = max(e, __builtin_amdgcn_frexp_exp(v1));
+            }
+        e = max(e, __shfl_xor(e, 16, 64));
+        e = max(e, __shfl_xor(e, 32, 64));
+        const int sh = e == -100000 ? 0 : e;
+        floatx4 acc[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[c] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            bf16x8 ah, al;
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const float f0 = (float)ldexp(a[ks][m].x, -sh), f1 = (float)ldexp(a[ks][m].y, -sh);
+                const uint16_t h0 = em_bf16_bits(f0), h1 = em_bf16_bits(f1);
+                ah[2 * m] = __builtin_bit_cast(__bf16, h0);
+                ah[2 * m + 1] = __builtin_bit_cast(__bf16, h1);
+                if (SPLIT) {
+                    al[2 * m] = __builtin_bit_cast(__bf16, em_bf16_bits(f0 - em_bf16_val(h0)));
+                    al[2 * m + 1] = __builtin_bit_cast(__bf16, em_bf16_bits(f1 - em_bf16_val(h1)));
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const bf16x8 bh = *reinterpret_cast<const bf16x8 *>(&b_hi[ks][q][16 * c + r][0]);
+                acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[c], 0, 0, 0);
+                if (SPLIT) {
+                    const bf16x8 bl = *reinterpret_cast<const bf16x8 *>(&b_lo[ks][q][16 * c + r][0]);
+                    acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc[c], 0, 0, 0);
+                    acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[c], 0, 0, 0);
+                }
+            }
+        }
+        // D layout: lane holds rows 4 (lane / 16) + j of the tile, column lane % 16
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int rr = 4 * q + j;
+            const int ii = tile * 16 + rr;
+            const int shr = __shfl(sh, rr, 64);
+            if (ii >= ntr) continue;
+            const double rs = row_scale ? row_scale[ii] : 1.0;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int col = c0 + 16 * c + r;
+                if (col >= nbins) continue;
+                double v = ldexp((double)acc[c][j], shr + col_shift[col]);
+                if (row_scale) v *= rs;
+                out[(size_t)ii * ld_out + col] = v;
+            }
+        }
+    }
+}
+
 // FP32 vector-ALU contraction: 4 x 4 outputs per thread.
 __global__ __launch_bounds__(256) void k_emis_f32(int ntr, int K, int nbins, int ld_out,
                                                   const double *__restrict__ spectra,
@@ -275,12 +386,38 @@ extern "C" int ck_emission_contract(int ntr, int K, int nbins, int nbins_pad, co
                            row_scale, out);
         break;
     case 1:
-        hipLaunchKernelGGL(k_emis_mfma<1>, grid, dim3(256), 0, st, ntr, K, nbins, nbins, spectra, kt_hi, kt_lo,
-                           col_shift, row_scale, out);
-        break;
     case 2:
-        hipLaunchKernelGGL(k_emis_mfma<0>, grid, dim3(256), 0, st, ntr, K, nbins, nbins, spectra, kt_hi, kt_lo,
-                           col_shift, row_scale, out);
+        if (K == 64 || K == 128 || K == 256) {
+            // persistent blocks: two per CU (the B tile of a split K = 256
+            // run takes 64 KB of LDS)
+            static int ncu = 0;
+            if (!ncu && (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, 0) != hipSuccess ||
+                         ncu <= 0))
+                ncu = 256;
+            const int tiles = (ntr + 15) / 16;
+            int gx = 2 * ncu;
+            if (gx > (tiles + 3) / 4) gx = (tiles + 3) / 4;
+            const dim3 gs(gx, nbins_pad / EM_TB);
+#define EM_STREAM(SP, KS_)                                                                                             \
+    hipLaunchKernelGGL((k_emis_stream<SP, KS_>), gs, dim3(256), 0, st, ntr, nbins, nbins, spectra, kt_hi, kt_lo,       \
+                       col_shift, row_scale, out)
+            if (mode == 1) {
+                if (K == 64) EM_STREAM(1, 2);
+                else if (K == 128) EM_STREAM(1, 4);
+                else EM_STREAM(1, 8);
+            } else {
+                if (K == 64) EM_STREAM(0, 2);
+                else if (K == 128) EM_STREAM(0, 4);
+                else EM_STREAM(0, 8);
+            }
+#undef EM_STREAM
+        } else if (mode == 1) {
+            hipLaunchKernelGGL(k_emis_mfma<1>, grid, dim3(256), 0, st, ntr, K, nbins, nbins, spectra, kt_hi, kt_lo,
+                               col_shift, row_scale, out);
+        } else {
+            hipLaunchKernelGGL(k_emis_mfma<0>, grid, dim3(256), 0, st, ntr, K, nbins, nbins, spectra, kt_hi, kt_lo,
+                               col_shift, row_scale, out);
+        }
         break;
     case 3:
         hipLaunchKernelGGL(k_emis_f64, grid, dim3(256), 0, st, ntr, K, nbins, nbins, spectra, kt64, row_scale,
@@ -291,13 +428,30 @@ extern "C" int ck_emission_contract(int ntr, int K, int nbins, int nbins_pad, co
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+// Logarithms of the synchrotron kernel table, transposed to [ib][je][nf]
+// (frequency fastest, so the lanes of a wave read one coalesced row); NaN
+// marks entries that are not positive (linear interpolation there).
+__global__ __launch_bounds__(256) void k_synch_logt(int nlogb, int nfreq, int n, const double *__restrict__ table,
+                                                    double *__restrict__ logt)
+{
+    const size_t o = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (o >= (size_t)nlogb * nfreq * n) return;
+    const int nf = (int)(o % nfreq);
+    const int je = (int)((o / nfreq) % n);
+    const size_t ib = o / ((size_t)nfreq * n);
+    const double v = table[(ib * nfreq + nf) * n + je];
+    logt[o] = v > 0.0 ? log(v) : __builtin_nan("");
+}
+
 // Synchrotron: one wave per tracer, lanes over frequencies.  The kernel is
 // interpolated in log B between the table planes ib, ib+1 as the reference
 // does (log-linear when both entries are positive, else linear); the sum
 // runs over je = 2.. up to the first bin whose successor is not positive.
+// With logt (FP64), the logarithms come from the table of k_synch_logt.
 template <typename T>
 __global__ __launch_bounds__(256) void k_emis_synch(int ntr, int n, int nfreq, int nlogb, double logb_min,
                                                     double inv_dlogb, const double *__restrict__ table,
+                                                    const double *__restrict__ logt,
                                                     const double *__restrict__ cre, const double *__restrict__ dp,
                                                     const double *__restrict__ b_field, double *__restrict__ eps)
 {
@@ -334,34 +488,56 @@ __global__ __launch_bounds__(256) void k_emis_synch(int ntr, int n, int nfreq, i
     const T t = (T)tt;
     const double A = sqrt(3.0) * pow(4.8e-10, 3.0) * b_field[it] / (2.0 * 9.11e-28 * 2.999e10 * 2.999e10) /
                      (4.0 * 3.14159265358979323846);
+    const bool inner = logt && tt > 0.0 && tt < 1.0;
     for (int nf = lane; nf < nfreq; nf += 64) {
         const double *r0 = table + ((size_t)ib * nfreq + nf) * n;
         const double *r1 = table + ((size_t)(ib + 1) * nfreq + nf) * n;
         T integral = 0;
-        for (int je = 2; je < jend; ++je) {
-            const T k0 = (T)r0[je], k1 = (T)r1[je];
-            T K;
-            if (tt <= 0.0) K = k0;
-            else if (tt >= 1.0) K = k1;
-            else if (k0 > (T)0 && k1 > (T)0) K = exp(((T)1 - t) * log(k0) + t * log(k1));
-            else K = ((T)1 - t) * k0 + t * k1;
-            integral += s_w[w][je] * K;
+        if (inner) {
+            const double *l0p = logt + (size_t)ib * n * nfreq + nf;
+            const double *l1p = l0p + (size_t)n * nfreq;
+            for (int je = 2; je < jend; ++je) {
+                const double l0 = l0p[(size_t)je * nfreq], l1 = l1p[(size_t)je * nfreq];
+                T K;
+                if (!__builtin_isnan(l0) && !__builtin_isnan(l1)) {
+                    K = exp(((T)1 - t) * (T)l0 + t * (T)l1);
+                } else {
+                    const T k0 = (T)r0[je], k1 = (T)r1[je];
+                    K = ((T)1 - t) * k0 + t * k1;
+                }
+                integral += s_w[w][je] * K;
+            }
+        } else {
+            for (int je = 2; je < jend; ++je) {
+                const T k0 = (T)r0[je], k1 = (T)r1[je];
+                T K;
+                if (tt <= 0.0) K = k0;
+                else if (tt >= 1.0) K = k1;
+                else if (k0 > (T)0 && k1 > (T)0) K = exp(((T)1 - t) * log(k0) + t * log(k1));
+                else K = ((T)1 - t) * k0 + t * k1;
+                integral += s_w[w][je] * K;
+            }
         }
         eps[(size_t)it * nfreq + nf] = A * (double)integral;
     }
 }
 
 extern "C" int ck_emission_synch(int ntr, int n, int nfreq, int nlogb, double logb_min, double inv_dlogb,
-                                 const double *table, const double *cre, const double *dp, const double *b_field,
-                                 double *eps, int fp64, void *stream)
+                                 const double *table, double *logt, const double *cre, const double *dp,
+                                 const double *b_field, double *eps, int fp64, void *stream)
 {
     if (ntr <= 0 || n <= 3 || n > 512 || nfreq <= 0 || nlogb < 2) return -1;
     const dim3 grid((ntr + 3) / 4);
+    if (fp64 && logt) {
+        const size_t tot = (size_t)nlogb * nfreq * n;
+        hipLaunchKernelGGL(k_synch_logt, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                           nlogb, nfreq, n, table, logt);
+    }
     if (fp64)
         hipLaunchKernelGGL(k_emis_synch<double>, grid, dim3(256), 0, (hipStream_t)stream, ntr, n, nfreq, nlogb,
-                           logb_min, inv_dlogb, table, cre, dp, b_field, eps);
+                           logb_min, inv_dlogb, table, (const double *)logt, cre, dp, b_field, eps);
     else
         hipLaunchKernelGGL(k_emis_synch<float>, grid, dim3(256), 0, (hipStream_t)stream, ntr, n, nfreq, nlogb,
-                           logb_min, inv_dlogb, table, cre, dp, b_field, eps);
+                           logb_min, inv_dlogb, table, (const double *)nullptr, cre, dp, b_field, eps);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }

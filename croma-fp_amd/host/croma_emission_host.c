@@ -193,8 +193,8 @@ int croma_hip_emission_synch(const CromaHipContext *ctx, int ntracer, int nfreq,
         return CROMA_HIP_EINVAL;
     g = ch_ctx_grid(ctx);
     n = g->n;
-    d = (char *)ck_malloc(nt_tab * n * 8 + (size_t)ntracer * n * 8 + (size_t)n * 8 + (size_t)ntracer * 8 +
-                          (size_t)ntracer * nfreq * 8 + 5 * 256);
+    d = (char *)ck_malloc(2 * nt_tab * n * 8 + (size_t)ntracer * n * 8 + (size_t)n * 8 + (size_t)ntracer * 8 +
+                          (size_t)ntracer * nfreq * 8 + 6 * 256);
     st = ck_stream_create();
     e0 = ck_event_create();
     e1 = ck_event_create();
@@ -204,9 +204,11 @@ int croma_hip_emission_synch(const CromaHipContext *ctx, int ntracer, int nfreq,
         goto out;
     }
     {
-        double *d_tab, *d_cre, *d_dp, *d_b, *d_eps;
+        double *d_tab, *d_logt, *d_cre, *d_dp, *d_b, *d_eps;
         p = d;
         d_tab = (double *)p;
+        p += (nt_tab * n * 8 + 255) & ~(size_t)255;
+        d_logt = (double *)p;
         p += (nt_tab * n * 8 + 255) & ~(size_t)255;
         d_cre = (double *)p;
         p += ((size_t)ntracer * n * 8 + 255) & ~(size_t)255;
@@ -218,8 +220,8 @@ int croma_hip_emission_synch(const CromaHipContext *ctx, int ntracer, int nfreq,
         if (ck_h2d(d_tab, table, nt_tab * n * 8, st) || ck_h2d(d_cre, cre, (size_t)ntracer * n * 8, st) ||
             ck_h2d(d_dp, g->edp, (size_t)n * 8, st) || ck_h2d(d_b, b_field, (size_t)ntracer * 8, st) ||
             ck_event_record(e0, st) ||
-            ck_emission_synch(ntracer, n, nfreq, nlogb, logb_min, inv_dlogb, d_tab, d_cre, d_dp, d_b, d_eps, 1,
-                              st) ||
+            ck_emission_synch(ntracer, n, nfreq, nlogb, logb_min, inv_dlogb, d_tab, d_logt, d_cre, d_dp, d_b, d_eps,
+                              1, st) ||
             ck_event_record(e1, st) || ck_d2h(eps, d_eps, (size_t)ntracer * nfreq * 8, st) || ck_stream_sync(st)) {
             rc = CROMA_HIP_EDEVICE;
             ch_set_error("synch emission: %s", ck_last_error());

@@ -1750,17 +1750,101 @@ static double at_normal(unsigned long long *s)
     return sqrt(-2.0 * log(u1)) * cos(2.0 * 3.14159265358979323846 * u2);
 }
 
-int croma_hip_autotune(CromaHipContext *c, int protons, int ntracer, CromaHipTile *best)
+/* Candidate tiles of the autotuners: rows per lane x waves per SIMD (x the
+ * split secondary contraction for two species), one snapshot per task. */
+static int tile_candidates(int n, int protons, CromaHipTile *out)
 {
     static const int rows[4] = {2, 4, 8, 16};
+    int r, w, g, k = 0;
+    for (g = protons ? 1 : 0; g >= 0; g--)
+        for (r = 0; r < 4; r++) {
+            if (!ck_fused_supported(n, rows[r])) continue;
+            for (w = 1; w <= 2; w++) {
+                out[k].rows_per_lane = rows[r];
+                out[k].waves_per_simd = w;
+                out[k].chain = 1;
+                out[k].secondary_gemm = g;
+                k++;
+            }
+        }
+    return k;
+}
+
+/* Ties within 2% go to more rows per lane (fewer cross-lane levels). */
+static int tile_better(double ms, const CromaHipTile *t, double best_ms, const CromaHipTile *bt)
+{
+    return ms < best_ms * 0.98 || (ms < best_ms * 1.02 && t->rows_per_lane > bt->rows_per_lane);
+}
+
+int croma_hip_batch_autotune(CromaHipBatch *b, int s, int nsnap_total, const double *z, const double *dt_snap,
+                             double elapsed, CromaHipTile *best)
+{
+    CromaHipContext *c;
+    CromaHipTile cand[16], keep, bt;
+    double best_ms = 1e300, *d_save = 0;
+    size_t m;
+    int k, nc, rep, rc = 0;
+    if (!b || !z || !dt_snap || !best || s < 0 || s + 1 > b->nsnap_res || nsnap_total < s + 1)
+        return CROMA_HIP_EINVAL;
+    c = b->ctx;
+    m = (size_t)b->ntr * c->g.n;
+    keep = c->tile[b->protons ? 1 : 0];
+    bt = keep;
+    /* every trial starts from the batch's current state */
+    d_save = (double *)ck_malloc(m * sizeof(double) * (b->protons ? 2 : 1));
+    if (!d_save) return dev_fail("autotune state copy");
+    if (ck_d2d(d_save, b->d_cre, m * sizeof(double), c->stream) ||
+        (b->protons && ck_d2d(d_save + m, b->d_crp, m * sizeof(double), c->stream))) {
+        rc = dev_fail("autotune state copy");
+        goto out;
+    }
+    nc = tile_candidates(c->g.n, b->protons, cand);
+    for (k = 0; k < nc; k++) {
+        double ms = 0.0;
+        c->tile[b->protons ? 1 : 0] = cand[k];
+        /* the first run of a shape also loads its code and captures nothing
+         * yet: a short snapshot is timed on its second run */
+        for (rep = 0; rep < 2; rep++) {
+            CromaHipTimes tm;
+            memset(&tm, 0, sizeof(tm));
+            if (ck_d2d(b->d_cre, d_save, m * sizeof(double), c->stream) ||
+                (b->protons && ck_d2d(b->d_crp, d_save + m, m * sizeof(double), c->stream))) {
+                rc = dev_fail("autotune state restore");
+                goto out;
+            }
+            if ((rc = croma_hip_batch_run(b, s, s + 1, nsnap_total, z, dt_snap, elapsed, CROMA_HIP_PATH_FUSED, &tm)) !=
+                0)
+                goto out;
+            ms = tm.kernel_ms;
+            if (ms > 50.0) break;
+        }
+        if (tile_better(ms, &cand[k], best_ms, &bt)) {
+            if (ms < best_ms) best_ms = ms;
+            bt = cand[k];
+        }
+    }
+out:
+    if (d_save) {
+        if (ck_d2d(b->d_cre, d_save, m * sizeof(double), c->stream) ||
+            (b->protons && ck_d2d(b->d_crp, d_save + m, m * sizeof(double), c->stream)) || ck_stream_sync(c->stream))
+            if (!rc) rc = dev_fail("autotune state restore");
+        ck_free(d_save);
+    }
+    c->tile[b->protons ? 1 : 0] = rc ? keep : bt;
+    if (!rc) *best = bt;
+    return rc;
+}
+
+int croma_hip_autotune(CromaHipContext *c, int protons, int ntracer, CromaHipTile *best)
+{
     const int n = c ? c->g.n : 0;
     const int nsnap = 3;
     double *bg = 0, *cre = 0, *crp = 0, z[4], dt[3];
     double best_ms = 1e300;
-    CromaHipTile keep, bt;
+    CromaHipTile keep, bt, cand[16];
     CromaHipBatch *b = 0;
     unsigned long long seed = 0x5EEDULL + (unsigned long long)n;
-    int i, s, f, r, w, rc = 0;
+    int i, s, f, k, nc, rc = 0;
     if (!c || ntracer <= 0 || !best) return CROMA_HIP_EINVAL;
     keep = c->tile[protons ? 1 : 0];
     bt = keep;
@@ -1810,26 +1894,18 @@ int croma_hip_autotune(CromaHipContext *c, int protons, int ntracer, CromaHipTil
         for (f = 0; f < CK_NBG; f++) fl[f] = bg + ((size_t)s * CK_NBG + f) * ntracer;
         if ((rc = croma_hip_batch_set_background(b, s, fl)) != 0) goto out;
     }
-    for (r = 0; r < 4 * (protons ? 2 : 1); r++) {
-        if (!ck_fused_supported(n, rows[r % 4])) continue;
-        for (w = 1; w <= 2; w++) {
-            CromaHipTile t;
-            CromaHipTimes tm;
-            t.rows_per_lane = rows[r % 4];
-            t.waves_per_simd = w;
-            t.chain = 1;
-            t.secondary_gemm = protons ? (r < 4) : 0;
-            c->tile[protons ? 1 : 0] = t;
-            memset(&tm, 0, sizeof(tm));
-            if ((rc = croma_hip_batch_set_state(b, protons ? crp : 0, cre, 0, 0, 0, 0)) != 0) goto out;
-            /* first snapshot warms the instance, the next two are timed */
-            if ((rc = croma_hip_batch_run(b, 0, 1, nsnap + 1, z, dt, 0.0, CROMA_HIP_PATH_FUSED, 0)) != 0) goto out;
-            if ((rc = croma_hip_batch_run(b, 1, nsnap, nsnap + 1, z, dt, dt[0], CROMA_HIP_PATH_FUSED, &tm)) != 0)
-                goto out;
-            if (tm.kernel_ms < best_ms * 0.98 || (tm.kernel_ms < best_ms * 1.02 && rows[r % 4] > bt.rows_per_lane)) {
-                if (tm.kernel_ms < best_ms) best_ms = tm.kernel_ms;
-                bt = t;
-            }
+    nc = tile_candidates(n, protons, cand);
+    for (k = 0; k < nc; k++) {
+        CromaHipTimes tm;
+        c->tile[protons ? 1 : 0] = cand[k];
+        memset(&tm, 0, sizeof(tm));
+        if ((rc = croma_hip_batch_set_state(b, protons ? crp : 0, cre, 0, 0, 0, 0)) != 0) goto out;
+        /* first snapshot warms the instance, the next two are timed */
+        if ((rc = croma_hip_batch_run(b, 0, 1, nsnap + 1, z, dt, 0.0, CROMA_HIP_PATH_FUSED, 0)) != 0) goto out;
+        if ((rc = croma_hip_batch_run(b, 1, nsnap, nsnap + 1, z, dt, dt[0], CROMA_HIP_PATH_FUSED, &tm)) != 0) goto out;
+        if (tile_better(tm.kernel_ms, &cand[k], best_ms, &bt)) {
+            if (tm.kernel_ms < best_ms) best_ms = tm.kernel_ms;
+            bt = cand[k];
         }
     }
 out:

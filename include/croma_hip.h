@@ -210,6 +210,13 @@ int croma_hip_batch_run(CromaHipBatch *b, int isnap_begin, int isnap_end, int ns
                         const double *dt_snap, double elapsed_gyr, int path, CromaHipTimes *times);
 /* Wait for the batch's device work. */
 int croma_hip_batch_sync(CromaHipBatch *b);
+/* Autotune the fused kernel's tile on the batch's own snapshot s: every
+ * candidate (rows per lane x waves per SIMD, x split secondary for two
+ * species) advances snapshot s from the current state, which is restored
+ * after each trial and at the end.  Stores the fastest tile in the context
+ * and returns it.  No reference counterpart (the reference has no tiles). */
+int croma_hip_batch_autotune(CromaHipBatch *b, int isnap, int nsnap_total, const double *z,
+                             const double *dt_snap, double elapsed_gyr, CromaHipTile *best);
 
 /* Batched Chang-Cooper solve (solve_cc_cuda_batch, fp_cuda_backend.h:112-116):
  * state_out = Thomas(a, b, c; dt*source + state_init) per system, cell-major

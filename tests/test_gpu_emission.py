@@ -97,3 +97,29 @@ def test_synch_matches_oracle(ctx128):
     err = rel(eps, ref.reshape(eps.shape))
     print(f"[emission] synch max rel err {err:.3e}", flush=True)
     assert err < 1e-12
+
+
+@pytest.mark.parametrize("n", [64, 256, 512])
+@pytest.mark.parametrize("mode", [ch.EMIS_MFMA_XF32, ch.EMIS_MFMA_BF16, ch.EMIS_FP64])
+def test_ic_grid_sizes(n, mode):
+    """Every spectrum length: K = 64..256 run the streaming MFMA kernel (the
+    kernel table resident in LDS), K = 512 the LDS-tiled one."""
+    o = po.Oracle(n)
+    rng = np.random.default_rng(6)
+    nb = 64
+    fic = po.ic_kernel_flat(o, nb, 8.0, 28.0, 0.4)
+    nt = 1000 + 13
+    q = rng.uniform(2.0, 3.5, nt)
+    cut = 10 ** rng.uniform(2, 6, nt)
+    cre = (o.ep[None, :] ** -q[:, None]) * np.exp(-o.ep[None, :] / cut[:, None]) * 10 ** rng.uniform(-3, 3, nt)[:, None]
+    cre[7] = 0.0  # an empty spectrum
+    ref = po.emission_ic(n, cre.ravel(), o.edp, fic, nb).reshape(nt, nb)
+    ctx = ch.Context(n, ch.default_params())
+    try:
+        eps, ms = ctx.emission_ic(cre, fic, nb, mode)
+    finally:
+        ctx.close()
+    assert np.all(eps[7] == 0.0)
+    err = rel(eps, ref)
+    print(f"[emission] ic n={n} mode={mode} max rel err {err:.3e} ({ms:.3f} ms)", flush=True)
+    assert err < TOL[mode]

@@ -55,3 +55,32 @@ def test_all_tiles_match_oracle(refs, case):
     finally:
         ctx.close()
     print(f"  worst rel err {worst:.3e}", flush=True)
+
+
+def test_batch_autotune_restores_state():
+    """The workload autotuner times every tile on the batch's own snapshot and
+    leaves the state as it found it; the chosen tile then meets the parity bar."""
+    n, nt, ns = 128, 64, 3
+    w = small_workload(n, nt, ns, 3, 1, dsa_frac=0.2, seed=31)
+    crp_ref, cre_ref = run_oracle.run_workload(w, run_oracle.oracle_params(w))
+    ctx = ch.Context(n, hip_params(w))
+    b = ch.Batch(ctx, w.ntracer, w.species, w.nsnap)
+    try:
+        b.set_state(w.crp0, w.cre0, mass=w.mass)
+        for s in range(w.nsnap):
+            b.set_background(s, w.bg[s])
+            if s in w.qpi or s in w.qepri:
+                b.set_sources(s, w.qpi.get(s), w.qepri.get(s))
+        print("[tiles] batch autotune", flush=True)
+        tile = b.autotune(0, w.nsnap, w.z, w.dt_snap)
+        assert tile == ctx.get_tile(True)
+        crp, cre = b.get_state()
+        assert np.array_equal(crp, w.crp0) and np.array_equal(cre, w.cre0)
+        b.run(0, ns, ns, w.z, w.dt_snap)
+        crp, cre = b.get_state()
+    finally:
+        b.close()
+        ctx.close()
+    e_err, e_ok = rel_err(cre, cre_ref)
+    p_err, p_ok = rel_err(crp, crp_ref)
+    assert e_ok and p_ok, f"tile {tile}: rel err e {e_err:.3e} p {p_err:.3e}"
