@@ -260,6 +260,21 @@ class Context:
              "nsub")
         return ns, non
 
+    def nsub_device(self, dt_snap, n_gas, kbt, b_field, divv, ric_lo, ric_hi, l_turb, dv):
+        """The nsub estimate computed on the device."""
+        nt = len(n_gas)
+        ns = np.zeros(nt, np.int32)
+        non = np.zeros(nt, np.int32)
+        _chk(lib().croma_hip_nsub_estimate_device(C.c_void_p(self.h), nt, C.c_double(dt_snap), _p(f64(n_gas)),
+                                                  _p(f64(kbt)), _p(f64(b_field)), _p(f64(divv)), C.c_double(ric_lo),
+                                                  C.c_double(ric_hi), _p(f64(l_turb)), _p(f64(dv)), _p(ns), _p(non)),
+             "nsub_device")
+        return ns, non
+
+    def set_nsub_device(self, enable):
+        """True: batch runs estimate nsub and order the queue on the device."""
+        _chk(lib().croma_hip_context_set_nsub_device(C.c_void_p(self.h), int(bool(enable))), "set_nsub_device")
+
     def secondary_weights(self):
         fqe = np.zeros(self.n * self.n)
         npmin = np.zeros(self.n, np.int32)

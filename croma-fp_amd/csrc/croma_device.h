@@ -281,6 +281,111 @@ __device__ __host__ __forceinline__ int d_choose_segments(int nsub, int allow, i
     return segments;
 }
 
+// ---------------------------------------------------------------------------
+// Substep count of one tracer from its current backgrounds
+// (tracer_fp_prepare_nsub_loss_endpoints, tracer_fp_nsub.c:242-282;
+// compute_nsubsteps_core :94-187; ladders :37-92): loss times p / b(p) at
+// bins 0 and n-1 of both species, dt_sub = safety * min, quantized down to
+// the mantissa ladder, nsub = max(10, ceil(dt_snap / dt_sub)), optional cap;
+// the on-window from f_eddy t_eddy quantized to {1/8, 1/4, 1/3, 1/2, 1}.
+// The loss terms are the endpoint forms of the reference (b_synch,
+// b_Coulomb_e, b_Coulomb_p; FP_Coef.c:438-469), association as written.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double d_nsub_b_synch(double p, double B)
+{
+#pragma clang fp contract(off)
+    const double b_micro = B * 1.0e+6, b_ratio = b_micro / 3.2;
+    return 4.8e-4 * p * p * b_ratio * b_ratio * 9.11e-28 * 3.0e+10 * 3.1536e+16;
+}
+
+__device__ __forceinline__ double d_nsub_b_coulomb_e(double nicm, double pe)
+{
+#pragma clang fp contract(off)
+    const double pe2 = pe * pe, beta2 = pe2 / (1.0 + pe2), A = 3.05e-29;
+    return A * nicm * (1.0 + log(pe / nicm) / 74.8) / (K_ME * K_GEV) * K_C * K_GYR / beta2;
+}
+
+__device__ __forceinline__ double d_quantize_dt(double t, const double *pow10)
+{
+#pragma clang fp contract(off)
+    const double ladder[10] = {1.0, 1.25, 1.6, 2.0, 2.5, 3.2, 4.0, 5.0, 6.4, 8.0};
+    if (!(t > 0.0) || !isfinite(t)) return t;
+    const double e10 = floor(log10(t));
+    const double scale = (e10 >= -40.0 && e10 <= 40.0) ? pow10[(int)e10 + 40] : pow(10.0, e10);
+    const double norm = t / scale;
+    double chosen = ladder[0];
+#pragma unroll
+    for (int i = 0; i < 10; ++i) {
+        if (ladder[i] <= norm) chosen = ladder[i];
+        else break;
+    }
+    return chosen * scale;
+}
+
+__device__ __forceinline__ double d_quantize_on(double f)
+{
+#pragma clang fp contract(off)
+    const double bins[5] = {1.0 / 8.0, 1.0 / 4.0, 1.0 / 3.0, 1.0 / 2.0, 1.0};
+    if (!(f > 0.0) || !isfinite(f)) return 1.0;
+    if (f >= 1.0) return 1.0;
+    double chosen = bins[0], best = fabs(f - chosen);
+#pragma unroll
+    for (int i = 1; i < 5; ++i) {
+        const double d = fabs(f - bins[i]);
+        if (d < best) {
+            chosen = bins[i];
+            best = d;
+        }
+    }
+    return chosen;
+}
+
+__device__ __forceinline__ void d_nsub_one(const CkGrid &g, const CkParams &prm, double dt_snap, double ric_lo,
+                                           double ric_hi, double n_gas, double kbt, double bf, double divv,
+                                           double lturb, double dv, int &ns_out, int &non_out)
+{
+#pragma clang fp contract(off)
+    const int n = g.n;
+    const double pp0 = g.pp[0], ppN = g.pp[n - 1], ep0 = g.ep[0], epN = g.ep[n - 1];
+    const double lo_p = d_b_coulomb_p(n_gas, pp0, kbt) + 1.0 / 3.0 * divv * pp0;
+    const double hi_p = d_b_coulomb_p(n_gas, ppN, kbt) + 1.0 / 3.0 * divv * ppN;
+    const double lo_e = d_nsub_b_synch(ep0, bf) + ric_lo + d_nsub_b_coulomb_e(n_gas, ep0) + 1.0 / 3.0 * divv * ep0;
+    const double hi_e = d_nsub_b_synch(epN, bf) + ric_hi + d_nsub_b_coulomb_e(n_gas, epN) + 1.0 / 3.0 * divv * epN;
+    const double te0 = ep0 / lo_e, tp0 = pp0 / lo_p;
+    const double teN = epN / hi_e, tpN = ppN / hi_p;
+    const double tmin = fmin(fabs(tp0), fmin(fabs(teN), fmin(fabs(te0), fabs(tpN))));
+    double target = dt_snap / 10.0, on_fraction = 1.0;
+    if (isfinite(tmin) && tmin > 0.0) {
+        const double st = prm.tracer_nsub_safety * tmin;
+        if (st < target) target = st;
+    }
+    if (target <= 0.0 || !isfinite(target)) target = dt_snap / 10.0;
+    double used = d_quantize_dt(target, g.pow10);
+    if (!(used > 0.0) || !isfinite(used) || used > target) used = target;
+    double raw = ceil(dt_snap / used);
+    if (!isfinite(raw) || raw > 1000000.0) raw = 1000000.0;
+    if (raw < 10.0) raw = 10.0;
+    int ns = (int)raw;
+    if (prm.tracer_nsub_max > 0 && ns > prm.tracer_nsub_max) ns = prm.tracer_nsub_max;
+    if (prm.reacc_window_mode == 1 && prm.f_eddy > 0.0 && dv > 0.0) {
+        const double t_eddy = lturb * 3.0857e24 / dv / 3.1536e16;
+        if (prm.f_eddy * t_eddy < dt_snap) on_fraction = d_quantize_on(prm.f_eddy * t_eddy / dt_snap);
+    }
+    int non = (int)floor((double)ns * on_fraction + 0.5);
+    if (non < 1) non = 1;
+    if (non > ns) non = ns;
+    ns_out = ns;
+    non_out = non;
+}
+
+// Cost bucket of the task order: 256 buckets per octave of the cost.
+__device__ __forceinline__ int d_cost_bucket(double cost)
+{
+    if (cost < 1.0) return 0;
+    const int k = (int)(log2(cost) * 256.0) + 1;
+    return k >= CK_NSUB_HIST ? CK_NSUB_HIST - 1 : k;
+}
+
 __device__ __forceinline__ double d_lerp(double cur, double nxt, double alpha)
 {
 #pragma clang fp contract(off)

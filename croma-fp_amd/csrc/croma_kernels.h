@@ -52,6 +52,7 @@ typedef struct {
      * code's own arithmetic: proton E_kin = gamma m_p - m_p, 2 m_p (gamma - 1),
      * 1 / beta^2; electron beta^2 */
     const double *p_Ep, *p_2mk, *p_ib2, *e_b2;
+    const double *pow10; /* 10^k for k = -40..40 (index k + 40), as the host's pow gives them */
     double ppm1, ppp1, epm1, epp1;
 } CkGrid;
 
@@ -83,6 +84,8 @@ typedef struct {
      * by mode 2), seg_inje [rows][n] = sum_p seg_rec[row][p] fqe_t[p][e] */
     double *seg_rec;
     const double *seg_inje;
+    /* IC cooling rate at bins 0 and n-1 of z_curr (the nsub estimate's endpoints) */
+    double ric_lo, ric_hi;
     const int *seg_off;
     CkParams prm;
 } CkSlot;
@@ -143,6 +146,21 @@ int ck_dgemm_nn(void *handle, int m, int n, int k, const double *A, int lda, con
  * CMB bounds span more nodes than the kernel holds. */
 int ck_ic_rows(int n, const double *ep_dev, double epm1, double epp1, int nz, const double *zb_dev,
                double *rows_dev, double *ghosts_dev, int *err_dev, void *stream);
+
+/* Substep counts, cost order and split segment rows of one window on the
+ * device: for every tracer and slot q < count, slots[q].nsub / n_on (and
+ * seg_off when split) are written, tasks[] receives the tracers in
+ * descending cost buckets, *cells accumulates sum nsub.  Work buffers:
+ * key [ntr], segloc [ntr], blk [(ntr + 255) / 256 + 1], hist [CK_NSUB_HIST].
+ * The slot array is read on the device (d_slot) and on the host (h_slot,
+ * for the scalars of the launch). */
+#define CK_NSUB_HIST 8192
+int ck_nsub_order(CkGrid g, const CkSlot *d_slot, int count, int ntr, int protons, int split, CkTask *tasks,
+                  int *key, int *segloc, int *blk, int *hist, unsigned long long *cells, void *stream);
+/* The nsub estimate alone for device arrays of ntr tracers. */
+int ck_nsub_arrays(CkGrid g, CkParams prm, int ntr, double dt_snap, double ric_lo, double ric_hi,
+                   const double *n_gas, const double *kbt, const double *b_field, const double *divv,
+                   const double *l_turb, const double *dv, int *nsub, int *n_on, void *stream);
 
 /* Fills the per-bin loss factors of g (p_Ep, p_2mk, p_ib2, e_b2; writable
  * device arrays of g.n). */

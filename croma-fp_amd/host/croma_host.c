@@ -181,6 +181,8 @@ struct CromaHipContext {
     int pws_cap;
     /* window launches replayed as hipGraphs */
     int use_graphs;
+    /* batch runs estimate nsub and order the queue on the device */
+    int nsub_device;
     void **retired; /* graph executables replaced while their last launch may still run */
     int nretired, retired_cap;
     /* croma_hip_execute_snapshot state kept between calls */
@@ -205,12 +207,13 @@ static void derived_tables(int n, const double *x, const double *dx, double *h, 
 }
 
 #define CH_NTAB 25 /* 21 uploaded, 4 loss-factor tables built on the device */
+#define CH_NPOW10 81 /* 10^k, k = -40..40, for the nsub ladder on the device */
 
 static int ctx_upload_grid(CromaHipContext *c)
 {
     const int n = c->g.n;
     const size_t nb = (size_t)n * sizeof(double);
-    double *tab = (double *)malloc(CH_NTAB * nb);
+    double *tab = (double *)malloc(CH_NTAB * nb + CH_NPOW10 * sizeof(double));
     const double *src[13] = {c->g.pp, c->g.pdp, c->g.pE, c->g.p_p2, c->g.p_expcut, c->g.p_sig, c->g.ep,
                              c->g.edp, c->g.eE, c->g.e_p2, c->g.e_expcut, c->g.p_ecr_w, c->g.e_ecr_w};
     const double *d;
@@ -221,12 +224,15 @@ static int ctx_upload_grid(CromaHipContext *c)
                    tab + 16 * (size_t)n);
     derived_tables(n, c->g.ep, c->g.edp, tab + 17 * (size_t)n, tab + 18 * (size_t)n, tab + 19 * (size_t)n,
                    tab + 20 * (size_t)n);
-    c->d_grid_mem = (double *)ck_malloc(CH_NTAB * nb);
+    for (k = 0; k < CH_NPOW10; k++) tab[21 * (size_t)n + k] = pow(10.0, (double)(k - 40));
+    c->d_grid_mem = (double *)ck_malloc(CH_NTAB * nb + CH_NPOW10 * sizeof(double));
     if (!c->d_grid_mem) {
         free(tab);
         return dev_fail("grid alloc");
     }
-    if (ck_h2d(c->d_grid_mem, tab, 21 * nb, c->stream) || ck_stream_sync(c->stream)) {
+    if (ck_h2d(c->d_grid_mem, tab, 21 * nb, c->stream) ||
+        ck_h2d(c->d_grid_mem + CH_NTAB * (size_t)n, tab + 21 * (size_t)n, CH_NPOW10 * sizeof(double), c->stream) ||
+        ck_stream_sync(c->stream)) {
         free(tab);
         return dev_fail("grid upload");
     }
@@ -259,6 +265,7 @@ static int ctx_upload_grid(CromaHipContext *c)
     c->dgrid.p_2mk = d + 22 * (size_t)n;
     c->dgrid.p_ib2 = d + 23 * (size_t)n;
     c->dgrid.e_b2 = d + 24 * (size_t)n;
+    c->dgrid.pow10 = d + CH_NTAB * (size_t)n;
     c->dgrid.ppm1 = c->g.ppm1;
     c->dgrid.ppp1 = c->g.ppp1;
     c->dgrid.epm1 = c->g.epm1;
@@ -337,6 +344,7 @@ CromaHipContext *croma_hip_context_create(const CromaHipGridSpec *gs, const Crom
     }
     default_tiles(c);
     c->use_graphs = 1;
+    c->nsub_device = 1;
     return c;
 }
 
@@ -409,6 +417,14 @@ int croma_hip_context_get_tile(const CromaHipContext *c, int protons, CromaHipTi
 {
     if (!c || !t) return CROMA_HIP_EINVAL;
     *t = c->tile[protons ? 1 : 0];
+    return 0;
+}
+
+int croma_hip_context_set_nsub_device(CromaHipContext *c, int enable)
+{
+    if (!c || (enable != 0 && enable != 1)) return CROMA_HIP_EINVAL;
+    ck_stream_sync(c->stream);
+    c->nsub_device = enable;
     return 0;
 }
 
@@ -547,6 +563,30 @@ int croma_hip_nsub_estimate(const CromaHipContext *c, int ntr, double dt_snap, c
                             nsub, n_on) == 0
                ? 0
                : CROMA_HIP_EINVAL;
+}
+
+int croma_hip_nsub_estimate_device(const CromaHipContext *c, int ntr, double dt_snap, const double *n_gas,
+                                   const double *kbt, const double *b_field, const double *divv, double ric_lo,
+                                   double ric_hi, const double *l_turb, const double *dv, int *nsub, int *n_on)
+{
+    const double *src[6] = {n_gas, kbt, b_field, divv, l_turb, dv};
+    double *d;
+    int *di, k, rc = 0;
+    if (!c || ntr < 0 || !n_gas || !kbt || !b_field || !divv || !l_turb || !dv || !nsub || !n_on || !(dt_snap > 0.0))
+        return CROMA_HIP_EINVAL;
+    if (ntr == 0) return 0;
+    d = (double *)ck_malloc((size_t)ntr * (6 * sizeof(double) + 2 * sizeof(int)));
+    if (!d) return dev_fail("nsub buffers");
+    di = (int *)(d + (size_t)6 * ntr);
+    for (k = 0; k < 6 && !rc; k++)
+        if (ck_h2d(d + (size_t)k * ntr, src[k], (size_t)ntr * sizeof(double), c->stream)) rc = 1;
+    if (rc || ck_nsub_arrays(c->dgrid, c->prm, ntr, dt_snap, ric_lo, ric_hi, d, d + ntr, d + 2 * (size_t)ntr,
+                             d + 3 * (size_t)ntr, d + 4 * (size_t)ntr, d + 5 * (size_t)ntr, di, di + ntr, c->stream) ||
+        ck_d2h(nsub, di, (size_t)ntr * sizeof(int), c->stream) ||
+        ck_d2h(n_on, di + ntr, (size_t)ntr * sizeof(int), c->stream) || ck_stream_sync(c->stream))
+        rc = dev_fail("device nsub");
+    ck_free(d);
+    return rc;
 }
 
 int ch_ctx_fqe(CromaHipContext *c)
@@ -708,6 +748,8 @@ typedef struct {
     int split;               /* the window runs as proton pass + GEMM + electron pass */
     void *ev_up;
     ChGraph graph;           /* launches of the last window shape run from this buffer set */
+    int dev_nsub;            /* nsub, n_on, order and segment rows are built on the device */
+    int *d_key, *d_segloc, *d_blk, *d_hist; /* their work buffers */
 } WinBuf;
 
 struct CromaHipBatch {
@@ -725,6 +767,7 @@ struct CromaHipBatch {
     long segrows_cap;
     long seg_quantum;             /* GEMM rows are rounded up to a multiple of this */
     int *hist, *keys;
+    unsigned long long *d_cells; /* sum nsub of the device-estimated windows of a run */
     void *ev_a, *ev_b;
     void **ev_k; /* per-launch timing events, 2 per window */
     int ev_k_cap;
@@ -743,6 +786,10 @@ static void win_free(CromaHipContext *c, WinBuf *w)
     ck_free(w->d_ric);
     ck_free(w->d_zb);
     ck_free(w->d_segoff);
+    ck_free(w->d_key);
+    ck_free(w->d_segloc);
+    ck_free(w->d_blk);
+    ck_free(w->d_hist);
     ck_host_free(w->h_segoff);
     ck_host_free(w->h_nsub);
     ck_host_free(w->h_non);
@@ -772,9 +819,14 @@ static int win_reserve(CromaHipContext *c, WinBuf *w, int cap, int ntr, int n)
     w->h_zb = (double *)ck_host_alloc((size_t)(cap + 1) * 3 * sizeof(double));
     w->d_segoff = (int *)ck_malloc((size_t)cap * ntr * sizeof(int));
     w->h_segoff = (int *)ck_host_alloc((size_t)cap * ntr * sizeof(int));
+    w->d_key = (int *)ck_malloc((size_t)ntr * sizeof(int));
+    w->d_segloc = (int *)ck_malloc((size_t)ntr * sizeof(int));
+    w->d_blk = (int *)ck_malloc(((size_t)ntr / 256 + 2) * sizeof(int));
+    w->d_hist = (int *)ck_malloc((size_t)CK_NSUB_HIST * sizeof(int));
     w->ev_up = ck_event_create();
     if (!w->d_nsub || !w->d_non || !w->d_tasks || !w->d_slot || !w->d_ric || !w->d_zb || !w->h_nsub ||
-        !w->h_non || !w->h_tasks || !w->h_slot || !w->h_zb || !w->ev_up || !w->d_segoff || !w->h_segoff) {
+        !w->h_non || !w->h_tasks || !w->h_slot || !w->h_zb || !w->ev_up || !w->d_segoff || !w->h_segoff ||
+        !w->d_key || !w->d_segloc || !w->d_blk || !w->d_hist) {
         win_free(c, w);
         return dev_fail("window buffers");
     }
@@ -800,6 +852,7 @@ void croma_hip_batch_destroy(CromaHipBatch *b)
     ck_free(b->d_idx);
     ck_free(b->d_segrec);
     ck_free(b->d_seginje);
+    ck_free(b->d_cells);
     free(b->h_bg);
     free(b->src_set);
     free(b->hist);
@@ -840,8 +893,9 @@ CromaHipBatch *croma_hip_batch_create(CromaHipContext *c, int ntr, int species, 
     b->src_set = (unsigned char *)calloc((size_t)nsnap_res, 1);
     b->ev_a = ck_event_create();
     b->ev_b = ck_event_create();
+    b->d_cells = (unsigned long long *)ck_malloc(256);
     if (!b->d_cre || (b->protons && !b->d_crp) || !b->d_bg || !b->h_bg || !b->hist || !b->keys || !b->src_set ||
-        !b->ev_a || !b->ev_b)
+        !b->ev_a || !b->ev_b || !b->d_cells)
         goto fail;
     for (k = 0; k < 2; k++)
         if (win_reserve(c, &b->win[k], 1, ntr, n)) goto fail;
@@ -976,7 +1030,7 @@ typedef struct {
 } PrepStats;
 
 static int prepare_window(CromaHipBatch *b, WinBuf *w, int s_first, int count, int nsnap_total, const double *z,
-                          const double *dt_snap, double elapsed, int split, PrepStats *ps)
+                          const double *dt_snap, double elapsed, int split, int dev_nsub, PrepStats *ps)
 {
     CromaHipContext *c = b->ctx;
     const int n = c->g.n, ntr = b->ntr;
@@ -990,6 +1044,7 @@ static int prepare_window(CromaHipBatch *b, WinBuf *w, int s_first, int count, i
         if (!allow_v) return CROMA_HIP_ENOMEM;
     }
     w->split = split;
+    w->dev_nsub = dev_nsub;
     w->segrows = 0;
     /* IC rows at z_first .. z_last (+ z_last+1 when the last snapshot has a
      * successor) are built on the device; the host only evaluates the two
@@ -1016,12 +1071,14 @@ static int prepare_window(CromaHipBatch *b, WinBuf *w, int s_first, int count, i
         t0 = ch_now_s();
         ch_ctx_ic_ends(c, z[s], ends);
         ps->ic_ms += 1e3 * (ch_now_s() - t0);
-        t0 = ch_now_s();
-        ch_nsub_estimate(&c->g, &c->prm, ntr, dt_snap[s], ends[0], ends[1], hb + 0 * (size_t)ntr,
-                         hb + 1 * (size_t)ntr, hb + 2 * (size_t)ntr, hb + 3 * (size_t)ntr, hb + 4 * (size_t)ntr,
-                         hb + 5 * (size_t)ntr, nsub, non);
-        ps->nsub_ms += 1e3 * (ch_now_s() - t0);
-        for (i = 0; i < ntr; i++) cs += nsub[i];
+        if (!dev_nsub) {
+            t0 = ch_now_s();
+            ch_nsub_estimate(&c->g, &c->prm, ntr, dt_snap[s], ends[0], ends[1], hb + 0 * (size_t)ntr,
+                             hb + 1 * (size_t)ntr, hb + 2 * (size_t)ntr, hb + 3 * (size_t)ntr, hb + 4 * (size_t)ntr,
+                             hb + 5 * (size_t)ntr, nsub, non);
+            ps->nsub_ms += 1e3 * (ch_now_s() - t0);
+            for (i = 0; i < ntr; i++) cs += nsub[i];
+        }
 
         memset(S, 0, sizeof(*S));
         S->crp = b->d_crp;
@@ -1052,24 +1109,33 @@ static int prepare_window(CromaHipBatch *b, WinBuf *w, int s_first, int count, i
         S->dt_snap = dt_snap[s];
         S->elapsed = elapsed;
         S->prm = c->prm;
+        S->ric_lo = ends[0];
+        S->ric_hi = ends[1];
         if (split) {
             int *off = w->h_segoff + (size_t)q * ntr;
-            for (i = 0; i < ntr; i++) {
-                off[i] = (int)w->segrows;
-                w->segrows += ch_choose_segments(&c->prm, nsub[i], al);
-            }
+            if (!dev_nsub)
+                for (i = 0; i < ntr; i++) {
+                    off[i] = (int)w->segrows;
+                    w->segrows += ch_choose_segments(&c->prm, nsub[i], al);
+                }
             S->seg_off = w->d_segoff + (size_t)q * ntr;
             S->seg_rec = b->d_segrec;
             S->seg_inje = b->d_seginje;
         }
         elapsed += dt_snap[s];
     }
-    w->segq = split ? (w->segrows + b->seg_quantum - 1) / b->seg_quantum * b->seg_quantum : 0;
-    t0 = ch_now_s();
-    order_tasks(&c->prm, allow_v, count, ntr, w->h_nsub, b->protons, w->h_tasks, b->hist, b->keys);
-    ps->order_ms += 1e3 * (ch_now_s() - t0);
-    ps->cell_steps += cs;
-    ps->species_steps += cs * (b->protons ? 2 : 1);
+    if (!dev_nsub) {
+        w->segq = split ? (w->segrows + b->seg_quantum - 1) / b->seg_quantum * b->seg_quantum : 0;
+        t0 = ch_now_s();
+        order_tasks(&c->prm, allow_v, count, ntr, w->h_nsub, b->protons, w->h_tasks, b->hist, b->keys);
+        ps->order_ms += 1e3 * (ch_now_s() - t0);
+        ps->cell_steps += cs;
+        ps->species_steps += cs * (b->protons ? 2 : 1);
+    } else {
+        /* the device assigns the segment rows: the GEMM covers every row the
+         * batch can hold (rows past this window's count are stale and unread) */
+        w->segq = split ? b->segrows_cap : 0;
+    }
     w->count = count;
     if (allow_v != allow) free(allow_v);
     return 0;
@@ -1079,12 +1145,13 @@ static int upload_window(CromaHipBatch *b, WinBuf *w)
 {
     CromaHipContext *c = b->ctx;
     const int ntr = b->ntr, n = c->g.n, q = w->count;
-    if (ck_h2d(w->d_nsub, w->h_nsub, (size_t)q * ntr * sizeof(int), c->stream) ||
-        ck_h2d(w->d_non, w->h_non, (size_t)q * ntr * sizeof(int), c->stream) ||
-        ck_h2d(w->d_tasks, w->h_tasks, (size_t)ntr * sizeof(CkTask), c->stream) ||
+    const int host = !w->dev_nsub;
+    if ((host && ck_h2d(w->d_nsub, w->h_nsub, (size_t)q * ntr * sizeof(int), c->stream)) ||
+        (host && ck_h2d(w->d_non, w->h_non, (size_t)q * ntr * sizeof(int), c->stream)) ||
+        (host && ck_h2d(w->d_tasks, w->h_tasks, (size_t)ntr * sizeof(CkTask), c->stream)) ||
         ck_h2d(w->d_slot, w->h_slot, (size_t)q * sizeof(CkSlot), c->stream) ||
         ck_h2d(w->d_zb, w->h_zb, (size_t)w->nz * 3 * sizeof(double), c->stream) ||
-        (w->split && ck_h2d(w->d_segoff, w->h_segoff, (size_t)q * ntr * sizeof(int), c->stream)) ||
+        (host && w->split && ck_h2d(w->d_segoff, w->h_segoff, (size_t)q * ntr * sizeof(int), c->stream)) ||
         ck_event_record(w->ev_up, c->stream))
         return dev_fail("window upload");
     if (ck_ic_rows(n, c->dgrid.ep, c->g.epm1, c->g.epp1, w->nz, w->d_zb, w->d_ric, 0, c->d_err, c->stream))
@@ -1140,6 +1207,10 @@ static int issue_window(void *arg)
 {
     WinIssue *wi = (WinIssue *)arg;
     CromaHipBatch *b = wi->b;
+    WinBuf *w = wi->w;
+    if (w->dev_nsub && ck_nsub_order(b->ctx->dgrid, w->d_slot, w->count, b->ntr, b->protons, w->split, w->d_tasks,
+                                     w->d_key, w->d_segloc, w->d_blk, w->d_hist, b->d_cells, b->ctx->stream))
+        return dev_fail("device nsub");
     if (wi->w->split) return launch_split(b, wi->w);
     return launch_tasks(b->ctx, b->protons, wi->w->d_slot, wi->w->d_tasks, b->ntr);
 }
@@ -1157,7 +1228,7 @@ static int launch_window(CromaHipBatch *b, WinBuf *w, GraphStats *gs)
     key[3] = t->waves_per_simd;
     key[4] = (long)(size_t)b->d_segrec;
     key[5] = b->protons;
-    key[6] = c->jp0;
+    key[6] = c->jp0 | ((long)w->dev_nsub << 20) | ((long)w->count << 21);
     key[7] = b->ntr;
     wi.b = b;
     wi.w = w;
@@ -1375,7 +1446,7 @@ int croma_hip_batch_run(CromaHipBatch *b, int s0, int s1, int nsnap_total, const
     PrepStats ps;
     GraphStats gst;
     double t_start, el = elapsed;
-    int s, rc = 0, k = 0, wsz, nwin, iw, split;
+    int s, rc = 0, k = 0, wsz, nwin, iw, split, dev_nsub;
     float dev_ms = 0.0f;
     double kern_ms = 0.0;
     if (!b || !z || !dt_snap || s0 < 0 || s1 < s0 || s1 > b->nsnap_res || nsnap_total < s1 ||
@@ -1388,6 +1459,8 @@ int croma_hip_batch_run(CromaHipBatch *b, int s0, int s1, int nsnap_total, const
     /* the split launches need the species decoupled within a segment: no eta cap */
     split = path == CROMA_HIP_PATH_FUSED && b->protons && c->tile[1].secondary_gemm &&
             !(c->prm.dpp_model != 3 && c->prm.eta_dpp_cap > 0.0);
+    /* the plain path keeps the reference's host-side buckets */
+    dev_nsub = path == CROMA_HIP_PATH_FUSED && c->nsub_device;
     if (split) {
         const long segmax = c->prm.coeff_interp_max_segments > 1 ? c->prm.coeff_interp_max_segments : 1;
         long need = (long)wsz * b->ntr * segmax, q = 256;
@@ -1430,8 +1503,9 @@ int croma_hip_batch_run(CromaHipBatch *b, int s0, int s1, int nsnap_total, const
     memset(&gst, 0, sizeof(gst));
     t_start = ch_now_s();
     if (ck_event_record(b->ev_a, c->stream)) return dev_fail("event");
+    if (dev_nsub && ck_fill_i32((int *)b->d_cells, 0, 2, c->stream)) return dev_fail("cell counter");
     rc = prepare_window(b, &b->win[0], s0, (s1 - s0 < wsz) ? s1 - s0 : wsz, nsnap_total, z, dt_snap, el, split,
-                        &ps);
+                        dev_nsub, &ps);
     if (rc) return rc;
     for (iw = 0, s = s0; s < s1; iw++) {
         WinBuf *w = &b->win[iw & 1];
@@ -1451,12 +1525,20 @@ int croma_hip_batch_run(CromaHipBatch *b, int s0, int s1, int nsnap_total, const
             /* the other window's host buffers were consumed by its upload */
             WinBuf *o = &b->win[(iw + 1) & 1];
             if (ck_event_sync(o->ev_up)) return dev_fail("upload event");
-            rc = prepare_window(b, o, s, (s1 - s < wsz) ? s1 - s : wsz, nsnap_total, z, dt_snap, el, split, &ps);
+            rc = prepare_window(b, o, s, (s1 - s < wsz) ? s1 - s : wsz, nsnap_total, z, dt_snap, el, split, dev_nsub,
+                                &ps);
             if (rc) return rc;
         }
     }
     if (ck_event_record(b->ev_b, c->stream)) return dev_fail("event");
     rc = croma_hip_batch_sync(b);
+    if (!rc && dev_nsub) {
+        unsigned long long cells = 0;
+        if (ck_d2h(&cells, b->d_cells, sizeof(cells), c->stream) || ck_stream_sync(c->stream))
+            return dev_fail("cell counter");
+        ps.cell_steps = (long long)cells;
+        ps.species_steps = (long long)cells * (b->protons ? 2 : 1);
+    }
     dev_ms = ck_event_elapsed_ms(b->ev_a, b->ev_b);
     if (path == CROMA_HIP_PATH_FUSED)
         for (k = 0; k < nwin; k++) kern_ms += ck_event_elapsed_ms(b->ev_k[2 * k], b->ev_k[2 * k + 1]);

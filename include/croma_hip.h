@@ -129,6 +129,10 @@ int croma_hip_context_get_tile(const CromaHipContext *ctx, int protons, CromaHip
  * (fill, fused kernel, GEMM, fused kernel); 0: plain stream launches. No
  * reference counterpart (the reference launches per bucket and segment). */
 int croma_hip_context_set_graphs(CromaHipContext *ctx, int enable);
+/* 1 (default): batch runs estimate nsub (tracer_fp_nsub.c:94-282), order the
+ * queue by cost and assign the split segment rows on the device, without a
+ * host pass over the tracers; 0: on the host (the same integers). */
+int croma_hip_context_set_nsub_device(CromaHipContext *ctx, int enable);
 /* Grid arrays (momentum_bin, FP_Coef.c:291-330); species 0 proton, 1 electron. */
 int croma_hip_context_grid(const CromaHipContext *ctx, int species, double *p, double *dp, double *E,
                            double *pm1, double *pp1);
@@ -170,6 +174,12 @@ int croma_hip_nsub_estimate(const CromaHipContext *ctx, int ntracer, double dt_s
                             const double *n_gas, const double *kbt, const double *b_field,
                             const double *divv, double rad_ic_lo, double rad_ic_hi,
                             const double *l_turb, const double *dv, int *nsubsteps, int *n_onsteps);
+/* The same estimate computed on the device (the batch runs' device step
+ * before the solve; host arrays in and out). */
+int croma_hip_nsub_estimate_device(const CromaHipContext *ctx, int ntracer, double dt_snap,
+                                   const double *n_gas, const double *kbt, const double *b_field,
+                                   const double *divv, double rad_ic_lo, double rad_ic_hi,
+                                   const double *l_turb, const double *dv, int *nsubsteps, int *n_onsteps);
 /* IC cooling row at redshift z (prepare_ic_cooling_row, fp_shared_core.c:844-864). */
 int croma_hip_ic_cooling_row(const CromaHipContext *ctx, double z, double *row, double *m1, double *p1);
 /* The same rows built on the device (the batch path's rows): rows [nz][n],
