@@ -360,6 +360,19 @@ class Batch:
     def sync(self):
         _chk(lib().croma_hip_batch_sync(C.c_void_p(self.h)), "batch_sync")
 
+    def checkpoint_write(self, ckpt_dir, meta, ids):
+        ids = np.ascontiguousarray(ids, np.int32)
+        _chk(_out_lib().croma_hip_batch_checkpoint_write(C.c_void_p(self.h), str(ckpt_dir).encode(), C.byref(meta),
+                                                         _p(ids)), "batch_checkpoint_write")
+
+    def checkpoint_read(self, ckpt_dir, meta, ids):
+        """Loads the checkpoint's spectra into the batch; returns the completed snapshot."""
+        ids = np.ascontiguousarray(ids, np.int32)
+        done = C.c_int()
+        _chk(_out_lib().croma_hip_batch_checkpoint_read(C.c_void_p(self.h), str(ckpt_dir).encode(), C.byref(meta),
+                                                        _p(ids), C.byref(done)), "batch_checkpoint_read")
+        return done.value
+
     def autotune(self, isnap, nsnap_total, z, dt_snap, elapsed=0.0):
         """Tile autotuned on snapshot isnap of this batch (state restored); returns the tile tuple."""
         t = Tile()
@@ -447,6 +460,89 @@ class Server:
         st = ServeStats()
         _chk(lib().croma_hip_serve_stats(C.c_void_p(self.h), C.byref(st)), "serve_stats")
         return {k: getattr(st, k) for k, _ in ServeStats._fields_}
+
+
+OUT_MAP, OUT_TILE = 0, 1
+
+
+class CheckpointMeta(C.Structure):
+    _fields_ = [("mpi_rank", C.c_int), ("world_size", C.c_int), ("ntracer", C.c_int), ("ntracer_global", C.c_int),
+                ("nsnap", C.c_int), ("nfreq", C.c_int), ("np_bins", C.c_int), ("npe_bins", C.c_int),
+                ("nsnp_i", C.c_int), ("nsnp_f", C.c_int), ("completed_snapshot", C.c_int),
+                ("tracer_start", C.c_long)]
+
+
+def _out_lib():
+    L = lib()
+    if not getattr(L, "_out_ready", False):
+        L.croma_hip_output_open.restype = C.c_void_p
+        L.croma_hip_output_open.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                            C.c_int]
+        L.croma_hip_output_write.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp]
+        L.croma_hip_output_close.argtypes = [C.c_void_p]
+        L.croma_hip_batch_write_spectra.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, _dp, _dp]
+        L.croma_hip_write_tracer_ids.argtypes = [C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_int)]
+        ip = C.POINTER(C.c_int)
+        L.croma_hip_checkpoint_write.argtypes = [C.c_char_p, C.POINTER(CheckpointMeta), ip, _dp, _dp]
+        L.croma_hip_checkpoint_read.argtypes = [C.c_char_p, C.POINTER(CheckpointMeta), ip, _dp, _dp, ip]
+        L.croma_hip_batch_checkpoint_write.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(CheckpointMeta), ip]
+        L.croma_hip_batch_checkpoint_read.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(CheckpointMeta), ip, ip]
+        L._out_ready = True
+    return L
+
+
+class Output:
+    """A spectra / emissivity file in the reference layout (OUT_MAP: [tracer][slot][nbin]
+    as <stem>_coreNN.bin; OUT_TILE: [slot][tracer][nbin] as <stem>_tile_coreNN.bin)."""
+
+    def __init__(self, out_dir, stem, rank, mode, ntracer, nbin, nslot, resume=False):
+        self.ntracer, self.nbin, self.nslot = ntracer, nbin, nslot
+        self.h = _out_lib().croma_hip_output_open(str(out_dir).encode(), stem.encode(), int(rank), int(mode),
+                                                  int(ntracer), int(nbin), int(nslot), int(bool(resume)))
+        if not self.h:
+            raise CromaHipError(-10, f"output_open: {last_error()}")
+
+    def write(self, slot, rows, rho=None, mass=None):
+        rows = f64(rows)
+        assert rows.size == self.ntracer * self.nbin
+        _chk(_out_lib().croma_hip_output_write(C.c_void_p(self.h), int(slot), _p(rows),
+                                               _p(f64(rho)) if rho is not None else None,
+                                               _p(f64(mass)) if mass is not None else None), "output_write")
+
+    def write_batch(self, batch, slot, crp_out=None, rho=None, mass=None):
+        """The batch's device spectra (electrons here, protons into crp_out)."""
+        _chk(_out_lib().croma_hip_batch_write_spectra(C.c_void_p(batch.h), C.c_void_p(self.h),
+                                                      C.c_void_p(crp_out.h) if crp_out is not None else None,
+                                                      int(slot), _p(f64(rho)) if rho is not None else None,
+                                                      _p(f64(mass)) if mass is not None else None),
+             "batch_write_spectra")
+
+    def close(self):
+        if self.h:
+            h, self.h = self.h, None
+            _chk(_out_lib().croma_hip_output_close(C.c_void_p(h)), "output_close")
+
+
+def write_tracer_ids(out_dir, rank, ids):
+    ids = np.ascontiguousarray(ids, np.int32)
+    _chk(_out_lib().croma_hip_write_tracer_ids(str(out_dir).encode(), int(rank), ids.size, _p(ids)),
+         "write_tracer_ids")
+
+
+def checkpoint_write(ckpt_dir, meta, ids, crp, cre):
+    ids = np.ascontiguousarray(ids, np.int32)
+    _chk(_out_lib().croma_hip_checkpoint_write(str(ckpt_dir).encode(), C.byref(meta), _p(ids), _p(f64(crp)),
+                                               _p(f64(cre))), "checkpoint_write")
+
+
+def checkpoint_read(ckpt_dir, meta, ids):
+    ids = np.ascontiguousarray(ids, np.int32)
+    crp = np.zeros(meta.ntracer * meta.np_bins)
+    cre = np.zeros(meta.ntracer * meta.npe_bins)
+    done = C.c_int()
+    _chk(_out_lib().croma_hip_checkpoint_read(str(ckpt_dir).encode(), C.byref(meta), _p(ids), _p(crp), _p(cre),
+                                              C.byref(done)), "checkpoint_read")
+    return crp, cre, done.value
 
 
 def chunk_bytes(ntracer, n, species, nsnap):

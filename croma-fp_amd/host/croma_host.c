@@ -954,6 +954,18 @@ int croma_hip_batch_set_state(CromaHipBatch *b, const double *crp, const double 
     return ck_stream_sync(c->stream) ? dev_fail("state sync") : 0;
 }
 
+int ch_batch_state(CromaHipBatch *b, ChBatchState *out)
+{
+    if (!b || !out) return CROMA_HIP_EINVAL;
+    out->d_cre = b->d_cre;
+    out->d_crp = b->protons ? b->d_crp : 0;
+    out->ntr = b->ntr;
+    out->n = b->ctx->g.n;
+    out->protons = b->protons;
+    out->stream = b->ctx->stream;
+    return 0;
+}
+
 int croma_hip_batch_get_state(CromaHipBatch *b, double *crp, double *cre)
 {
     CromaHipContext *c;

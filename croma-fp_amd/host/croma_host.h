@@ -65,4 +65,12 @@ int ch_ctx_jp0(const CromaHipContext *c);
 void ch_ctx_ic_ends(CromaHipContext *c, double z, double ends[2]); /* IC rate at bins 0, n-1 (cached) */
 int ch_cost_bucket(double cost);                 /* log-scale bucket in [0, CH_HIST) */
 
+/* device-resident state of a batch (read by the file writers) */
+typedef struct {
+    double *d_cre, *d_crp; /* [ntr * n]; d_crp is NULL for electron-only batches */
+    int ntr, n, protons;
+    void *stream;
+} ChBatchState;
+int ch_batch_state(CromaHipBatch *b, ChBatchState *out);
+
 #endif

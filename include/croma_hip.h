@@ -33,6 +33,7 @@ extern "C" {
 #define CROMA_HIP_ECANCELED (-7)   /* request was cancelled                 */
 #define CROMA_HIP_EUNSUPPORTED (-8)/* configuration this build does not run */
 #define CROMA_HIP_EBUSY (-9)       /* result not ready yet                   */
+#define CROMA_HIP_EIO (-10)        /* file error or file that does not match */
 
 /* Backend selector value for the reference's `backend = auto|cpu|cuda`
  * switch (tracer_fp_setup.c:401-416, include/tracer_fp_types.h:173-177). */
@@ -255,6 +256,61 @@ int croma_hip_emission_hadronic(const CromaHipContext *ctx, int ntracer, int nbi
 int croma_hip_emission_synch(const CromaHipContext *ctx, int ntracer, int nfreq, int nlogb,
                              double logb_min, double inv_dlogb, const double *table, const double *cre,
                              const double *b_field, double *eps, double *elapsed_ms);
+
+/* ---- output and checkpoint files (tracer_fp_output.c, tracer_fp_restart.c)
+ * The reference's byte layouts, float64 native-endian:
+ *   CROMA_HIP_OUT_MAP   <dir>/<stem>_core%02d.bin       [tracer][slot][nbin]
+ *                       (tracer_fp_map_open :613-664; the same file and
+ *                       layout as the buffered CRE/CRP_core%02d.bin writer
+ *                       output_CRspectrum_buffer_node :38-72)
+ *   CROMA_HIP_OUT_TILE  <dir>/<stem>_tile_core%02d.bin  [slot][tracer][nbin]
+ *                       (tracer_fp_tile_open :353-416)
+ * Spectra use nslot = nsnap + 1 (slot 0 = initial state), emissivities
+ * nslot = nsnap; stems CRE, CRP, eSyn, eIC, eGamma, eNu.  Files are sized
+ * at open (resume_existing: the file must exist with that size). -------- */
+#define CROMA_HIP_OUT_MAP 0
+#define CROMA_HIP_OUT_TILE 1
+typedef struct CromaHipOutput CromaHipOutput;
+CromaHipOutput *croma_hip_output_open(const char *dir, const char *stem, int rank, int mode, int ntracer,
+                                      int nbin, int nslot, int resume_existing);
+/* Rows [ntracer][nbin] (host) into slot.  rho_gcc and tracer_mass both
+ * non-NULL select per-cm^3 output: rows scaled by rho / (mass M_sun), 0 for a
+ * tracer without positive rho and mass (output_volume_factor :175-188). */
+int croma_hip_output_write(CromaHipOutput *out, int slot, const double *rows, const double *rho_gcc,
+                           const double *tracer_mass);
+/* The batch's device-resident spectra into slot of cre_out (and crp_out,
+ * may be NULL): staged through pinned host memory in chunks, the copy of
+ * one chunk overlapping the file write of the previous one
+ * (tracer_fp_write_cr_map :249-296, tracer_fp_tile_write_cr :439-477). */
+int croma_hip_batch_write_spectra(CromaHipBatch *b, CromaHipOutput *cre_out, CromaHipOutput *crp_out, int slot,
+                                  const double *rho_gcc, const double *tracer_mass);
+int croma_hip_output_close(CromaHipOutput *out);
+/* <dir>/tracerid_core%02d.txt, one "%09d" id per line (output_tracerid_node :74-96). */
+int croma_hip_write_tracer_ids(const char *dir, int rank, int ntracer, const int *ids);
+
+/* Checkpoint TFPCHK1 (tracer_fp_ckpt_write / _read, tracer_fp_restart.c:136-265):
+ * <dir>/checkpoint_rank%02d.bin = the 72-byte header (TracerFpCheckpointHeader,
+ * include/tracer_fp_types.h:356-371), int ids[ntracer], crp[ntracer * np_bins],
+ * cre[ntracer * npe_bins]; written to a .tmp file, synced and renamed.  Reads
+ * check every header field and the tracer order against meta / ids. */
+typedef struct {
+    int mpi_rank, world_size;
+    int ntracer, ntracer_global;
+    int nsnap, nfreq;
+    int np_bins, npe_bins;
+    int nsnp_i, nsnp_f;
+    int completed_snapshot; /* written; ignored by reads */
+    long tracer_start;
+} CromaHipCheckpointMeta;
+int croma_hip_checkpoint_write(const char *dir, const CromaHipCheckpointMeta *meta, const int *ids,
+                               const double *crp, const double *cre);
+int croma_hip_checkpoint_read(const char *dir, const CromaHipCheckpointMeta *meta, const int *ids, double *crp,
+                              double *cre, int *completed_snapshot);
+/* The same from / into a batch's device state (np_bins = npe_bins = the grid). */
+int croma_hip_batch_checkpoint_write(CromaHipBatch *b, const char *dir, const CromaHipCheckpointMeta *meta,
+                                     const int *ids);
+int croma_hip_batch_checkpoint_read(CromaHipBatch *b, const char *dir, const CromaHipCheckpointMeta *meta,
+                                    const int *ids, int *completed_snapshot);
 
 /* ---- serving: continuous batching of tracer chunks ------------------ */
 typedef struct CromaHipServer CromaHipServer;

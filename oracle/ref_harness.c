@@ -22,6 +22,9 @@
 #include "FP_Coef.h"
 #include "fp_shared_core.h"
 #include "tracer_fp_nsub.h"
+#include "tracer_fp_types.h"
+
+#include <stddef.h>
 
 #define REF_API __attribute__((visibility("default")))
 
@@ -211,4 +214,30 @@ REF_API int ref_synch_table(int nfreq, int ntheta, int nlogb, double logb_min, d
     rc = prepare_synch_pitch_kernel_table(&in, table);
     ref_free_planes(in.logy);
     return rc;
+}
+
+/* Byte layout of the reference's checkpoint header TracerFpCheckpointHeader
+ * (include/tracer_fp_types.h:356-371): out[0] = sizeof, out[1..14] = offsetof
+ * of magic, version, mpi_rank, world_size, ntracer, ntracer_global, nsnap,
+ * nfreq, np_bins, npe_bins, nsnp_i_value, nsnp_f_value, completed_snapshot,
+ * tracer_start; out[15] = sizeof(magic).  Returns the number written. */
+REF_API int ref_ckpt_header_layout(long *out)
+{
+    out[0] = (long)sizeof(TracerFpCheckpointHeader);
+    out[1] = (long)offsetof(TracerFpCheckpointHeader, magic);
+    out[2] = (long)offsetof(TracerFpCheckpointHeader, version);
+    out[3] = (long)offsetof(TracerFpCheckpointHeader, mpi_rank);
+    out[4] = (long)offsetof(TracerFpCheckpointHeader, world_size);
+    out[5] = (long)offsetof(TracerFpCheckpointHeader, ntracer);
+    out[6] = (long)offsetof(TracerFpCheckpointHeader, ntracer_global);
+    out[7] = (long)offsetof(TracerFpCheckpointHeader, nsnap);
+    out[8] = (long)offsetof(TracerFpCheckpointHeader, nfreq);
+    out[9] = (long)offsetof(TracerFpCheckpointHeader, np_bins);
+    out[10] = (long)offsetof(TracerFpCheckpointHeader, npe_bins);
+    out[11] = (long)offsetof(TracerFpCheckpointHeader, nsnp_i_value);
+    out[12] = (long)offsetof(TracerFpCheckpointHeader, nsnp_f_value);
+    out[13] = (long)offsetof(TracerFpCheckpointHeader, completed_snapshot);
+    out[14] = (long)offsetof(TracerFpCheckpointHeader, tracer_start);
+    out[15] = (long)sizeof(((TracerFpCheckpointHeader *)0)->magic);
+    return 16;
 }

@@ -14,7 +14,10 @@ tracer_fp_synch.c:53-56, 95-97) with scipy, using the reference's own
 asymptotic branch for x > 300 (Synchrotron.c:36).  GSL is absent here, so
 F(x) itself is checked only against its asymptotes (parity unpinned vs GSL).
 
-Usage:  python tests/golden/make_golden.py
+It also records the byte layout of the reference's checkpoint header
+(ckpt_header_layout.json) from the compiled struct.
+
+Usage:  python tests/golden/make_golden.py [--layout-only]
 """
 from __future__ import annotations
 
@@ -214,11 +217,29 @@ def make_size(n, logx, logfx, fx, xmin, rng):
     return out
 
 
+def ckpt_layout():
+    """sizeof / offsetof of the reference's TracerFpCheckpointHeader
+    (include/tracer_fp_types.h:356-371), compiled into the harness."""
+    import json
+    lib = ref_lib(128)
+    out = (C.c_long * 16)()
+    cnt = lib.ref_ckpt_header_layout(out)
+    names = ["sizeof", "magic", "version", "mpi_rank", "world_size", "ntracer", "ntracer_global", "nsnap", "nfreq",
+             "np_bins", "npe_bins", "nsnp_i_value", "nsnp_f_value", "completed_snapshot", "tracer_start",
+             "sizeof_magic"]
+    with open(os.path.join(HERE, "ckpt_header_layout.json"), "w") as f:
+        json.dump({k: int(out[i]) for i, k in enumerate(names[:cnt])}, f, indent=1)
+        f.write("\n")
+
+
 def main():
     if not os.path.isdir("/root/reference"):
         print("reference sources absent; nothing to do")
         return 0
     build_ref()
+    ckpt_layout()
+    if "--layout-only" in sys.argv:
+        return 0
     logx, logfx, fx, xmin = fx_table()
     np.savez_compressed(os.path.join(HERE, "fx_table.npz"), logx=logx, logfx=logfx, fx=fx,
                         xmin=np.array([xmin]))
