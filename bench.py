@@ -166,9 +166,18 @@ def emission_bench(ch, ntracer=1_000_000, n=256, nbins=64, seed=11):
                                      "tflops_issued": passes * flops / (t * 1e-3) / 1e12,
                                      "peak_tflops": peak, "peak_unit": unit,
                                      "compute_frac": passes * flops / (t * 1e-3) / 1e12 / peak}
-        nlogb, logb_min, inv_dlogb = 512, -10.0, 512 / 6.0
-        table = np.exp(-0.5 * ((lp[None, None, :] - centres[None, :, None]) / 0.8) ** 2)
-        table = np.ascontiguousarray(np.broadcast_to(table, (nlogb, nbins, n))).ravel()
+        # synchrotron: the pitch-kernel table (512 log-B planes x 64 frequencies
+        # x n bins, 32 pitch angles) is built on the device from the F(x)
+        # table, then each tracer interpolates it at its own B
+        nlogb, logb_min, inv_dlogb, ntheta = 512, -3.0, 512 / 6.0, 32
+        fx = np.load(os.path.join(REPO, "tests", "golden", "fx_table.npz"))
+        theta = (np.arange(ntheta) + 0.5) * np.pi / ntheta
+        pw = np.sin(theta) ** 2 * (np.pi / ntheta)
+        nus = 10.0 ** np.linspace(7.0, 11.0, nbins)
+        logy = np.log10(nus[:, None, None] / (1.0e-6 * (1.0 + pe ** 2)[None, :, None] * 4.194284e6 *
+                                              np.sin(theta)[None, None, :]))
+        table, ms_tab = ctx.synch_kernel_table(nbins, ntheta, nlogb, logb_min, inv_dlogb, fx["logx"], fx["logfx"],
+                                               fx["fx"], float(fx["xmin"][0]), logy.ravel(), pw)
         bfield = 10.0 ** rng.uniform(-8.0, -5.0, ntracer)
         ms = []
         for _ in range(2):
@@ -176,7 +185,8 @@ def emission_bench(ch, ntracer=1_000_000, n=256, nbins=64, seed=11):
             ms.append(t)
         t = ms[-1]
         res["synch"] = {"fp64": {"ms": t, "hbm_gbs": (byts + 8.0 * ntracer) / (t * 1e-3) / 1e9,
-                                 "hbm_frac": (byts + 8.0 * ntracer) / (t * 1e-3) / 1e9 / HBM_PEAK_GBS}}
+                                 "hbm_frac": (byts + 8.0 * ntracer) / (t * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                 "table_build_ms": ms_tab}}
         out["kernels"] = res
         out["bytes_per_call"] = byts
         out["flops_per_call"] = flops

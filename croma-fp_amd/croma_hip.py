@@ -241,6 +241,17 @@ class Context:
                                                int(mode), C.byref(ms)), "emission_hadronic")
         return eps.reshape(nt, nbins), ms.value
 
+    def synch_kernel_table(self, nfreq, ntheta, nlogb, logb_min, inv_dlogb, logx, logfx, fx, xmin, logy, pw):
+        """Pitch-kernel table [nlogb * nfreq * n] built on the device; returns (table, device ms)."""
+        table = np.zeros(nlogb * nfreq * self.n)
+        ms = C.c_double()
+        _chk(lib().croma_hip_synch_kernel_table(C.c_void_p(self.h), int(nfreq), int(ntheta), int(nlogb),
+                                                C.c_double(logb_min), C.c_double(inv_dlogb), len(logx),
+                                                C.c_double(xmin), _p(f64(logx)), _p(f64(logfx)), _p(f64(fx)),
+                                                _p(f64(logy)), _p(f64(pw)), _p(table), C.byref(ms)),
+             "synch_kernel_table")
+        return table, ms.value
+
     def emission_synch(self, cre, b_field, table, nfreq, nlogb, logb_min, inv_dlogb):
         nt = len(b_field)
         eps = np.zeros(nt * nfreq)

@@ -428,6 +428,63 @@ extern "C" int ck_emission_contract(int ntr, int K, int nbins, int nbins_pad, co
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+// Synchrotron pitch-kernel table (prepare_synch_pitch_kernel_table,
+// fp_shared_core.c:2008-2052): table[ib][nf][je] = sum_k F(x_k) w_k with
+// log x_k = logy[nf][je][k] - logb_ib and F interpolated log-linearly in the
+// log F(x) table (0 below its first node, its last value past the end).  One
+// thread per (ib, nf, je); the k sum runs in the reference's order.
+__global__ __launch_bounds__(256) void k_synch_table(int n, int nfreq, int ntheta, int nlogb, double logb_min,
+                                                     double inv_dlogb, int nx_tab, double xmin,
+                                                     const double *__restrict__ logx_tab,
+                                                     const double *__restrict__ logfx_tab,
+                                                     const double *__restrict__ fx_tab,
+                                                     const double *__restrict__ logy,
+                                                     const double *__restrict__ pitch_weight,
+                                                     double *__restrict__ table)
+{
+#pragma clang fp contract(off)
+    const size_t o = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (o >= (size_t)nlogb * nfreq * n) return;
+    const int je = (int)(o % n);
+    const int nf = (int)((o / n) % nfreq);
+    const int ib = (int)(o / ((size_t)n * nfreq));
+    const double logb = logb_min + (double)ib / inv_dlogb;
+    const double inv_dlogx = (nx_tab > 1) ? 1.0 / (logx_tab[1] - logx_tab[0]) : 0.0;
+    const double xlast = logx_tab[nx_tab - 1];
+    const double *ly = logy + ((size_t)nf * n + je) * ntheta;
+    double int_th = 0.0;
+    for (int k = 0; k < ntheta; ++k) {
+        double Fx = 0.0;
+        const double logx = ly[k] - logb;
+        if (logx <= xlast) {
+            const int ix = (int)floor((logx - xmin) * inv_dlogx);
+            if (ix >= 0) {
+                if (ix >= nx_tab - 1) {
+                    Fx = fx_tab[nx_tab - 1];
+                } else {
+                    const double frac = (logx - logx_tab[ix]) * inv_dlogx;
+                    Fx = pow(10.0, logfx_tab[ix] + frac * (logfx_tab[ix + 1] - logfx_tab[ix]));
+                }
+            }
+        }
+        int_th += Fx * pitch_weight[k];
+    }
+    table[o] = int_th;
+}
+
+extern "C" int ck_synch_table(int n, int nfreq, int ntheta, int nlogb, double logb_min, double inv_dlogb,
+                              int nx_tab, double xmin, const double *logx_tab, const double *logfx_tab,
+                              const double *fx_tab, const double *logy, const double *pitch_weight, double *table,
+                              void *stream)
+{
+    if (n <= 0 || nfreq <= 0 || ntheta <= 0 || nlogb <= 0 || nx_tab < 1) return -1;
+    const size_t tot = (size_t)nlogb * nfreq * n;
+    hipLaunchKernelGGL(k_synch_table, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, (hipStream_t)stream, n,
+                       nfreq, ntheta, nlogb, logb_min, inv_dlogb, nx_tab, xmin, logx_tab, logfx_tab, fx_tab, logy,
+                       pitch_weight, table);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 // Logarithms of the synchrotron kernel table, transposed to [ib][je][nf]
 // (frequency fastest, so the lanes of a wave read one coalesced row); NaN
 // marks entries that are not positive (linear interpolation there).
@@ -448,17 +505,76 @@ __global__ __launch_bounds__(256) void k_synch_logt(int nlogb, int nfreq, int n,
 // does (log-linear when both entries are positive, else linear); the sum
 // runs over je = 2.. up to the first bin whose successor is not positive.
 // With logt (FP64), the logarithms come from the table of k_synch_logt.
+// Table plane ib of a tracer's field (the same arithmetic as the
+// interpolation weight below).
+__device__ __forceinline__ int em_synch_plane(double bval, double logb_min, double inv_dlogb, int nlogb)
+{
+    if (!(bval > 0.0) || !isfinite(bval)) bval = 1.0e-30;
+    const double u = (log10(bval / 1.0e-6) - logb_min) * inv_dlogb;
+    int ib = (int)floor(u);
+    if (ib < 0) ib = 0;
+    if (ib > nlogb - 2) ib = nlogb - 2;
+    return ib;
+}
+
+// Tracers grouped by table plane (counting sort): waves that run one after
+// the other read the same two planes, which then stay in L2.
+__global__ __launch_bounds__(256) void k_synch_plane_count(int ntr, const double *__restrict__ b_field,
+                                                           double logb_min, double inv_dlogb, int nlogb,
+                                                           int *__restrict__ key, int *__restrict__ hist)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= ntr) return;
+    const int ib = em_synch_plane(b_field[i], logb_min, inv_dlogb, nlogb);
+    key[i] = ib;
+    atomicAdd(&hist[ib], 1);
+}
+
+__global__ __launch_bounds__(256) void k_synch_plane_scan(int nh, int *__restrict__ hist)
+{
+    __shared__ int s_part[256];
+    __shared__ int s_carry;
+    const int t = threadIdx.x;
+    if (t == 0) s_carry = 0;
+    __syncthreads();
+    for (int base = 0; base < nh; base += 256) {
+        const int x = base + t < nh ? hist[base + t] : 0;
+        s_part[t] = x;
+        __syncthreads();
+        for (int o = 1; o < 256; o <<= 1) {
+            const int y = t >= o ? s_part[t - o] : 0;
+            __syncthreads();
+            s_part[t] += y;
+            __syncthreads();
+        }
+        const int carry = s_carry;
+        if (base + t < nh) hist[base + t] = carry + s_part[t] - x;
+        __syncthreads();
+        if (t == 0) s_carry = carry + s_part[255];
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_synch_plane_place(int ntr, const int *__restrict__ key,
+                                                           int *__restrict__ hist, int *__restrict__ perm)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= ntr) return;
+    perm[atomicAdd(&hist[key[i]], 1)] = i;
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_emis_synch(int ntr, int n, int nfreq, int nlogb, double logb_min,
                                                     double inv_dlogb, const double *__restrict__ table,
-                                                    const double *__restrict__ logt,
+                                                    const double *__restrict__ logt, const int *__restrict__ perm,
                                                     const double *__restrict__ cre, const double *__restrict__ dp,
                                                     const double *__restrict__ b_field, double *__restrict__ eps)
 {
     __shared__ T s_w[4][512];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int it = blockIdx.x * 4 + w;
-    if (it >= ntr) return;
+    const int slot = blockIdx.x * 4 + w;
+    if (slot >= ntr) return;
+    const int it = perm ? perm[slot] : slot;
     const double *cl = cre + (size_t)it * n;
     // end of the summation: first je >= 2 with cl[je + 1] <= 0 (or n - 1)
     int jend = n - 1;
@@ -496,16 +612,28 @@ __global__ __launch_bounds__(256) void k_emis_synch(int ntr, int n, int nfreq, i
         if (inner) {
             const double *l0p = logt + (size_t)ib * n * nfreq + nf;
             const double *l1p = l0p + (size_t)n * nfreq;
-            for (int je = 2; je < jend; ++je) {
-                const double l0 = l0p[(size_t)je * nfreq], l1 = l1p[(size_t)je * nfreq];
-                T K;
-                if (!__builtin_isnan(l0) && !__builtin_isnan(l1)) {
-                    K = exp(((T)1 - t) * (T)l0 + t * (T)l1);
-                } else {
-                    const T k0 = (T)r0[je], k1 = (T)r1[je];
-                    K = ((T)1 - t) * k0 + t * k1;
+            // loads of 8 bins are issued before their exps (the sum keeps its order)
+            for (int j0 = 2; j0 < jend; j0 += 8) {
+                double l0[8], l1[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int je = j0 + u < jend ? j0 + u : j0;
+                    l0[u] = l0p[(size_t)je * nfreq];
+                    l1[u] = l1p[(size_t)je * nfreq];
                 }
-                integral += s_w[w][je] * K;
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int je = j0 + u;
+                    if (je >= jend) break;
+                    T K;
+                    if (!__builtin_isnan(l0[u]) && !__builtin_isnan(l1[u])) {
+                        K = exp(((T)1 - t) * (T)l0[u] + t * (T)l1[u]);
+                    } else {
+                        const T k0 = (T)r0[je], k1 = (T)r1[je];
+                        K = ((T)1 - t) * k0 + t * k1;
+                    }
+                    integral += s_w[w][je] * K;
+                }
             }
         } else {
             for (int je = 2; je < jend; ++je) {
@@ -523,21 +651,32 @@ __global__ __launch_bounds__(256) void k_emis_synch(int ntr, int n, int nfreq, i
 }
 
 extern "C" int ck_emission_synch(int ntr, int n, int nfreq, int nlogb, double logb_min, double inv_dlogb,
-                                 const double *table, double *logt, const double *cre, const double *dp,
+                                 const double *table, double *logt, int *work, const double *cre, const double *dp,
                                  const double *b_field, double *eps, int fp64, void *stream)
 {
     if (ntr <= 0 || n <= 3 || n > 512 || nfreq <= 0 || nlogb < 2) return -1;
-    const dim3 grid((ntr + 3) / 4);
+    hipStream_t st = (hipStream_t)stream;
+    const dim3 grid((ntr + 3) / 4), g1((ntr + 255) / 256);
+    int *perm = nullptr;
+    if (work) {
+        int *key = work, *hist = work + ntr;
+        perm = work + ntr + nlogb;
+        if (ck_fill_i32(hist, 0, (size_t)nlogb, stream)) return -1;
+        hipLaunchKernelGGL(k_synch_plane_count, g1, dim3(256), 0, st, ntr, b_field, logb_min, inv_dlogb, nlogb, key,
+                           hist);
+        hipLaunchKernelGGL(k_synch_plane_scan, dim3(1), dim3(256), 0, st, nlogb, hist);
+        hipLaunchKernelGGL(k_synch_plane_place, g1, dim3(256), 0, st, ntr, key, hist, perm);
+    }
     if (fp64 && logt) {
         const size_t tot = (size_t)nlogb * nfreq * n;
-        hipLaunchKernelGGL(k_synch_logt, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
-                           nlogb, nfreq, n, table, logt);
+        hipLaunchKernelGGL(k_synch_logt, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, nlogb, nfreq, n,
+                           table, logt);
     }
     if (fp64)
-        hipLaunchKernelGGL(k_emis_synch<double>, grid, dim3(256), 0, (hipStream_t)stream, ntr, n, nfreq, nlogb,
-                           logb_min, inv_dlogb, table, (const double *)logt, cre, dp, b_field, eps);
+        hipLaunchKernelGGL(k_emis_synch<double>, grid, dim3(256), 0, st, ntr, n, nfreq, nlogb, logb_min, inv_dlogb,
+                           table, (const double *)logt, (const int *)perm, cre, dp, b_field, eps);
     else
-        hipLaunchKernelGGL(k_emis_synch<float>, grid, dim3(256), 0, (hipStream_t)stream, ntr, n, nfreq, nlogb,
-                           logb_min, inv_dlogb, table, (const double *)nullptr, cre, dp, b_field, eps);
+        hipLaunchKernelGGL(k_emis_synch<float>, grid, dim3(256), 0, st, ntr, n, nfreq, nlogb, logb_min, inv_dlogb,
+                           table, (const double *)nullptr, (const int *)perm, cre, dp, b_field, eps);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }

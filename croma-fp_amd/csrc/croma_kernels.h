@@ -225,11 +225,17 @@ int ck_solve_cc_batch(int nsys, int nrow, double dt, const double *a, const doub
 int ck_emission_contract(int ntr, int K, int nbins, int nbins_pad, const double *spectra, const uint16_t *kt_hi,
                          const uint16_t *kt_lo, const float *kt32, const double *kt64, const int *col_shift,
                          const double *row_scale, double *out, int mode, void *stream);
+/* Synchrotron pitch-kernel table [nlogb][nfreq][n] from logy [nfreq][n][ntheta]. */
+int ck_synch_table(int n, int nfreq, int ntheta, int nlogb, double logb_min, double inv_dlogb, int nx_tab,
+                   double xmin, const double *logx_tab, const double *logfx_tab, const double *fx_tab,
+                   const double *logy, const double *pitch_weight, double *table, void *stream);
 /* Synchrotron: per-tracer log-B interpolated table contraction (VALU, FP64 or FP32).
  * logt (FP64 only; NULL computes logarithms per element): workspace of
- * nlogb * nfreq * n doubles, filled here with the table's logarithms. */
+ * nlogb * nfreq * n doubles, filled here with the table's logarithms.
+ * work (NULL: tracers in input order): 2 * ntr + nlogb ints, used to run the
+ * tracers grouped by table plane. */
 int ck_emission_synch(int ntr, int n, int nfreq, int nlogb, double logb_min, double inv_dlogb,
-                      const double *table, double *logt, const double *cre, const double *dp,
+                      const double *table, double *logt, int *work, const double *cre, const double *dp,
                       const double *b_field, double *eps, int fp64, void *stream);
 
 #ifdef __cplusplus

@@ -180,6 +180,57 @@ int croma_hip_emission_hadronic(const CromaHipContext *ctx, int ntracer, int nbi
     return rc;
 }
 
+int croma_hip_synch_kernel_table(const CromaHipContext *ctx, int nfreq, int ntheta, int nlogb, double logb_min,
+                                 double inv_dlogb, int nx_tab, double xmin, const double *logx_tab,
+                                 const double *logfx_tab, const double *fx_tab, const double *logy,
+                                 const double *pitch_weight, double *table, double *elapsed_ms)
+{
+    const ChGrid *g;
+    size_t ny, nt, bytes;
+    double *d = 0;
+    void *st = 0, *e0 = 0, *e1 = 0;
+    int n, rc = 0;
+    if (!ctx || nfreq <= 0 || ntheta <= 0 || nlogb <= 0 || nx_tab < 1 || !logx_tab || !logfx_tab || !fx_tab ||
+        !logy || !pitch_weight || !table || !(inv_dlogb > 0.0))
+        return CROMA_HIP_EINVAL;
+    g = ch_ctx_grid(ctx);
+    n = g->n;
+    ny = (size_t)nfreq * n * ntheta;
+    nt = (size_t)nlogb * nfreq * n;
+    bytes = (3 * (size_t)nx_tab + ny + (size_t)ntheta + nt) * sizeof(double);
+    d = (double *)ck_malloc(bytes);
+    st = ck_stream_create();
+    e0 = ck_event_create();
+    e1 = ck_event_create();
+    if (!d || !st || !e0 || !e1) {
+        rc = CROMA_HIP_EDEVICE;
+        ch_set_error("synch table: device allocation: %s", ck_last_error());
+        goto out;
+    }
+    {
+        double *d_lx = d, *d_lfx = d + nx_tab, *d_fx = d + 2 * (size_t)nx_tab, *d_ly = d + 3 * (size_t)nx_tab;
+        double *d_pw = d_ly + ny, *d_tab = d_pw + ntheta;
+        if (ck_h2d(d_lx, logx_tab, (size_t)nx_tab * 8, st) || ck_h2d(d_lfx, logfx_tab, (size_t)nx_tab * 8, st) ||
+            ck_h2d(d_fx, fx_tab, (size_t)nx_tab * 8, st) || ck_h2d(d_ly, logy, ny * 8, st) ||
+            ck_h2d(d_pw, pitch_weight, (size_t)ntheta * 8, st) || ck_event_record(e0, st) ||
+            ck_synch_table(n, nfreq, ntheta, nlogb, logb_min, inv_dlogb, nx_tab, xmin, d_lx, d_lfx, d_fx, d_ly,
+                           d_pw, d_tab, st) ||
+            ck_event_record(e1, st) || ck_d2h(table, d_tab, nt * 8, st) || ck_stream_sync(st)) {
+            rc = CROMA_HIP_EDEVICE;
+            ch_set_error("synch table: %s", ck_last_error());
+            goto out;
+        }
+        if (elapsed_ms) *elapsed_ms = ck_event_elapsed_ms(e0, e1);
+    }
+out:
+    if (st) ck_stream_sync(st);
+    ck_free(d);
+    ck_event_destroy(e0);
+    ck_event_destroy(e1);
+    if (st) ck_stream_destroy(st);
+    return rc;
+}
+
 int croma_hip_emission_synch(const CromaHipContext *ctx, int ntracer, int nfreq, int nlogb, double logb_min,
                              double inv_dlogb, const double *table, const double *cre, const double *b_field,
                              double *eps, double *elapsed_ms)
@@ -194,7 +245,7 @@ int croma_hip_emission_synch(const CromaHipContext *ctx, int ntracer, int nfreq,
     g = ch_ctx_grid(ctx);
     n = g->n;
     d = (char *)ck_malloc(2 * nt_tab * n * 8 + (size_t)ntracer * n * 8 + (size_t)n * 8 + (size_t)ntracer * 8 +
-                          (size_t)ntracer * nfreq * 8 + 6 * 256);
+                          (size_t)ntracer * nfreq * 8 + (3 * (size_t)ntracer + (size_t)nlogb) * 4 + 7 * 256);
     st = ck_stream_create();
     e0 = ck_event_create();
     e1 = ck_event_create();
@@ -205,6 +256,7 @@ int croma_hip_emission_synch(const CromaHipContext *ctx, int ntracer, int nfreq,
     }
     {
         double *d_tab, *d_logt, *d_cre, *d_dp, *d_b, *d_eps;
+        int *d_work;
         p = d;
         d_tab = (double *)p;
         p += (nt_tab * n * 8 + 255) & ~(size_t)255;
@@ -217,11 +269,13 @@ int croma_hip_emission_synch(const CromaHipContext *ctx, int ntracer, int nfreq,
         d_b = (double *)p;
         p += ((size_t)ntracer * 8 + 255) & ~(size_t)255;
         d_eps = (double *)p;
+        p += ((size_t)ntracer * nfreq * 8 + 255) & ~(size_t)255;
+        d_work = (int *)p;
         if (ck_h2d(d_tab, table, nt_tab * n * 8, st) || ck_h2d(d_cre, cre, (size_t)ntracer * n * 8, st) ||
             ck_h2d(d_dp, g->edp, (size_t)n * 8, st) || ck_h2d(d_b, b_field, (size_t)ntracer * 8, st) ||
             ck_event_record(e0, st) ||
-            ck_emission_synch(ntracer, n, nfreq, nlogb, logb_min, inv_dlogb, d_tab, d_logt, d_cre, d_dp, d_b, d_eps,
-                              1, st) ||
+            ck_emission_synch(ntracer, n, nfreq, nlogb, logb_min, inv_dlogb, d_tab, d_logt, d_work, d_cre, d_dp, d_b,
+                              d_eps, 1, st) ||
             ck_event_record(e1, st) || ck_d2h(eps, d_eps, (size_t)ntracer * nfreq * 8, st) || ck_stream_sync(st)) {
             rc = CROMA_HIP_EDEVICE;
             ch_set_error("synch emission: %s", ck_last_error());

@@ -251,6 +251,15 @@ int croma_hip_emission_ic(const CromaHipContext *ctx, int ntracer, int nbins, co
 int croma_hip_emission_hadronic(const CromaHipContext *ctx, int ntracer, int nbins, double emin,
                                 double emax, const double *n_gas, const double *crp,
                                 const double *f_flat, double *eps, int mode, double *elapsed_ms);
+/* Synchrotron pitch-kernel table [nlogb][nfreq][n], built on the device
+ * (prepare_synch_pitch_kernel_table, fp_shared_core.c:2008-2052): the inputs
+ * are the reference's SynchData tables (tracer_fp_synch.c:87-121): the log
+ * F(x) table logx_tab / logfx_tab / fx_tab of nx_tab nodes from xmin, logy
+ * [nfreq][n][ntheta] and the pitch weights [ntheta]. */
+int croma_hip_synch_kernel_table(const CromaHipContext *ctx, int nfreq, int ntheta, int nlogb, double logb_min,
+                                 double inv_dlogb, int nx_tab, double xmin, const double *logx_tab,
+                                 const double *logfx_tab, const double *fx_tab, const double *logy,
+                                 const double *pitch_weight, double *table, double *elapsed_ms);
 /* Synchrotron from a pitch-kernel table [nlogb][nfreq][n] with log
  * interpolation in log B (fp_shared_core.c:2070-2129). */
 int croma_hip_emission_synch(const CromaHipContext *ctx, int ntracer, int nfreq, int nlogb,

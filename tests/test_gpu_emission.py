@@ -123,3 +123,39 @@ def test_ic_grid_sizes(n, mode):
     err = rel(eps, ref)
     print(f"[emission] ic n={n} mode={mode} max rel err {err:.3e} ({ms:.3f} ms)", flush=True)
     assert err < TOL[mode]
+
+
+@pytest.mark.parametrize("n", [64, 128, 256, 512])
+def test_synch_table_on_device_matches_reference(n):
+    """Device pitch-kernel table vs the compiled reference's table (fixture)
+    and, at a larger shape, vs the oracle: 1e-13 relative (pow of the device
+    libm against the host's, summed over the pitch angles)."""
+    d = np.load(os.path.join(GOLD, f"ref_n{n}.npz"))
+    fx = np.load(os.path.join(GOLD, "fx_table.npz"))
+    nfreq, nth, nlogb = (int(v) for v in d["sy_dims"])
+    inv = float(d["sy_inv_dlogb"][0])
+    pmin, pmax, pemin, pemax = (float(v) for v in d["grid_bounds"])
+    ctx = ch.Context(n, ch.default_params(), pmin=pmin, pmax=pmax, pemin=pemin, pemax=pemax)
+    try:
+        tab, ms = ctx.synch_kernel_table(nfreq, nth, nlogb, -6.0, inv, fx["logx"], fx["logfx"], fx["fx"],
+                                         float(fx["xmin"][0]), d["sy_logy"], d["sy_pw"])
+        ref = d["sy_table"]
+        m = ref != 0
+        assert np.array_equal(tab[~m], ref[~m])
+        err = float(np.max(np.abs(tab[m] - ref[m]) / np.abs(ref[m]))) if m.any() else 0.0
+        print(f"[emission] synch table n={n} vs reference: max rel err {err:.3e}", flush=True)
+        assert err < 1e-13
+        # a larger table: 32 log-B planes x 64 frequencies x 24 pitch angles
+        o = po.Oracle(n, pmin, pmax, pemin, pemax)
+        logy, pw = po.synch_logy(o, 64, 7.0, 11.0, 24)
+        big_ref = po.synch_kernel_table(n, 64, 24, 32, -2.0, 8.0, fx["logx"], fx["logfx"], fx["fx"],
+                                        float(fx["xmin"][0]), logy, pw)
+        big, ms = ctx.synch_kernel_table(64, 24, 32, -2.0, 8.0, fx["logx"], fx["logfx"], fx["fx"],
+                                         float(fx["xmin"][0]), logy, pw)
+    finally:
+        ctx.close()
+    m = big_ref != 0
+    assert np.array_equal(big[~m], big_ref[~m])
+    err = float(np.max(np.abs(big[m] - big_ref[m]) / np.abs(big_ref[m])))
+    print(f"  32x64x{n} table: max rel err {err:.3e} ({ms:.3f} ms)", flush=True)
+    assert err < 1e-13
