@@ -56,16 +56,17 @@ __device__ double d_b_coulomb_p(double nicm, double p, double kBT)
     return K_GYR * K_C / (K_MP * K_GEV) * sum * nicm * (1.0 / betap2) * thermal * 3.5e-29;
 }
 
-// The same rate from per-bin factors: Ep = gamma m_p - m_p, two_mk =
-// 2 m_p (gamma - 1), ib2 = 1 / beta^2 (ck_grid_loss_tables evaluates them with
-// the expressions above, so the result is bit-identical).
-__device__ __forceinline__ double d_b_coulomb_p_tab(double nicm, double kBT, double Ep, double two_mk, double ib2)
+// The same rate from per-bin factors of the segment build (Ep = gamma m_p -
+// m_p, i2mk = 1 / (2 m_p (gamma - 1)), ib2 = 1 / beta^2) with the divisions
+// by kBT done as one reciprocal: within a few ulp of d_b_coulomb_p.
+__device__ __forceinline__ double d_b_coulomb_p_tab(double nicm, double kBT, double ikbt, double Ep, double i2mk,
+                                                    double ib2)
 {
 #pragma clang fp contract(off)
-    const double xp = Ep / kBT;
+    const double xp = Ep * ikbt;
     const double xe = xp * (K_ME / K_MP);
     const double sum = d_coulomb_kernel(xe) + d_coulomb_kernel(xp) * (K_ME / K_MP);
-    const double thermal = 1.0 - kBT / two_mk;
+    const double thermal = 1.0 - kBT * i2mk;
     if (thermal < 0.0) return 0.0;
     return K_GYR * K_C / (K_MP * K_GEV) * sum * nicm * ib2 * thermal * 3.5e-29;
 }
@@ -714,7 +715,7 @@ __device__ __forceinline__ Face d_face2(double Bl, double Br, double Cl, double 
         const double E = h2 * h2;
         const bool small = aw < 0.1;
         const double w2 = w * w;
-        const double den = small ? h2 * (1.0 + w2 / 24.0 + (w2 * w2) / 1920.0) : 1.0 - E;
+        const double den = small ? h2 * (1.0 + w2 * (1.0 / 24.0) + (w2 * w2) * (1.0 / 1920.0)) : 1.0 - E;
         const double big = (small ? 1.0 : aw) / den;
         const double sm = big * E;
         f.wp = (w > 0.0) ? big : sm;
@@ -785,7 +786,7 @@ __device__ __forceinline__ void row_BC(const CkGrid &g, const SpGrid &sg, const 
     double D;
     if (!rc.model) D = sg.p2[j] * rc.dfac;
     else if (ELEC) D = sg.p2[j] * rc.dfac * sg.expcut[j];
-    else D = sg.p2[j] * rc.dfac * exp(-g.pE[j] * rc.inv_epmax) * sg.expcut[j];
+    else D = sg.p2[j] * rc.dfac * g.p_expE[j] * sg.expcut[j];
     C = D;
     B = radp - sg.two_over_p[j] * D;
 }
@@ -854,14 +855,14 @@ __device__ __forceinline__ void seg_radp(const CkGrid &g, const CkSlot &S, int b
         const double synch_coeff = 4.8e-4 * b_ratio * b_ratio * 9.11e-28 * 3.0e+10 * 3.1536e+16;
         const double coul_coeff = 3.05e-29 * q.n_gas * K_C * K_GYR / (K_ME * K_GEV);
         const double adia = 1.0 / 3.0 * q.divv;
+        const double ing = 1.0 / q.n_gas;
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             const int j = base + k;
             const double pe = g.ep[j];
             const double pe2 = g.e_p2[j];
-            const double beta2 = g.e_b2[j];
             const double ric = (nseg > 1) ? d_lerp(S.ric_c[j], S.ric_n[j], alpha) : S.ric_c[j];
-            const double coul = coul_coeff * (1.0 + log(pe / q.n_gas) / 74.8) / beta2;
+            const double coul = coul_coeff * (1.0 + log(pe * ing) * (1.0 / 74.8)) * g.e_ib2[j];
             radp[k] = synch_coeff * pe2 + ric + coul + adia * pe;
         }
     } else {
@@ -888,10 +889,12 @@ __device__ __forceinline__ void seg_radp_p(const CkGrid &g, int gl, const SegSca
 {
 #pragma clang fp contract(off)
     const double adia = 1.0 / 3.0 * q.divv;
+    const double ikbt = 1.0 / q.kbt;
 #pragma unroll
     for (int m = 0; m < R; ++m) {
         const int j = m * G + gl;
-        s_tr[j + j / R] = d_b_coulomb_p_tab(q.n_gas, q.kbt, g.p_Ep[j], g.p_2mk[j], g.p_ib2[j]) + adia * g.pp[j];
+        s_tr[j + j / R] =
+            d_b_coulomb_p_tab(q.n_gas, q.kbt, ikbt, g.p_Ep[j], g.p_i2mk[j], g.p_ib2[j]) + adia * g.pp[j];
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();

@@ -48,10 +48,10 @@ typedef struct {
      * (0 for j = n-1), 1/h_j, 1/dx_j and 2/p_j */
     const double *p_h, *p_inv_h, *p_inv_dx, *p_2op;
     const double *e_h, *e_inv_h, *e_inv_dx, *e_2op;
-    /* per-bin factors of the loss rates, built on the device with the loss
-     * code's own arithmetic: proton E_kin = gamma m_p - m_p, 2 m_p (gamma - 1),
-     * 1 / beta^2; electron beta^2 */
-    const double *p_Ep, *p_2mk, *p_ib2, *e_b2;
+    /* per-bin factors of the loss rates and D_pp, built on the device:
+     * proton E_kin = gamma m_p - m_p, 1 / (2 m_p (gamma - 1)), 1 / beta^2,
+     * exp(-E / E_pmax) (rebuilt when the parameters change); electron 1 / beta^2 */
+    const double *p_Ep, *p_i2mk, *p_ib2, *p_expE, *e_ib2;
     const double *pow10; /* 10^k for k = -40..40 (index k + 40), as the host's pow gives them */
     double ppm1, ppp1, epm1, epp1;
 } CkGrid;
@@ -162,9 +162,9 @@ int ck_nsub_arrays(CkGrid g, CkParams prm, int ntr, double dt_snap, double ric_l
                    const double *n_gas, const double *kbt, const double *b_field, const double *divv,
                    const double *l_turb, const double *dv, int *nsub, int *n_on, void *stream);
 
-/* Fills the per-bin loss factors of g (p_Ep, p_2mk, p_ib2, e_b2; writable
- * device arrays of g.n). */
-int ck_grid_loss_tables(CkGrid g, void *stream);
+/* Fills the per-bin loss factors of g (p_Ep, p_i2mk, p_ib2, p_expE, e_ib2;
+ * writable device arrays of g.n) for the given E_pmax [GeV]. */
+int ck_grid_loss_tables(CkGrid g, double epmax, void *stream);
 
 /* Fill kernel (used instead of memset so it is safe under graph capture). */
 int ck_fill_i32(int *dst, int value, size_t count, void *stream);

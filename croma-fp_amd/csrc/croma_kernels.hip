@@ -350,9 +350,8 @@ __global__ void k_fill(T *dst, T v, size_t count)
         dst[i] = v;
 }
 
-// Per-bin loss factors, with the expressions of d_b_coulomb_p and of the
-// electron beta^2 in seg_radp (so the tabulated forms are bit-identical).
-__global__ void k_grid_loss_tables(CkGrid g)
+// Per-bin factors of the segment build (loss rates and the D_pp cutoff).
+__global__ void k_grid_loss_tables(CkGrid g, double inv_epmax)
 {
 #pragma clang fp contract(off)
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -363,16 +362,18 @@ __global__ void k_grid_loss_tables(CkGrid g)
     const double betap2 = p2 / (1.0 + p2);
     const double kinetic = gamma - 1.0;
     const_cast<double *>(g.p_Ep)[j] = gamma * K_MP - K_MP;
-    const_cast<double *>(g.p_2mk)[j] = 2.0 * K_MP * kinetic;
+    const_cast<double *>(g.p_i2mk)[j] = 1.0 / (2.0 * K_MP * kinetic);
     const_cast<double *>(g.p_ib2)[j] = 1.0 / betap2;
+    const_cast<double *>(g.p_expE)[j] = exp(-g.pE[j] * inv_epmax);
     const double pe2 = g.e_p2[j];
-    const_cast<double *>(g.e_b2)[j] = pe2 / (1.0 + pe2);
+    const_cast<double *>(g.e_ib2)[j] = 1.0 / (pe2 / (1.0 + pe2));
 }
 
-extern "C" int ck_grid_loss_tables(CkGrid g, void *stream)
+extern "C" int ck_grid_loss_tables(CkGrid g, double epmax, void *stream)
 {
-    if (g.n <= 0) return 0;
-    hipLaunchKernelGGL(k_grid_loss_tables, dim3((g.n + 255) / 256), dim3(256), 0, (hipStream_t)stream, g);
+    if (g.n <= 0 || !(epmax > 0.0)) return -1;
+    hipLaunchKernelGGL(k_grid_loss_tables, dim3((g.n + 255) / 256), dim3(256), 0, (hipStream_t)stream, g,
+                       1.0 / epmax);
     return hip_ok(hipGetLastError(), "k_grid_loss_tables");
 }
 

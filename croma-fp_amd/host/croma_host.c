@@ -127,6 +127,7 @@ static int valid_params(const CromaHipParams *p)
     if (!p) return 0;
     if (p->dpp_model < 1 || p->dpp_model > 4) return 0;
     if (!(p->tracer_nsub_safety > 0.0)) return 0;
+    if (!(p->epmax > 0.0) || !isfinite(p->epmax)) return 0;
     return 1;
 }
 
@@ -206,7 +207,7 @@ static void derived_tables(int n, const double *x, const double *dx, double *h, 
     }
 }
 
-#define CH_NTAB 25 /* 21 uploaded, 4 loss-factor tables built on the device */
+#define CH_NTAB 26 /* 21 uploaded, 5 segment-build factor tables built on the device */
 #define CH_NPOW10 81 /* 10^k, k = -40..40, for the nsub ladder on the device */
 
 static int ctx_upload_grid(CromaHipContext *c)
@@ -262,15 +263,17 @@ static int ctx_upload_grid(CromaHipContext *c)
     c->dgrid.e_inv_dx = d + 19 * (size_t)n;
     c->dgrid.e_2op = d + 20 * (size_t)n;
     c->dgrid.p_Ep = d + 21 * (size_t)n;
-    c->dgrid.p_2mk = d + 22 * (size_t)n;
+    c->dgrid.p_i2mk = d + 22 * (size_t)n;
     c->dgrid.p_ib2 = d + 23 * (size_t)n;
-    c->dgrid.e_b2 = d + 24 * (size_t)n;
+    c->dgrid.p_expE = d + 24 * (size_t)n;
+    c->dgrid.e_ib2 = d + 25 * (size_t)n;
     c->dgrid.pow10 = d + CH_NTAB * (size_t)n;
     c->dgrid.ppm1 = c->g.ppm1;
     c->dgrid.ppp1 = c->g.ppp1;
     c->dgrid.epm1 = c->g.epm1;
     c->dgrid.epp1 = c->g.epp1;
-    if (ck_grid_loss_tables(c->dgrid, c->stream) || ck_stream_sync(c->stream)) return dev_fail("grid loss tables");
+    if (ck_grid_loss_tables(c->dgrid, c->prm.epmax, c->stream) || ck_stream_sync(c->stream))
+        return dev_fail("grid loss tables");
     return 0;
 }
 
@@ -398,9 +401,15 @@ void croma_hip_context_destroy(CromaHipContext *c)
 
 int croma_hip_context_set_params(CromaHipContext *c, const CromaHipParams *prm)
 {
+    const double old_epmax = c ? c->prm.epmax : 0.0;
     if (!c || !valid_params(prm)) return CROMA_HIP_EINVAL;
     c->uprm = *prm;
     ch_params_to_ck(prm, &c->prm);
+    /* the D_pp cutoff table depends on E_pmax */
+    if (c->prm.epmax != old_epmax &&
+        (ck_stream_sync(c->stream) || ck_grid_loss_tables(c->dgrid, c->prm.epmax, c->stream) ||
+         ck_stream_sync(c->stream)))
+        return dev_fail("grid loss tables");
     return 0;
 }
 
