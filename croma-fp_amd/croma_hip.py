@@ -475,6 +475,59 @@ class Server:
 
 OUT_MAP, OUT_TILE = 0, 1
 
+DSA_STATE, DSA_SOURCE = 1, 2
+DSA_KR07, DSA_KR07R, DSA_KR13, DSA_KR13R, DSA_CS14, DSA_CS14R, DSA_RYU19, DSA_RYU19R = range(8)
+
+
+class DsaParams(C.Structure):
+    _fields_ = [("eta_model", C.c_int), ("inject_species", C.c_int), ("chi_p", C.c_double), ("chi_e", C.c_double),
+                ("k_ep", C.c_double), ("pmax_p", C.c_double), ("pmax_e", C.c_double), ("gamma_gas", C.c_double),
+                ("min_mach", C.c_double), ("mu_mol", C.c_double), ("pmin_dsa_p_log10", C.c_double),
+                ("pmin_dsa_e_log10", C.c_double)]
+
+
+def dsa_default_params(**kw):
+    p = DsaParams()
+    lib().croma_hip_dsa_default_params(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def dsa_eta(mach, model):
+    L = lib()
+    L.croma_hip_dsa_eta.restype = C.c_double
+    L.croma_hip_dsa_eta.argtypes = [C.c_double, C.c_int]
+    return L.croma_hip_dsa_eta(float(mach), int(model))
+
+
+def dsa_shock_sources(n, params, mach, temp1, rho1, u1=0.0, pmin=-1.0, pmax=8.0, pemin=0.0, pemax=6.0):
+    """Source rows (4 pi p^2 f per unit volume) of one shock: (proton, electron, flags)."""
+    gs = GridSpec(n, pmin, pmax, pemin, pemax)
+    sp, se = np.zeros(n), np.zeros(n)
+    fl = (C.c_int * 2)()
+    _chk(lib().croma_hip_dsa_shock_sources(C.byref(gs), C.byref(params), C.c_double(mach), C.c_double(temp1),
+                                           C.c_double(rho1), C.c_double(u1), _p(sp), _p(se), fl),
+         "dsa_shock_sources")
+    return sp, se, (fl[0], fl[1])
+
+
+def dsa_inject(n, params, mode, mach, rho1, mass, temp1=None, u1=None, flux=None, side=None, dt_gyr=0.0,
+               crp=None, cre=None, qpi=None, qepri=None, pmin=-1.0, pmax=8.0, pemin=0.0, pemax=6.0):
+    """Adds one snapshot's DSA injection in place (state or source arrays); returns the injected count."""
+    gs = GridSpec(n, pmin, pmax, pemin, pemax)
+    nt = len(mach)
+    ninj = C.c_int()
+    opt = lambda a: _p(f64(a)) if a is not None else None
+    for a in (crp, cre, qpi, qepri):
+        assert a is None or (a.dtype == np.float64 and a.flags.c_contiguous and a.size == nt * n)
+    _chk(lib().croma_hip_dsa_inject(C.byref(gs), C.byref(params), int(mode), nt, C.c_double(dt_gyr), _p(f64(mach)),
+                                    opt(temp1), _p(f64(rho1)), opt(u1), opt(flux), opt(side), _p(f64(mass)),
+                                    _p(crp) if crp is not None else None, _p(cre) if cre is not None else None,
+                                    _p(qpi) if qpi is not None else None, _p(qepri) if qepri is not None else None,
+                                    C.byref(ninj)), "dsa_inject")
+    return ninj.value
+
 
 class CheckpointMeta(C.Structure):
     _fields_ = [("mpi_rank", C.c_int), ("world_size", C.c_int), ("ntracer", C.c_int), ("ntracer_global", C.c_int),

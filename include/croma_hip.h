@@ -321,6 +321,59 @@ int croma_hip_batch_checkpoint_write(CromaHipBatch *b, const char *dir, const Cr
 int croma_hip_batch_checkpoint_read(CromaHipBatch *b, const char *dir, const CromaHipCheckpointMeta *meta,
                                     const int *ids, int *completed_snapshot);
 
+/* ---- DSA source builder (tracer_fp_apply_dsa_tracer_injection_snapshot,
+ * tracer_fp_dsa.c:184-505; DSA_MODELS.c:394-1558) -----------------------
+ * Shock-accelerated CR injection per tracer, without reacceleration: the
+ * proton spectrum A_p (p/p_inj)^-q exp(-(p/p_max)^2) normalised to
+ * eta(M) of the upstream kinetic energy flux, the electrons at K_ep of the
+ * protons at p_inj,p (suprathermal injection), on the FP grid as 4 pi p^2 f. */
+#define CROMA_HIP_DSA_KR07 0   /* eta models, DSAModelID order (DSA_MODELS.h:37-47) */
+#define CROMA_HIP_DSA_KR07R 1
+#define CROMA_HIP_DSA_KR13 2
+#define CROMA_HIP_DSA_KR13R 3
+#define CROMA_HIP_DSA_CS14 4
+#define CROMA_HIP_DSA_CS14R 5
+#define CROMA_HIP_DSA_RYU19 6
+#define CROMA_HIP_DSA_RYU19R 7
+#define CROMA_HIP_DSA_SPECIES_BOTH 0
+#define CROMA_HIP_DSA_SPECIES_ELECTRON 1
+#define CROMA_HIP_DSA_SPECIES_PROTON 2
+#define CROMA_HIP_DSA_STATE 1   /* add the fresh spectrum to the state (TRACER_STATE) */
+#define CROMA_HIP_DSA_SOURCE 2  /* add spectrum / dt to qpi, qepri (TRACER_SOURCE) */
+typedef struct {
+    int eta_model;          /* DSAEtaModelInitial (params.c:97): RYU19 */
+    int inject_species;     /* DSAInjectSpecies: both */
+    double chi_p, chi_e;    /* DSAChiP, DSAChiE: 3.5, 3.5 (chi_e <= 0 uses chi_p) */
+    double k_ep;            /* DSAKep: 1e-2 */
+    double pmax_p, pmax_e;  /* DSAPmaxPmc, DSAPmaxEmc: 1e6, 1e5 (p / m c) */
+    double gamma_gas;       /* DSAGammaGas: 5/3 */
+    double min_mach;        /* DSAMinMach: 1.5 */
+    double mu_mol;          /* mu_mol: 0.59 */
+    double pmin_dsa_p_log10, pmin_dsa_e_log10; /* DSA grid start (tracer_fp.c:841): -3, -3 */
+} CromaHipDsaParams;
+void croma_hip_dsa_default_params(CromaHipDsaParams *p);
+/* eta(M) of a model (dsa_eta_from_id). */
+double croma_hip_dsa_eta(double mach, int eta_model);
+/* Source rows of one shock on a grid: source_p [n], source_e [n] (per unit
+ * volume, 4 pi p^2 f), flags[0] / [1] = proton / electron source built.
+ * u1 <= 0 derives it from mach and temp1. */
+int croma_hip_dsa_shock_sources(const CromaHipGridSpec *grid, const CromaHipDsaParams *p, double mach,
+                                double temp1_K, double rho1_cgs, double u1_cgs, double *source_p,
+                                double *source_e, int *flags);
+/* Per-tracer injection of one snapshot with shock data (the non-merged
+ * input branch: mach, pre-shock temperature [K], density [g cm^-3], upstream
+ * speed [cm/s] or NULL, side code or NULL (-1 and 99 skip)); or, with
+ * kinetic_flux non-NULL, the merged branch (u1 = (2 F / rho1)^(1/3), temp1
+ * from u1 and M).  tracer_mass [M_sun].  mode STATE adds V_down S to
+ * crp / cre [ntracer * n]; SOURCE adds V_down S / dt_gyr to qpi / qepri.
+ * Tracers with mach NaN or below min_mach are untouched.  *ninjected counts
+ * the tracers that received a source. */
+int croma_hip_dsa_inject(const CromaHipGridSpec *grid, const CromaHipDsaParams *p, int mode, int ntracer,
+                         double dt_gyr, const double *mach, const double *temp1_K, const double *rho1_cgs,
+                         const double *u1_cgs, const double *kinetic_flux, const double *side_code,
+                         const double *tracer_mass, double *crp, double *cre, double *qpi, double *qepri,
+                         int *ninjected);
+
 /* ---- serving: continuous batching of tracer chunks ------------------ */
 typedef struct CromaHipServer CromaHipServer;
 

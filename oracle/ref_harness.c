@@ -23,6 +23,7 @@
 #include "fp_shared_core.h"
 #include "tracer_fp_nsub.h"
 #include "tracer_fp_types.h"
+#include "DSA_MODELS.h"
 
 #include <stddef.h>
 
@@ -241,3 +242,58 @@ REF_API int ref_ckpt_header_layout(long *out)
     out[15] = (long)sizeof(((TracerFpCheckpointHeader *)0)->magic);
     return 16;
 }
+
+/* DSA source rows of one shock from the reference's own builders
+ * (dsa_init_grid with the pmin_dsa of tracer_fp.c:841, dsa_build_proton /
+ * electron_source_fp), with the shock state and injection model filled as
+ * tracer_fp_dsa.c:240-312 does (non-merged input branch, reacceleration off,
+ * suprathermal electron injection).  eta_model: DSAModelID.  flags[0] /
+ * flags[1] = fresh proton / electron source built; spec[0..4] = proton q, A_f,
+ * p_inj, p_max, electron A_f. */
+REF_API int ref_dsa_sources(int eta_model, double chi_p, double chi_e, double k_ep, double pmax_p, double pmax_e,
+                            double gamma_gas, double mach, double temp1, double rho1, double u1,
+                            double *source_p, double *source_e, int *flags, double *spec)
+{
+    static DSAGrid grid;
+    static int grid_ready = 0;
+    DSAInjectionModel model;
+    DSAShockState shock;
+    DSASpectrum sp, se;
+    double ratio;
+    if (!grid_ready) {
+        memset(&grid, 0, sizeof(grid));
+        if (!dsa_init_grid(&grid, &g_crp, &g_cre, -3.0, -3.0)) return -1;
+        grid_ready = 1;
+    }
+    memset(&model, 0, sizeof(model));
+    model.eta_model_initial = (DSAModelID)eta_model;
+    model.chi_p = chi_p;
+    model.chi_e = (chi_e > 0.0) ? chi_e : chi_p;
+    model.K_ep = k_ep;
+    model.p_dsamax_p_mc = pmax_p;
+    model.p_dsamax_e_mc = pmax_e;
+    model.electron_mode = DSA_ELECTRON_INJ_SUPRATHERMAL;
+    memset(source_p, 0, sizeof(double) * np);
+    memset(source_e, 0, sizeof(double) * npe);
+    memset(&sp, 0, sizeof(sp));
+    memset(&se, 0, sizeof(se));
+    flags[0] = flags[1] = 0;
+    ratio = dsa_compression_ratio(mach, gamma_gas);
+    memset(&shock, 0, sizeof(shock));
+    shock.mach = mach;
+    shock.gamma_gas = gamma_gas;
+    shock.rho1 = rho1;
+    shock.u1 = u1;
+    shock.u2 = u1 / ratio;
+    shock.temperature = temp1;
+    flags[0] = dsa_build_proton_source_fp(&shock, &model, &grid, &g_crp, source_p, &sp);
+    if (flags[0]) flags[1] = dsa_build_electron_source_fp(&shock, &model, &grid, &g_cre, &sp, source_e, &se);
+    spec[0] = sp.q;
+    spec[1] = sp.A_f;
+    spec[2] = sp.p_inj_mc;
+    spec[3] = sp.p_max_mc;
+    spec[4] = se.A_f;
+    return 0;
+}
+
+REF_API double ref_dsa_eta(double mach, int eta_model) { return dsa_eta_from_id(mach, (DSAModelID)eta_model); }

@@ -17,7 +17,9 @@ F(x) itself is checked only against its asymptotes (parity unpinned vs GSL).
 It also records the byte layout of the reference's checkpoint header
 (ckpt_header_layout.json) from the compiled struct.
 
-Usage:  python tests/golden/make_golden.py [--layout-only]
+and the DSA source rows of seeded shocks (dsa_n{64,256}.npz).
+
+Usage:  python tests/golden/make_golden.py [--layout-only | --dsa-only]
 """
 from __future__ import annotations
 
@@ -232,6 +234,42 @@ def ckpt_layout():
         f.write("\n")
 
 
+def dsa_fixture(n, rng):
+    """DSA source rows of seeded shocks from the reference's own builders
+    (DSA_MODELS.c through ref_dsa_sources) and eta(M) of every model."""
+    lib = ref_lib(n)
+    pp, pdp, pE = np.zeros(n), np.zeros(n), np.zeros(n)
+    ep, edp, eE = np.zeros(n), np.zeros(n), np.zeros(n)
+    pg, eg, bp = np.zeros(2), np.zeros(2), np.zeros(n)
+    d = C.c_double
+    lib.ref_grid(d(-1.0), d(8.0), d(0.0), d(6.0), P(pp), P(pdp), P(pE), P(pg), P(ep), P(edp), P(eE), P(eg), P(bp))
+    nsh = 48
+    mach = 10 ** rng.uniform(np.log10(1.2), np.log10(40.0), nsh)
+    temp = 10 ** rng.uniform(5.0, 8.5, nsh)
+    rho = 10 ** rng.uniform(-30.0, -25.0, nsh)
+    u1 = mach * np.sqrt(5.0 / 3.0 * 1.380649e-16 * temp / (0.59 * 1.67262192369e-24)) * rng.uniform(0.8, 1.2, nsh)
+    # (eta model, chi_p, chi_e, K_ep, pmax_p, pmax_e) per shock
+    models = rng.integers(0, 8, nsh)
+    chi = rng.choice([3.0, 3.5, 4.0], nsh)
+    chie = rng.choice([0.0, 3.5], nsh)
+    kep = rng.choice([1e-2, 1e-3], nsh)
+    pmax_e = rng.choice([1e5, 1e4], nsh)
+    sp, se = np.zeros((nsh, n)), np.zeros((nsh, n))
+    flags = np.zeros((nsh, 2), np.int32)
+    spec = np.zeros((nsh, 5))
+    for i in range(nsh):
+        fl = (C.c_int * 2)()
+        row_p, row_e, sv = np.zeros(n), np.zeros(n), np.zeros(5)
+        lib.ref_dsa_sources(int(models[i]), d(chi[i]), d(chie[i]), d(kep[i]), d(1e6), d(pmax_e[i]), d(5.0 / 3.0),
+                            d(mach[i]), d(temp[i]), d(rho[i]), d(u1[i]), P(row_p), P(row_e), fl, P(sv))
+        sp[i], se[i], flags[i], spec[i] = row_p, row_e, (fl[0], fl[1]), sv
+    lib.ref_dsa_eta.restype = C.c_double
+    mgrid = np.concatenate([np.linspace(1.0, 3.0, 81), np.linspace(3.0, 40.0, 149)])
+    eta = np.array([[lib.ref_dsa_eta(d(m), k) for m in mgrid] for k in range(8)])
+    return dict(mach=mach, temp=temp, rho=rho, u1=u1, model=models.astype(np.int32), chi=chi, chie=chie, kep=kep,
+                pmax_e=pmax_e, source_p=sp, source_e=se, flags=flags, spec=spec, eta_mach=mgrid, eta=eta)
+
+
 def main():
     if not os.path.isdir("/root/reference"):
         print("reference sources absent; nothing to do")
@@ -239,6 +277,11 @@ def main():
     build_ref()
     ckpt_layout()
     if "--layout-only" in sys.argv:
+        return 0
+    drng = np.random.default_rng(8080)
+    for n in (64, 256):
+        np.savez_compressed(os.path.join(HERE, f"dsa_n{n}.npz"), **dsa_fixture(n, drng))
+    if "--dsa-only" in sys.argv:
         return 0
     logx, logfx, fx, xmin = fx_table()
     np.savez_compressed(os.path.join(HERE, "fx_table.npz"), logx=logx, logfx=logfx, fx=fx,
