@@ -518,6 +518,26 @@ __device__ __forceinline__ void scan_coeffs(int gl, int rl, double &A, double &B
     }
 }
 
+// Per-row inputs of a build (loss rates, sources) and the parked spectrum are
+// read from memory once per run at an offset that passes through an empty asm:
+// the compiler can neither hoist those loads out of the run loop nor forward
+// the parked values, so none of them holds registers across the substeps or
+// the build.
+__device__ __forceinline__ int run_opaque(int v)
+{
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
+// Source rows of one run: src_k = scale * (inj_k * ka * kb + q_k), a null
+// pointer contributing nothing (q: the tracer's direct source rows, or rows
+// already scaled in LDS; inj: the contracted secondary rows of the segment).
+struct SrcRows {
+    const double *q;
+    const double *inj;
+    double ka, kb, scale;
+};
+
 // Thomas LU of the rows held in F (on entry al = a, iv = b, g = c), as in
 // fp_thomas_solve_ws (fp_shared_core.c:226-257): denom_i = b_i + a_i cprime_{i-1},
 // cprime_i = -c_i / denom_i, pivot guard |denom| <= 1e-300.  The pivot
@@ -525,7 +545,7 @@ __device__ __forceinline__ void scan_coeffs(int gl, int rl, double &A, double &B
 // group's lanes (renormalised by powers of two, so exact up to rounding of
 // the products); inside the lane the reference recurrence is applied.
 template <int R, int G>
-__device__ __forceinline__ void factorize(int gl, int n, const double (&src)[R], Factor<R, G> &F, int &bad)
+__device__ __forceinline__ void factorize(int gl, int n, const SrcRows &src, int o, Factor<R, G> &F, int &bad)
 {
     double m00 = 1.0, m01 = 0.0, m10 = 0.0, m11 = 1.0;
 #pragma unroll
@@ -568,7 +588,9 @@ __device__ __forceinline__ void factorize(int gl, int n, const double (&src)[R],
         const double cp = (base + k == n - 1) ? 0.0 : -c * inv;
         F.iv[k] = inv;
         F.al[k] = a * inv;
-        F.vs[k] = inv * src[k];
+        const double qv = src.q ? src.q[o + k] : 0.0;
+        const double sv = src.inj ? src.inj[o + k] * src.ka * src.kb + qv : qv;
+        F.vs[k] = inv * (src.scale * sv);
         F.g[k] = -cp;
         A *= F.al[k];
         Bq *= F.g[k];
@@ -669,21 +691,50 @@ __device__ __forceinline__ void apply_solve(int gl, const Factor<R, G> &F, doubl
 // Coefficient build of one species for one run
 // ---------------------------------------------------------------------------
 
-// Per-species device grid view.
+// Per-species view of the per-row grid tables of the build and the loss
+// rates: the device tables (row j at j), or their copies in LDS (row j at
+// j + (j >> ps), the padding of the rate rows).  ps is a literal at every
+// construction, so the index arithmetic folds away.
 struct SpGrid {
-    const double *x, *h, *inv_h, *inv_dx, *two_over_p, *p2, *expcut;
+    const double *x, *h, *inv_h, *inv_dx, *two_over_p, *p2, *expcut, *ib2;
+    const double *expE, *sig, *Ep, *i2mk;  // protons only
+    int ps;                                // < 0: device tables
+    __device__ __forceinline__ int ix(int j) const { return ps < 0 ? j : j + (j >> ps); }
 };
 
-__device__ __forceinline__ SpGrid sp_grid(const CkGrid &g, bool elec)
+// Tables of one species, in the order of their LDS copies.
+enum { GT_X, GT_H, GT_INVH, GT_INVDX, GT_2OP, GT_P2, GT_EXPCUT, GT_IB2, GT_NE,
+       GT_EXPE = GT_NE, GT_SIG, GT_EP, GT_I2MK, GT_NP };
+
+__device__ __forceinline__ const double *sp_table(const CkGrid &g, bool elec, int t)
+{
+    switch (t) {
+    case GT_X: return elec ? g.ep : g.pp;
+    case GT_H: return elec ? g.e_h : g.p_h;
+    case GT_INVH: return elec ? g.e_inv_h : g.p_inv_h;
+    case GT_INVDX: return elec ? g.e_inv_dx : g.p_inv_dx;
+    case GT_2OP: return elec ? g.e_2op : g.p_2op;
+    case GT_P2: return elec ? g.e_p2 : g.p_p2;
+    case GT_EXPCUT: return elec ? g.e_expcut : g.p_expcut;
+    case GT_IB2: return elec ? g.e_ib2 : g.p_ib2;
+    case GT_EXPE: return g.p_expE;
+    case GT_SIG: return g.p_sig;
+    case GT_EP: return g.p_Ep;
+    default: return g.p_i2mk;
+    }
+}
+
+// tab: the device tables (ps < 0) or the species' LDS copies, np doubles apart
+__device__ __forceinline__ SpGrid sp_grid(const CkGrid &g, bool elec, const double *tab, int np, int ps)
 {
     SpGrid s;
-    if (elec) {
-        s.x = g.ep; s.h = g.e_h; s.inv_h = g.e_inv_h; s.inv_dx = g.e_inv_dx; s.two_over_p = g.e_2op;
-        s.p2 = g.e_p2; s.expcut = g.e_expcut;
-    } else {
-        s.x = g.pp; s.h = g.p_h; s.inv_h = g.p_inv_h; s.inv_dx = g.p_inv_dx; s.two_over_p = g.p_2op;
-        s.p2 = g.p_p2; s.expcut = g.p_expcut;
-    }
+    const double *t[GT_NP];
+#pragma unroll
+    for (int i = 0; i < GT_NP; ++i) t[i] = (ps < 0) ? sp_table(g, elec, i) : tab + (size_t)i * np;
+    s.x = t[GT_X]; s.h = t[GT_H]; s.inv_h = t[GT_INVH]; s.inv_dx = t[GT_INVDX]; s.two_over_p = t[GT_2OP];
+    s.p2 = t[GT_P2]; s.expcut = t[GT_EXPCUT]; s.ib2 = t[GT_IB2];
+    s.expE = t[GT_EXPE]; s.sig = t[GT_SIG]; s.Ep = t[GT_EP]; s.i2mk = t[GT_I2MK];
+    s.ps = ps;
     return s;
 }
 
@@ -779,34 +830,33 @@ struct RunCoef {
 };
 
 template <bool ELEC>
-__device__ __forceinline__ void row_BC(const CkGrid &g, const SpGrid &sg, const RunCoef &rc, int j, double radp,
-                                       double &B, double &C)
+__device__ __forceinline__ void row_BC(const SpGrid &sg, const RunCoef &rc, int j, double radp, double &B, double &C)
 {
 #pragma clang fp contract(off)
+    const int i = sg.ix(j);
     double D;
-    if (!rc.model) D = sg.p2[j] * rc.dfac;
-    else if (ELEC) D = sg.p2[j] * rc.dfac * sg.expcut[j];
-    else D = sg.p2[j] * rc.dfac * g.p_expE[j] * sg.expcut[j];
+    if (!rc.model) D = sg.p2[i] * rc.dfac;
+    else if (ELEC) D = sg.p2[i] * rc.dfac * sg.expcut[i];
+    else D = sg.p2[i] * rc.dfac * sg.expE[i] * sg.expcut[i];
     C = D;
-    B = radp - sg.two_over_p[j] * D;
+    B = radp - sg.two_over_p[i] * D;
 }
 
 // Builds the rows of the lane into F (al = a, iv = b, g = c) and factorises.
 template <bool ELEC, int R, int G>
-__device__ __forceinline__ void build_factor(int gl, int n, const CkGrid &g, const RunCoef &rc,
-                                             const double (&radp)[R], const double (&src)[R], Factor<R, G> &F,
-                                             int &bad)
+__device__ __forceinline__ void build_factor(int gl, int n, const SpGrid &sg, const RunCoef &rc,
+                                             const double *radp, const SrcRows &src, Factor<R, G> &F, int &bad)
 {
 #pragma clang fp contract(off)
-    const SpGrid sg = sp_grid(g, ELEC);
     const int base = gl * R;
+    const int o = run_opaque(0);
     double B0, C0, BL, CL;
-    row_BC<ELEC>(g, sg, rc, base, radp[0], B0, C0);
-    row_BC<ELEC>(g, sg, rc, base + R - 1, radp[R - 1], BL, CL);
+    row_BC<ELEC>(sg, rc, base, radp[o], B0, C0);
+    row_BC<ELEC>(sg, rc, base + R - 1, radp[o + R - 1], BL, CL);
     const double Bn = g_down<G>(B0, 1), Cn = g_down<G>(C0, 1);
     const double Bp = g_up<G>(BL, 1), Cp = g_up<G>(CL, 1);
     Face fl = {0.0, 0.0, 0.0, 0.0, 0.0};
-    if (gl > 0) fl = d_face2(Bp, B0, Cp, C0, sg.h[base - 1], sg.inv_h[base - 1]);
+    if (gl > 0) fl = d_face2(Bp, B0, Cp, C0, sg.h[sg.ix(base - 1)], sg.inv_h[sg.ix(base - 1)]);
     double Cl = Cp;
     double Bk = B0, Ck = C0;
 #pragma unroll
@@ -816,12 +866,12 @@ __device__ __forceinline__ void build_factor(int gl, int n, const CkGrid &g, con
         double Br, Cr;
         if (k == R - 1) { Br = Bn; Cr = Cn; }
         else if (k == R - 2) { Br = BL; Cr = CL; }
-        else row_BC<ELEC>(g, sg, rc, j + 1, radp[k + 1], Br, Cr);
+        else row_BC<ELEC>(sg, rc, j + 1, radp[o + k + 1], Br, Cr);
         Face fr = {0.0, 0.0, 0.0, 0.0, 0.0};
-        if (!last) fr = d_face2(Bk, Br, Ck, Cr, sg.h[j], sg.inv_h[j]);
+        if (!last) fr = d_face2(Bk, Br, Ck, Cr, sg.h[sg.ix(j)], sg.inv_h[sg.ix(j)]);
         const int kind = (j == 0) ? 1 : (last ? 2 : 0);
-        const double t1 = rc.dt * sg.inv_dx[j];
-        const double dtT = ELEC ? 0.0 : rc.dtT_fac * g.p_sig[j];
+        const double t1 = rc.dt * sg.inv_dx[sg.ix(j)];
+        const double dtT = ELEC ? 0.0 : rc.dtT_fac * sg.sig[sg.ix(j)];
         double a, b, c;
         d_cc_row2(kind, t1, Ck, dtT, fl, fr, Cl, a, b, c);
         F.al[k] = a;
@@ -832,7 +882,7 @@ __device__ __forceinline__ void build_factor(int gl, int n, const CkGrid &g, con
         Bk = Br;
         Ck = Cr;
     }
-    factorize<R, G>(gl, n, src, F, bad);
+    factorize<R, G>(gl, n, src, o, F, bad);
 }
 
 // ---------------------------------------------------------------------------
@@ -842,14 +892,14 @@ struct SegScalars {
     double n_gas, kbt, bfield, divv, lturb, dv, cs, beta;
 };
 
-// Loss rates of the lane's rows for one segment (prepare_cre_losses_1d,
-// fp_shared_core.c:780-842; prepare_crp_losses_1d_with_threshold :729-778).
-template <bool ELEC, int R>
-__device__ __forceinline__ void seg_radp(const CkGrid &g, const CkSlot &S, int base, int nseg, double alpha,
-                                         const SegScalars &q, double (&radp)[R])
+// Electron loss rates of the lane's rows for one segment
+// (prepare_cre_losses_1d, fp_shared_core.c:780-842).
+template <int R>
+__device__ __forceinline__ void seg_radp_e(const SpGrid &sg, const CkSlot &S, int base, int nseg, double alpha,
+                                           const SegScalars &q, double *radp)
 {
 #pragma clang fp contract(off)
-    if (ELEC) {
+    {
         const double b_micro = q.bfield * 1.0e+6;
         const double b_ratio = b_micro / 3.2;
         const double synch_coeff = 4.8e-4 * b_ratio * b_ratio * 9.11e-28 * 3.0e+10 * 3.1536e+16;
@@ -859,33 +909,25 @@ __device__ __forceinline__ void seg_radp(const CkGrid &g, const CkSlot &S, int b
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             const int j = base + k;
-            const double pe = g.ep[j];
-            const double pe2 = g.e_p2[j];
+            const double pe = sg.x[sg.ix(j)];
+            const double pe2 = sg.p2[sg.ix(j)];
             const double ric = (nseg > 1) ? d_lerp(S.ric_c[j], S.ric_n[j], alpha) : S.ric_c[j];
-            const double coul = coul_coeff * (1.0 + log(pe * ing) * (1.0 / 74.8)) * g.e_ib2[j];
+            const double coul = coul_coeff * (1.0 + log(pe * ing) * (1.0 / 74.8)) * sg.ib2[sg.ix(j)];
             radp[k] = synch_coeff * pe2 + ric + coul + adia * pe;
-        }
-    } else {
-        const double adia = 1.0 / 3.0 * q.divv;
-#pragma unroll
-        for (int k = 0; k < R; ++k) {
-            const double p = g.pp[base + k];
-            radp[k] = d_b_coulomb_p(q.n_gas, p, q.kbt) + adia * p;
         }
     }
 }
 
 // Proton loss rates (prepare_crp_losses_1d_with_threshold,
 // fp_shared_core.c:729-778) evaluated in the strided row order m G + gl and
-// moved to the lane's own block of rows through LDS.  The thermal (erf)
+// left in LDS, where the lane owning each row reads it during the builds.  The thermal (erf)
 // branch of the Coulomb kernel is taken only by the lowest bins; in the
 // block order the lane holding them would make the whole wave take the
 // branch for each of its R rows, in the strided order the wave takes it
 // ceil(rows / G) times.  s_tr holds N + N / R doubles (row j at j + j / R:
 // one pad slot per block of R rows against LDS bank conflicts on the reads).
 template <int N, int R, int G>
-__device__ __forceinline__ void seg_radp_p(const CkGrid &g, int gl, const SegScalars &q, double *s_tr,
-                                           double (&radp)[R])
+__device__ __forceinline__ void seg_radp_p(const SpGrid &sg, int gl, const SegScalars &q, double *s_tr)
 {
 #pragma clang fp contract(off)
     const double adia = 1.0 / 3.0 * q.divv;
@@ -893,24 +935,38 @@ __device__ __forceinline__ void seg_radp_p(const CkGrid &g, int gl, const SegSca
 #pragma unroll
     for (int m = 0; m < R; ++m) {
         const int j = m * G + gl;
-        s_tr[j + j / R] =
-            d_b_coulomb_p_tab(q.n_gas, q.kbt, ikbt, g.p_Ep[j], g.p_i2mk[j], g.p_ib2[j]) + adia * g.pp[j];
+        const int i = sg.ix(j);
+        s_tr[j + j / R] = d_b_coulomb_p_tab(q.n_gas, q.kbt, ikbt, sg.Ep[i], sg.i2mk[i], sg.ib2[i]) + adia * sg.x[i];
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The lane's R spectrum values move to LDS (own slots, stride R + 1) while a
+// build runs and come back before its substeps.
+template <int R>
+__device__ __forceinline__ void park_rows(double *own, const double (&x)[R])
+{
 #pragma unroll
-    for (int k = 0; k < R; ++k) radp[k] = s_tr[gl * (R + 1) + k];
-    __builtin_amdgcn_wave_barrier();
+    for (int k = 0; k < R; ++k) own[k] = x[k];
+}
+template <int R>
+__device__ __forceinline__ void unpark_rows(const double *own, double (&x)[R])
+{
+    const int o = run_opaque(0);
+#pragma unroll
+    for (int k = 0; k < R; ++k) x[k] = own[o + k];
 }
 
 // All runs of one species over one segment: up to three runs of constant
 // coefficients (before / inside / after the on-window, tracer_fp_cpu.c:150-157).
+// On entry the spectrum is parked (park_rows); on exit it is in x.
 template <bool ELEC, int N, int R, int G>
-__device__ __forceinline__ void species_segment(int gl, const CkGrid &g, int steps, bool windowed, int ls, int le,
+__device__ __forceinline__ void species_segment(int gl, const SpGrid &sg, int steps, bool windowed, int ls, int le,
                                                 RunCoef rc, double dfac_on, double dfac_off, bool mode_on,
-                                                const double (&radp)[R], const double (&src)[R], double (&x)[R],
-                                                int &bad)
+                                                const double *radp, const SrcRows &src, double *xpark,
+                                                double (&x)[R], int &bad)
 {
     int s0 = 0;
     while (s0 < steps) {
@@ -924,8 +980,10 @@ __device__ __forceinline__ void species_segment(int gl, const CkGrid &g, int ste
         if (s1 > steps) s1 = steps;
         rc.model = on && mode_on;
         rc.dfac = on ? dfac_on : dfac_off;
+        if (s0 > 0) park_rows<R>(xpark, x);
         Factor<R, G> F;
-        build_factor<ELEC, R, G>(gl, N, g, rc, radp, src, F, bad);
+        build_factor<ELEC, R, G>(gl, N, sg, rc, radp, src, F, bad);
+        unpark_rows<R>(xpark, x);
         for (int s = s0; s < s1; ++s) apply_solve<R, G>(gl, F, x);
         s0 = s1;
     }
@@ -943,10 +1001,42 @@ enum { CK_MODE_E = 0,      // electron_only: protons are exactly zero
        CK_MODE_E_SEC = 3   // electrons only; secondaries from the contracted rows
 };
 
-template <int N, int R, int MODE>
-__device__ __forceinline__ void snapshot_update(const CkGrid &g, const CkSlot &S, int t, int gl, double *s_row,
-                                                double *s_tr, double (&xe)[R], double (&xp)[R], int &bad)
+// LDS doubles per lane group of the fused kernel: loss rates (padded), the
+// parked spectrum, and in mode EP the contraction's protons / electron sources.
+__host__ __device__ constexpr int fused_lds_doubles(int N, int R, int MODE)
 {
+    return N + N / R + 1 + (N / R) * (R + 1) + (MODE == CK_MODE_EP ? N : 0);
+}
+
+// Grid tables copied to LDS per block (electron set, proton set or both).
+__host__ __device__ constexpr int fused_grid_tables(int MODE)
+{
+    return MODE == CK_MODE_EP ? GT_NE + GT_NP : MODE == CK_MODE_P_REC ? GT_NP : GT_NE;
+}
+
+// The grid tables go to LDS when the blocks that the register budget lets
+// share a CU (OCC waves per SIMD, WPB waves per block) still fit its 160 KiB
+// with them; otherwise the build reads the device tables.
+__host__ __device__ constexpr bool fused_grid_in_lds(int N, int R, int WPB, int MODE, int OCC)
+{
+    return (size_t)8 * (WPB * (64 / (N / R)) * fused_lds_doubles(N, R, MODE) +
+                        fused_grid_tables(MODE) * (N + N / R + 1)) *
+               (OCC * 4 / WPB > 0 ? OCC * 4 / WPB : 1) <=
+           (size_t)160 * 1024;
+}
+
+template <int N, int R, int MODE>
+__device__ __forceinline__ void snapshot_update(const CkGrid &g, const SpGrid &sge, const SpGrid &sgp,
+                                                const CkSlot &S, int t, int gl, double *s_grp, double (&xe)[R],
+                                                double (&xp)[R], int &bad)
+{
+    // The group's LDS region (fused_lds_doubles): the loss rates of the
+    // segment (N + N / R + 1 doubles, row j at j + j / R), the parked spectrum
+    // (G (R + 1) doubles, the lane's rows at gl (R + 1)), and in mode EP N
+    // doubles for the segment-start protons of the contraction, then the
+    // electron sources.  Rates and sources are read once per run (sources
+    // without additions straight from the tracer's rows), so they take no
+    // registers during the substeps, and the spectrum takes none during builds.
     constexpr int G = N / R;
     constexpr bool PROT = (MODE == CK_MODE_EP || MODE == CK_MODE_P_REC);
     constexpr bool ELEC = (MODE != CK_MODE_P_REC);
@@ -964,6 +1054,11 @@ __device__ __forceinline__ void snapshot_update(const CkGrid &g, const CkSlot &S
     const bool dis = S.dis ? (S.dis[t] != 0) : false;
     const bool sec = (MODE == CK_MODE_EP) && S.fqe_t;
     const int row0 = (MODE == CK_MODE_P_REC || MODE == CK_MODE_E_SEC) ? S.seg_off[t] : 0;
+    double *const s_rad = s_grp;
+    const double *const own_rad = s_grp + gl * (R + 1);
+    double *const own_x = s_grp + (N + N / R + 1) + gl * (R + 1);
+    double *const s_row = s_grp + (N + N / R + 1) + G * (R + 1);
+    double *const own_src = s_row + base;
 
     for (int iseg = 0; iseg < nseg; ++iseg) {
         const int fb = (int)(((long long)iseg * nsub) / nseg);
@@ -1037,26 +1132,18 @@ __device__ __forceinline__ void snapshot_update(const CkGrid &g, const CkSlot &S
 #pragma unroll
                 for (int k = 0; k < R; ++k) rec[k] = xp[k];
             }
-            double radp[R], src[R];
-            seg_radp_p<N, R, G>(g, gl, q, s_tr, radp);
-#pragma unroll
-            for (int k = 0; k < R; ++k) src[k] = S.qpi ? dt * S.qpi[roff + k] : 0.0;
+            park_rows<R>(own_x, xp);
+            seg_radp_p<N, R, G>(sgp, gl, q, s_rad);
             RunCoef rp = rc;
             rp.dtT_fac = dt * (0.5 * K_C * q.n_gas) * K_GYR;
-            species_segment<false, N, R, G>(gl, g, steps, windowed, ls, le, rp, dfac_on, dfac_off, mode_on, radp, src,
-                                            xp, bad);
+            const SrcRows sp = {S.qpi ? S.qpi + roff : nullptr, nullptr, 0.0, 0.0, dt};
+            species_segment<false, N, R, G>(gl, sgp, steps, windowed, ls, le, rp, dfac_on, dfac_off, mode_on, own_rad,
+                                            sp, own_x, xp, bad);
         }
-        if (ELEC) {
+        if (ELEC && MODE == CK_MODE_EP) {
             double src[R];
 #pragma unroll
             for (int k = 0; k < R; ++k) src[k] = S.qepri ? S.qepri[roff + k] : 0.0;
-            if (MODE == CK_MODE_E_SEC) {
-                // the same source as below, with the proton contraction done
-                // for all tracers at once outside the kernel
-                const double *inj = S.seg_inje + (size_t)(row0 + iseg) * N + base;
-#pragma unroll
-                for (int k = 0; k < R; ++k) src[k] = inj[k] * K_GYR * q.n_gas + src[k];
-            }
             if (sec) {
                 // inje = Gyr n_gas sum_p N_p fqe[p, e] + qepri (tracer_fp_cpu.c:142-147)
                 double acc[R];
@@ -1071,39 +1158,68 @@ __device__ __forceinline__ void snapshot_update(const CkGrid &g, const CkSlot &S
 #pragma unroll
                     for (int k = 0; k < R; ++k) acc[k] += nn * wrow[k];
                 }
+                // every lane of the group has read the protons before the
+                // sources overwrite them
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
                 for (int k = 0; k < R; ++k) src[k] = acc[k] * K_GYR * q.n_gas + src[k];
             }
 #pragma unroll
-            for (int k = 0; k < R; ++k) src[k] = dt * src[k];
-            double radp[R];
-            seg_radp<true, R>(g, S, base, nseg, alpha, q, radp);
-            species_segment<true, N, R, G>(gl, g, steps, windowed, ls, le, rc, dfac_on, dfac_off, mode_on, radp, src,
-                                           xe, bad);
+            for (int k = 0; k < R; ++k) own_src[k] = dt * src[k];
+            park_rows<R>(own_x, xe);
+            seg_radp_e<R>(sge, S, base, nseg, alpha, q, s_grp + gl * (R + 1));
+            const SrcRows se = {own_src, nullptr, 0.0, 0.0, 1.0};
+            species_segment<true, N, R, G>(gl, sge, steps, windowed, ls, le, rc, dfac_on, dfac_off, mode_on, own_rad,
+                                           se, own_x, xe, bad);
+        } else if (ELEC) {
+            // mode E_SEC: the same source as the in-kernel contraction, with
+            // the proton contraction done for all tracers at once outside the
+            // kernel (seg_inje rows)
+            const double *inj =
+                (MODE == CK_MODE_E_SEC) ? S.seg_inje + (size_t)(row0 + iseg) * N + base : nullptr;
+            park_rows<R>(own_x, xe);
+            seg_radp_e<R>(sge, S, base, nseg, alpha, q, s_grp + gl * (R + 1));
+            const SrcRows se = {S.qepri ? S.qepri + roff : nullptr, inj, K_GYR, q.n_gas, dt};
+            species_segment<true, N, R, G>(gl, sge, steps, windowed, ls, le, rc, dfac_on, dfac_off, mode_on, own_rad,
+                                           se, own_x, xe, bad);
         }
     }
 }
 
-template <int N, int R, int WPB, int MODE>
+template <int N, int R, int WPB, int MODE, bool LG>
 __device__ __forceinline__ void k_fused_body(const CkGrid &g, const CkSlot *__restrict__ slots,
                                              const CkTask *__restrict__ tasks, int ntasks,
                                              int *__restrict__ counter, int *__restrict__ err)
 {
     constexpr bool PROT = (MODE == CK_MODE_EP || MODE == CK_MODE_P_REC);
     constexpr bool ELEC = (MODE != CK_MODE_P_REC);
-    constexpr bool SEC_LDS = (MODE == CK_MODE_EP);
     constexpr int G = N / R;
     constexpr int GPW = 64 / G;
-    __shared__ double s_crp[SEC_LDS ? WPB : 1][SEC_LDS ? GPW : 1][SEC_LDS ? N : 1];
-    __shared__ double s_trp[PROT ? WPB : 1][PROT ? GPW : 1][PROT ? N + N / R + 1 : 1];
+    __shared__ double s_lds[WPB][GPW][fused_lds_doubles(N, R, MODE)];
+    constexpr int NP = N + N / R + 1;
+    constexpr int PS = LogG<R>::v;
+    __shared__ double s_tab[LG ? fused_grid_tables(MODE) : 1][LG ? NP : 1];
+    if constexpr (LG) {
+        // the launch's grid tables, row j at j + j / R: electron set first
+        constexpr int NT = fused_grid_tables(MODE);
+        for (int i = threadIdx.x; i < NT * N; i += WPB * 64) {
+            const int tb = i / N, j = i - tb * N;
+            const bool el = ELEC && tb < GT_NE;
+            const int ti = el ? tb : (ELEC ? tb - GT_NE : tb);
+            s_tab[tb][j + (j >> PS)] = sp_table(g, el, ti)[j];
+        }
+        __syncthreads();
+    }
+    const SpGrid sge = LG ? sp_grid(g, true, &s_tab[0][0], NP, PS) : sp_grid(g, true, nullptr, 0, -1);
+    const SpGrid sgp = LG ? sp_grid(g, false, &s_tab[ELEC ? GT_NE : 0][0], NP, PS) : sp_grid(g, false, nullptr, 0, -1);
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int grp = lane / G;
     const int gl = lane % G;
     const int base = gl * R;
-    double *s_row = &s_crp[SEC_LDS ? wave : 0][SEC_LDS ? grp : 0][0];
-    double *s_tr = &s_trp[PROT ? wave : 0][PROT ? grp : 0][0];
+    double *s_grp = &s_lds[wave][grp][0];
     int bad = 0;
 
     for (;;) {
@@ -1127,7 +1243,7 @@ __device__ __forceinline__ void k_fused_body(const CkGrid &g, const CkSlot *__re
             // a task advances one tracer over nslot consecutive snapshots
             for (int sl = task.slot; sl < task.slot + task.nslot; ++sl) {
                 const CkSlot &S = slots[sl];
-                snapshot_update<N, R, MODE>(g, S, t, gl, s_row, s_tr, xe, xp, bad);
+                snapshot_update<N, R, MODE>(g, sge, sgp, S, t, gl, s_grp, xe, xp, bad);
                 if (ELEC && S.out_e) {
 #pragma unroll
                     for (int k = 0; k < R; ++k) S.out_e[roff + k] = xe[k];
@@ -1156,5 +1272,5 @@ __global__ __launch_bounds__(WPB * 64) __attribute__((amdgpu_waves_per_eu(OCC, 8
     CkGrid g, const CkSlot *__restrict__ slots, const CkTask *__restrict__ tasks, int ntasks,
     int *__restrict__ counter, int *__restrict__ err)
 {
-    k_fused_body<N, R, WPB, MODE>(g, slots, tasks, ntasks, counter, err);
+    k_fused_body<N, R, WPB, MODE, fused_grid_in_lds(N, R, WPB, MODE, OCC)>(g, slots, tasks, ntasks, counter, err);
 }
