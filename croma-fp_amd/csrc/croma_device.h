@@ -945,15 +945,18 @@ __device__ __forceinline__ void seg_radp_p(const SpGrid &sg, int gl, const SegSc
 
 // The lane's R spectrum values move to LDS (own slots, stride R + 1) while a
 // build runs and come back before its substeps.
+// own null: the kernel instance keeps the spectrum in registers throughout.
 template <int R>
 __device__ __forceinline__ void park_rows(double *own, const double (&x)[R])
 {
+    if (!own) return;
 #pragma unroll
     for (int k = 0; k < R; ++k) own[k] = x[k];
 }
 template <int R>
 __device__ __forceinline__ void unpark_rows(const double *own, double (&x)[R])
 {
+    if (!own) return;
     const int o = run_opaque(0);
 #pragma unroll
     for (int k = 0; k < R; ++k) x[k] = own[o + k];
@@ -1002,10 +1005,11 @@ enum { CK_MODE_E = 0,      // electron_only: protons are exactly zero
 };
 
 // LDS doubles per lane group of the fused kernel: loss rates (padded), the
-// parked spectrum, and in mode EP the contraction's protons / electron sources.
-__host__ __device__ constexpr int fused_lds_doubles(int N, int R, int MODE)
+// parked spectrum (PK), and in mode EP the contraction's protons / electron
+// sources.
+__host__ __device__ constexpr int fused_lds_doubles(int N, int R, int MODE, bool PK)
 {
-    return N + N / R + 1 + (N / R) * (R + 1) + (MODE == CK_MODE_EP ? N : 0);
+    return N + N / R + 1 + (PK ? (N / R) * (R + 1) : 0) + (MODE == CK_MODE_EP ? N : 0);
 }
 
 // Grid tables copied to LDS per block (electron set, proton set or both).
@@ -1014,18 +1018,28 @@ __host__ __device__ constexpr int fused_grid_tables(int MODE)
     return MODE == CK_MODE_EP ? GT_NE + GT_NP : MODE == CK_MODE_P_REC ? GT_NP : GT_NE;
 }
 
-// The grid tables go to LDS when the blocks that the register budget lets
-// share a CU (OCC waves per SIMD, WPB waves per block) still fit its 160 KiB
-// with them; otherwise the build reads the device tables.
-__host__ __device__ constexpr bool fused_grid_in_lds(int N, int R, int WPB, int MODE, int OCC)
+// Whether the blocks that the register budget lets share a CU (OCC waves per
+// SIMD, WPB waves per block) fit its 160 KiB of LDS with the grid tables (LG)
+// and the parked spectra (PK).
+__host__ __device__ constexpr bool fused_lds_fits(int N, int R, int WPB, int MODE, int OCC, bool LG, bool PK)
 {
-    return (size_t)8 * (WPB * (64 / (N / R)) * fused_lds_doubles(N, R, MODE) +
-                        fused_grid_tables(MODE) * (N + N / R + 1)) *
+    return (size_t)8 * (WPB * (64 / (N / R)) * fused_lds_doubles(N, R, MODE, PK) +
+                        (LG ? fused_grid_tables(MODE) * (N + N / R + 1) : 0)) *
                (OCC * 4 / WPB > 0 ? OCC * 4 / WPB : 1) <=
            (size_t)160 * 1024;
 }
+// LDS use of an instance: the grid tables take precedence over parking (the
+// tables cut the build's load latency, parking its register pressure).
+__host__ __device__ constexpr bool fused_use_lg(int N, int R, int WPB, int MODE, int OCC)
+{
+    return fused_lds_fits(N, R, WPB, MODE, OCC, true, false);
+}
+__host__ __device__ constexpr bool fused_use_pk(int N, int R, int WPB, int MODE, int OCC)
+{
+    return fused_lds_fits(N, R, WPB, MODE, OCC, fused_use_lg(N, R, WPB, MODE, OCC), true);
+}
 
-template <int N, int R, int MODE>
+template <int N, int R, int MODE, bool PK>
 __device__ __forceinline__ void snapshot_update(const CkGrid &g, const SpGrid &sge, const SpGrid &sgp,
                                                 const CkSlot &S, int t, int gl, double *s_grp, double (&xe)[R],
                                                 double (&xp)[R], int &bad)
@@ -1056,8 +1070,8 @@ __device__ __forceinline__ void snapshot_update(const CkGrid &g, const SpGrid &s
     const int row0 = (MODE == CK_MODE_P_REC || MODE == CK_MODE_E_SEC) ? S.seg_off[t] : 0;
     double *const s_rad = s_grp;
     const double *const own_rad = s_grp + gl * (R + 1);
-    double *const own_x = s_grp + (N + N / R + 1) + gl * (R + 1);
-    double *const s_row = s_grp + (N + N / R + 1) + G * (R + 1);
+    double *const own_x = PK ? s_grp + (N + N / R + 1) + gl * (R + 1) : nullptr;
+    double *const s_row = s_grp + (N + N / R + 1) + (PK ? G * (R + 1) : 0);
     double *const own_src = s_row + base;
 
     for (int iseg = 0; iseg < nseg; ++iseg) {
@@ -1188,7 +1202,7 @@ __device__ __forceinline__ void snapshot_update(const CkGrid &g, const SpGrid &s
     }
 }
 
-template <int N, int R, int WPB, int MODE, bool LG>
+template <int N, int R, int WPB, int MODE, bool LG, bool PK>
 __device__ __forceinline__ void k_fused_body(const CkGrid &g, const CkSlot *__restrict__ slots,
                                              const CkTask *__restrict__ tasks, int ntasks,
                                              int *__restrict__ counter, int *__restrict__ err)
@@ -1197,7 +1211,7 @@ __device__ __forceinline__ void k_fused_body(const CkGrid &g, const CkSlot *__re
     constexpr bool ELEC = (MODE != CK_MODE_P_REC);
     constexpr int G = N / R;
     constexpr int GPW = 64 / G;
-    __shared__ double s_lds[WPB][GPW][fused_lds_doubles(N, R, MODE)];
+    __shared__ double s_lds[WPB][GPW][fused_lds_doubles(N, R, MODE, PK)];
     constexpr int NP = N + N / R + 1;
     constexpr int PS = LogG<R>::v;
     __shared__ double s_tab[LG ? fused_grid_tables(MODE) : 1][LG ? NP : 1];
@@ -1243,7 +1257,7 @@ __device__ __forceinline__ void k_fused_body(const CkGrid &g, const CkSlot *__re
             // a task advances one tracer over nslot consecutive snapshots
             for (int sl = task.slot; sl < task.slot + task.nslot; ++sl) {
                 const CkSlot &S = slots[sl];
-                snapshot_update<N, R, MODE>(g, sge, sgp, S, t, gl, s_grp, xe, xp, bad);
+                snapshot_update<N, R, MODE, PK>(g, sge, sgp, S, t, gl, s_grp, xe, xp, bad);
                 if (ELEC && S.out_e) {
 #pragma unroll
                     for (int k = 0; k < R; ++k) S.out_e[roff + k] = xe[k];
@@ -1272,5 +1286,6 @@ __global__ __launch_bounds__(WPB * 64) __attribute__((amdgpu_waves_per_eu(OCC, 8
     CkGrid g, const CkSlot *__restrict__ slots, const CkTask *__restrict__ tasks, int ntasks,
     int *__restrict__ counter, int *__restrict__ err)
 {
-    k_fused_body<N, R, WPB, MODE, fused_grid_in_lds(N, R, WPB, MODE, OCC)>(g, slots, tasks, ntasks, counter, err);
+    k_fused_body<N, R, WPB, MODE, fused_use_lg(N, R, WPB, MODE, OCC), fused_use_pk(N, R, WPB, MODE, OCC)>(
+        g, slots, tasks, ntasks, counter, err);
 }
