@@ -538,6 +538,42 @@ struct SrcRows {
     double ka, kb, scale;
 };
 
+// 2x2 projective maps of the pivot recurrence: m <- m o (matrix product),
+// and renormalisation by a power of two (exact; the map is unchanged).
+__device__ __forceinline__ void pm_mul(double &m00, double &m01, double &m10, double &m11, double o00, double o01,
+                                       double o10, double o11)
+{
+    const double n00 = m00 * o00 + m01 * o10, n01 = m00 * o01 + m01 * o11;
+    const double n10 = m10 * o00 + m11 * o10, n11 = m10 * o01 + m11 * o11;
+    m00 = n00; m01 = n01; m10 = n10; m11 = n11;
+}
+__device__ __forceinline__ void pm_renorm(double &m00, double &m01, double &m10, double &m11)
+{
+    const double mx = fmax(fmax(fabs(m00), fabs(m01)), fmax(fabs(m10), fabs(m11)));
+    if (mx > 0.0 && isfinite(mx)) {
+        const int e = __builtin_amdgcn_frexp_exp(mx);
+        m00 = ldexp(m00, -e); m01 = ldexp(m01, -e); m10 = ldexp(m10, -e); m11 = ldexp(m11, -e);
+    }
+}
+// In-row Hillis-Steele levels of the pivot-map scan (shift 1, 2, 4, 8 by DPP;
+// a source lane outside the row, or outside a group narrower than a row,
+// contributes nothing).
+template <int G, int ST>
+__device__ __forceinline__ void pm_row_scan(int gl, int rl, double &m00, double &m01, double &m10, double &m11)
+{
+    constexpr int LV = LogG<G>::v < 4 ? LogG<G>::v : 4;
+    if constexpr (ST < LV) {
+        constexpr int s = 1 << ST;
+        const double o00 = row_up<s>(m00), o01 = row_up<s>(m01);
+        const double o10 = row_up<s>(m10), o11 = row_up<s>(m11);
+        if ((G >= 16) ? (rl >= s) : (gl >= s)) {
+            pm_mul(m00, m01, m10, m11, o00, o01, o10, o11);
+            pm_renorm(m00, m01, m10, m11);
+        }
+        pm_row_scan<G, ST + 1>(gl, rl, m00, m01, m10, m11);
+    }
+}
+
 // Thomas LU of the rows held in F (on entry al = a, iv = b, g = c), as in
 // fp_thomas_solve_ws (fp_shared_core.c:226-257): denom_i = b_i + a_i cprime_{i-1},
 // cprime_i = -c_i / denom_i, pivot guard |denom| <= 1e-300.  The pivot
@@ -560,22 +596,43 @@ __device__ __forceinline__ void factorize(int gl, int n, const SrcRows &src, int
             m00 = ldexp(m00, -e); m01 = ldexp(m01, -e); m10 = ldexp(m10, -e); m11 = ldexp(m11, -e);
         }
     }
+    // inclusive prefix over the group's lanes, P_l = M_l ... M_0: Hillis-Steele
+    // levels inside 16-lane rows (DPP), then the rows of a wider group take the
+    // full prefix at the last lane of the row before (readlane)
+    const int rl = (int)(threadIdx.x & 15);
+    pm_row_scan<G, 0>(gl, rl, m00, m01, m10, m11);
+    if constexpr (G > 16) {
+        const int row = (int)((threadIdx.x & 63) >> 4);
+        double c00, c01, c10, c11;  // full prefix entering this lane's row
+        if constexpr (G == 32) {
+            // rows (0, 1) and (2, 3) form groups: odd rows take lane 15 / 47
+            const double a00 = rdlane64(m00, 15), a01 = rdlane64(m01, 15);
+            const double a10 = rdlane64(m10, 15), a11 = rdlane64(m11, 15);
+            const double b00 = rdlane64(m00, 47), b01 = rdlane64(m01, 47);
+            const double b10 = rdlane64(m10, 47), b11 = rdlane64(m11, 47);
+            const bool hi = row >= 2;
+            c00 = hi ? b00 : a00; c01 = hi ? b01 : a01; c10 = hi ? b10 : a10; c11 = hi ? b11 : a11;
+        } else {
+            // one group per wave: carries through rows 0, 1, 2
+            double f00 = rdlane64(m00, 15), f01 = rdlane64(m01, 15);
+            double f10 = rdlane64(m10, 15), f11 = rdlane64(m11, 15);  // full prefix at lane 15
+            c00 = f00; c01 = f01; c10 = f10; c11 = f11;
 #pragma unroll
-    for (int s = 1; s < G; s <<= 1) {
-        const double o00 = g_up<G>(m00, s), o01 = g_up<G>(m01, s);
-        const double o10 = g_up<G>(m10, s), o11 = g_up<G>(m11, s);
-        if (gl >= s) {
-            const double n00 = m00 * o00 + m01 * o10, n01 = m00 * o01 + m01 * o11;
-            const double n10 = m10 * o00 + m11 * o10, n11 = m10 * o01 + m11 * o11;
-            m00 = n00; m01 = n01; m10 = n10; m11 = n11;
-            const double mx = fmax(fmax(fabs(m00), fabs(m01)), fmax(fabs(m10), fabs(m11)));
-            if (mx > 0.0 && isfinite(mx)) {
-                const int e = __builtin_amdgcn_frexp_exp(mx);
-                m00 = ldexp(m00, -e); m01 = ldexp(m01, -e); m10 = ldexp(m10, -e); m11 = ldexp(m11, -e);
+            for (int r = 1; r < 3; ++r) {
+                double u00 = rdlane64(m00, 16 * r + 15), u01 = rdlane64(m01, 16 * r + 15);
+                double u10 = rdlane64(m10, 16 * r + 15), u11 = rdlane64(m11, 16 * r + 15);
+                pm_mul(u00, u01, u10, u11, f00, f01, f10, f11);
+                pm_renorm(u00, u01, u10, u11);
+                f00 = u00; f01 = u01; f10 = u10; f11 = u11;
+                if (row == r + 1) { c00 = f00; c01 = f01; c10 = f10; c11 = f11; }
             }
         }
+        if (row & (G / 16 - 1)) {
+            pm_mul(m00, m01, m10, m11, c00, c01, c10, c11);
+            pm_renorm(m00, m01, m10, m11);
+        }
     }
-    const double uin = g_up<G>(m01, 1), vin = g_up<G>(m11, 1);
+    const double uin = wave_up1(m01), vin = wave_up1(m11);
     double cprev = (gl == 0) ? 0.0 : uin / vin;
     const int base = gl * R;
     double A = 1.0, Bq = 1.0;
@@ -599,7 +656,6 @@ __device__ __forceinline__ void factorize(int gl, int n, const SrcRows &src, int
     // Scan multipliers of the substep solve.  Hillis-Steele levels run
     // inside 16-lane rows (DPP); a group spanning several rows completes the
     // scan with per-row carries (see apply_solve).
-    const int rl = (int)(threadIdx.x & 15);
     scan_coeffs<G, 0>(gl, rl, A, Bq, F);
     F.PA = A;
     F.PQ = Bq;
@@ -697,14 +753,17 @@ __device__ __forceinline__ void apply_solve(int gl, const Factor<R, G> &F, doubl
 // construction, so the index arithmetic folds away.
 struct SpGrid {
     const double *x, *h, *inv_h, *inv_dx, *two_over_p, *p2, *expcut, *ib2;
+    const double *lnp;                     // electrons only
     const double *expE, *sig, *Ep, *i2mk;  // protons only
     int ps;                                // < 0: device tables
     __device__ __forceinline__ int ix(int j) const { return ps < 0 ? j : j + (j >> ps); }
 };
 
-// Tables of one species, in the order of their LDS copies.
-enum { GT_X, GT_H, GT_INVH, GT_INVDX, GT_2OP, GT_P2, GT_EXPCUT, GT_IB2, GT_NE,
-       GT_EXPE = GT_NE, GT_SIG, GT_EP, GT_I2MK, GT_NP };
+// Tables of one species, in the order of their LDS copies: the common set,
+// then ln p (electrons, GT_NE tables) or the proton factors (GT_NP tables).
+enum { GT_X, GT_H, GT_INVH, GT_INVDX, GT_2OP, GT_P2, GT_EXPCUT, GT_IB2, GT_NC,
+       GT_LNP = GT_NC, GT_NE,
+       GT_EXPE = GT_NC, GT_SIG, GT_EP, GT_I2MK, GT_NP };
 
 __device__ __forceinline__ const double *sp_table(const CkGrid &g, bool elec, int t)
 {
@@ -717,7 +776,7 @@ __device__ __forceinline__ const double *sp_table(const CkGrid &g, bool elec, in
     case GT_P2: return elec ? g.e_p2 : g.p_p2;
     case GT_EXPCUT: return elec ? g.e_expcut : g.p_expcut;
     case GT_IB2: return elec ? g.e_ib2 : g.p_ib2;
-    case GT_EXPE: return g.p_expE;
+    case GT_EXPE: return elec ? g.e_lnp : g.p_expE;
     case GT_SIG: return g.p_sig;
     case GT_EP: return g.p_Ep;
     default: return g.p_i2mk;
@@ -730,10 +789,15 @@ __device__ __forceinline__ SpGrid sp_grid(const CkGrid &g, bool elec, const doub
     SpGrid s;
     const double *t[GT_NP];
 #pragma unroll
-    for (int i = 0; i < GT_NP; ++i) t[i] = (ps < 0) ? sp_table(g, elec, i) : tab + (size_t)i * np;
+    for (int i = 0; i < GT_NP; ++i)
+        t[i] = (i >= (elec ? GT_NE : GT_NP)) ? nullptr : (ps < 0) ? sp_table(g, elec, i) : tab + (size_t)i * np;
     s.x = t[GT_X]; s.h = t[GT_H]; s.inv_h = t[GT_INVH]; s.inv_dx = t[GT_INVDX]; s.two_over_p = t[GT_2OP];
     s.p2 = t[GT_P2]; s.expcut = t[GT_EXPCUT]; s.ib2 = t[GT_IB2];
-    s.expE = t[GT_EXPE]; s.sig = t[GT_SIG]; s.Ep = t[GT_EP]; s.i2mk = t[GT_I2MK];
+    s.lnp = elec ? t[GT_LNP] : nullptr;
+    s.expE = elec ? nullptr : t[GT_EXPE];
+    s.sig = elec ? nullptr : t[GT_SIG];
+    s.Ep = elec ? nullptr : t[GT_EP];
+    s.i2mk = elec ? nullptr : t[GT_I2MK];
     s.ps = ps;
     return s;
 }
@@ -853,8 +917,10 @@ __device__ __forceinline__ void build_factor(int gl, int n, const SpGrid &sg, co
     double B0, C0, BL, CL;
     row_BC<ELEC>(sg, rc, base, radp[o], B0, C0);
     row_BC<ELEC>(sg, rc, base + R - 1, radp[o + R - 1], BL, CL);
-    const double Bn = g_down<G>(B0, 1), Cn = g_down<G>(C0, 1);
-    const double Bp = g_up<G>(BL, 1), Cp = g_up<G>(CL, 1);
+    // neighbour lanes' edge rows (DPP across the wave; a group's edge lanes
+    // read the next group's values and do not use them)
+    const double Bn = wave_dn1(B0), Cn = wave_dn1(C0);
+    const double Bp = wave_up1(BL), Cp = wave_up1(CL);
     Face fl = {0.0, 0.0, 0.0, 0.0, 0.0};
     if (gl > 0) fl = d_face2(Bp, B0, Cp, C0, sg.h[sg.ix(base - 1)], sg.inv_h[sg.ix(base - 1)]);
     double Cl = Cp;
@@ -905,14 +971,15 @@ __device__ __forceinline__ void seg_radp_e(const SpGrid &sg, const CkSlot &S, in
         const double synch_coeff = 4.8e-4 * b_ratio * b_ratio * 9.11e-28 * 3.0e+10 * 3.1536e+16;
         const double coul_coeff = 3.05e-29 * q.n_gas * K_C * K_GYR / (K_ME * K_GEV);
         const double adia = 1.0 / 3.0 * q.divv;
-        const double ing = 1.0 / q.n_gas;
+        // ln(p / n_gas) as ln p (table) + ln(1 / n_gas), one log per segment
+        const double lning = log(1.0 / q.n_gas);
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             const int j = base + k;
             const double pe = sg.x[sg.ix(j)];
             const double pe2 = sg.p2[sg.ix(j)];
             const double ric = (nseg > 1) ? d_lerp(S.ric_c[j], S.ric_n[j], alpha) : S.ric_c[j];
-            const double coul = coul_coeff * (1.0 + log(pe * ing) * (1.0 / 74.8)) * sg.ib2[sg.ix(j)];
+            const double coul = coul_coeff * (1.0 + (sg.lnp[sg.ix(j)] + lning) * (1.0 / 74.8)) * sg.ib2[sg.ix(j)];
             radp[k] = synch_coeff * pe2 + ric + coul + adia * pe;
         }
     }

@@ -50,8 +50,9 @@ typedef struct {
     const double *e_h, *e_inv_h, *e_inv_dx, *e_2op;
     /* per-bin factors of the loss rates and D_pp, built on the device:
      * proton E_kin = gamma m_p - m_p, 1 / (2 m_p (gamma - 1)), 1 / beta^2,
-     * exp(-E / E_pmax) (rebuilt when the parameters change); electron 1 / beta^2 */
-    const double *p_Ep, *p_i2mk, *p_ib2, *p_expE, *e_ib2;
+     * exp(-E / E_pmax) (rebuilt when the parameters change); electron 1 / beta^2
+     * and ln p */
+    const double *p_Ep, *p_i2mk, *p_ib2, *p_expE, *e_ib2, *e_lnp;
     const double *pow10; /* 10^k for k = -40..40 (index k + 40), as the host's pow gives them */
     double ppm1, ppp1, epm1, epp1;
 } CkGrid;

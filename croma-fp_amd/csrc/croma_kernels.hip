@@ -367,6 +367,7 @@ __global__ void k_grid_loss_tables(CkGrid g, double inv_epmax)
     const_cast<double *>(g.p_expE)[j] = exp(-g.pE[j] * inv_epmax);
     const double pe2 = g.e_p2[j];
     const_cast<double *>(g.e_ib2)[j] = 1.0 / (pe2 / (1.0 + pe2));
+    const_cast<double *>(g.e_lnp)[j] = log(g.ep[j]);
 }
 
 extern "C" int ck_grid_loss_tables(CkGrid g, double epmax, void *stream)
