@@ -71,10 +71,14 @@ __device__ __forceinline__ double d_b_coulomb_p_tab(double nicm, double kBT, dou
     return K_GYR * K_C / (K_MP * K_GEV) * sum * nicm * ib2 * thermal * 3.5e-29;
 }
 
+// Integer powers below are products (pow with an integer exponent differs
+// from them by an ulp or two; these enter t_acc, and through it D_pp, only
+// multiplicatively).
 __device__ double d_I_theta(double x)
 {
 #pragma clang fp contract(off)
-    return pow(x, 4.0) / 4.0 + pow(x, 2.0) - (1.0 + 2.0 * x * x) * log(x) - 5.0 / 4.0;
+    const double x2 = x * x;
+    return x2 * x2 / 4.0 + x2 - (1.0 + 2.0 * x * x) * log(x) - 5.0 / 4.0;
 }
 
 // accelerationtime_TTD / _ASA (FP_Coef.c:634-666)
@@ -85,20 +89,20 @@ __device__ double d_tacc_ttd(double L, double cs, double Ms, double v, int model
     if (model == 1) {
         const double theta = d_I_theta(x);
         if (!(x > 0.0) || !(theta > 0.0) || !(Ms > 0.0) || !(L > 0.0) || !(cs > 0.0)) return 0.0;
-        return 2.5e-3 / (x * theta) * pow(Ms / 0.5, -4.0) * (L / (300.0 * K_KPC)) *
-               pow(cs / 1500.0e5, -1.0);
+        const double m2 = (Ms / 0.5) * (Ms / 0.5);
+        return 2.5e-3 / (x * theta) * (1.0 / (m2 * m2)) * (L / (300.0 * K_KPC)) * (1.0 / (cs / 1500.0e5));
     }
     const double kL = 2.0 * K_PI / L;
-    const double k_cut = 1.04e+4 * pow(Ms, 4.0) * kL;
-    const double kWk_L = 0.5 * pow(v, 2.0);
-    const double kWk_cut = kWk_L * pow(k_cut / kL, -0.5);
+    const double k_cut = 1.04e+4 * ((Ms * Ms) * (Ms * Ms)) * kL;
+    const double kWk_L = 0.5 * (v * v);
+    const double kWk_cut = kWk_L * (1.0 / sqrt(k_cut / kL));
     return K_C / K_PI / d_I_theta(x) / k_cut / kWk_cut / K_GYR;
 }
 
 __device__ double d_tacc_asa(double L, double cs, double Ms, double beta, double psi)
 {
 #pragma clang fp contract(off)
-    return sqrt(6.0 / 5.0) / 12.0 * (K_C / cs / cs) * L / sqrt(beta) / pow(Ms, 3.0) * pow(psi, 3.0) /
+    return sqrt(6.0 / 5.0) / 12.0 * (K_C / cs / cs) * L / sqrt(beta) / (Ms * Ms * Ms) * (psi * psi * psi) /
            K_GYR;
 }
 
