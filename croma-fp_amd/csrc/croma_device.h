@@ -479,6 +479,14 @@ template <int S>
 __device__ __forceinline__ double row_up(double v) { return dpp64<0x110 + S>(v); }
 template <int S>
 __device__ __forceinline__ double row_dn(double v) { return dpp64<0x100 + S>(v); }
+// odd rows (1, 3): the value at lane 15 of the row before (row_bcast:15);
+// even rows: 0
+__device__ __forceinline__ double bcast15_odd(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x142, 0xA, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x142, 0xA, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
 // lane l - 1 / l + 1 across the whole wave (wave_shr:1 / wave_shl:1)
 __device__ __forceinline__ double wave_up1(double v) { return dpp64<0x138>(v); }
 __device__ __forceinline__ double wave_dn1(double v) { return dpp64<0x130>(v); }
@@ -499,7 +507,7 @@ struct Factor {
     double Af[LV + 1];  // forward scan multipliers, zero where the source lane is outside the row / group
     double Qb[LV + 1];  // backward scan multipliers
     double PA, PQ;      // groups wider than a row: in-row inclusive products (prefix of A, suffix of Q)
-    double Tf[4], Tb[4];  // and the row totals of those products (uniform across the wave)
+    double Tf[4], Tb[4];  // and (one group per wave) the row totals of those products
 };
 
 // Multipliers of the in-row scan levels (and the running in-row products).
@@ -663,7 +671,7 @@ __device__ __forceinline__ void factorize(int gl, int n, const SrcRows &src, int
     scan_coeffs<G, 0>(gl, rl, A, Bq, F);
     F.PA = A;
     F.PQ = Bq;
-    if constexpr (G > 16) {
+    if constexpr (G == 64) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             F.Tf[r] = rdlane64(A, 16 * r + 15);
@@ -704,15 +712,18 @@ __device__ __forceinline__ void apply_solve(int gl, const Factor<R, G> &F, doubl
         L = fma(F.al[k], L, t[k]);
     }
     scan_fwd<G, 0, Factor<R, G>::LV>(F.Af, L);
-    if constexpr (G > 16) {
-        // carries between the rows of a group: the running value entering
-        // row r is T_{r-1} + Tf_{r-1} C_{r-1} (T: inclusive in-row value at
-        // the row's last lane); rows starting a group take none
-        constexpr int RG = G / 16;
+    if constexpr (G == 32) {
+        // rows 1 and 3 continue the groups begun in rows 0 and 2: they take
+        // the inclusive value at the last lane of the row before
+        L = fma(F.PA, bcast15_odd(L), L);
+    } else if constexpr (G == 64) {
+        // carries through the four rows of the group: the running value
+        // entering row r is T_{r-1} + Tf_{r-1} C_{r-1} (T: inclusive in-row
+        // value at the row's last lane)
         const double T0 = rdlane64(L, 15), T1 = rdlane64(L, 31), T2 = rdlane64(L, 47);
-        const double C1 = (1 % RG == 0) ? 0.0 : T0;
-        const double C2 = (2 % RG == 0) ? 0.0 : fma(F.Tf[1], C1, T1);
-        const double C3 = (3 % RG == 0) ? 0.0 : fma(F.Tf[2], C2, T2);
+        const double C1 = T0;
+        const double C2 = fma(F.Tf[1], C1, T1);
+        const double C3 = fma(F.Tf[2], C2, T2);
         const int row = (int)((threadIdx.x & 63) >> 4);
         const double C = (row == 1) ? C1 : (row == 2) ? C2 : (row == 3) ? C3 : 0.0;
         L = fma(F.PA, C, L);
@@ -728,12 +739,17 @@ __device__ __forceinline__ void apply_solve(int gl, const Factor<R, G> &F, doubl
 #pragma unroll
     for (int k = R - 1; k >= 0; --k) M = fma(F.g[k], M, t[k]);
     scan_bwd<G, 0, Factor<R, G>::LV>(F.Qb, M);
-    if constexpr (G > 16) {
-        constexpr int RG = G / 16;
+    if constexpr (G == 32) {
+        // rows 0 and 2 take the value at the first lane of the row after
+        const double U1 = rdlane64(M, 16), U3 = rdlane64(M, 48);
+        const int row = (int)((threadIdx.x & 63) >> 4);
+        const double D = (row & 1) ? 0.0 : ((row & 2) ? U3 : U1);
+        M = fma(F.PQ, D, M);
+    } else if constexpr (G == 64) {
         const double U1 = rdlane64(M, 16), U2 = rdlane64(M, 32), U3 = rdlane64(M, 48);
-        const double D2 = (3 % RG == 0) ? 0.0 : U3;
-        const double D1 = (2 % RG == 0) ? 0.0 : fma(F.Tb[2], D2, U2);
-        const double D0 = (1 % RG == 0) ? 0.0 : fma(F.Tb[1], D1, U1);
+        const double D2 = U3;
+        const double D1 = fma(F.Tb[2], D2, U2);
+        const double D0 = fma(F.Tb[1], D1, U1);
         const int row = (int)((threadIdx.x & 63) >> 4);
         const double D = (row == 0) ? D0 : (row == 1) ? D1 : (row == 2) ? D2 : 0.0;
         M = fma(F.PQ, D, M);
