@@ -391,12 +391,12 @@ __device__ __forceinline__ int d_cost_bucket(double cost)
     return k >= CK_NSUB_HIST ? CK_NSUB_HIST - 1 : k;
 }
 
+// Branch-free (selects), so loads of both ends can issue together.
 __device__ __forceinline__ double d_lerp(double cur, double nxt, double alpha)
 {
 #pragma clang fp contract(off)
-    if (alpha <= 0.0) return cur;
-    if (alpha >= 1.0) return nxt;
-    return cur + alpha * (nxt - cur);
+    const double v = cur + alpha * (nxt - cur);
+    return (alpha <= 0.0) ? cur : ((alpha >= 1.0) ? nxt : v);
 }
 
 // ---------------------------------------------------------------------------
@@ -541,14 +541,58 @@ __device__ __forceinline__ int run_opaque(int v)
     return v;
 }
 
-// Source rows of one run: src_k = scale * (inj_k * ka * kb + q_k), a null
-// pointer contributing nothing (q: the tracer's direct source rows, or rows
-// already scaled in LDS; inj: the contracted secondary rows of the segment).
+// Source rows of one run: src_k = scale * (inj_k * ka * kb + q_k) (q: the
+// tracer's direct source rows, or rows already scaled in LDS; inj: the
+// contracted secondary rows of the segment); an absent one points at the
+// lane's rows of the grid's zero table.
 struct SrcRows {
     const double *q;
     const double *inj;
     double ka, kb, scale;
 };
+
+// exp(x) for x <= 0 (the face factor exp(-|w|/2)): Cody-Waite reduction
+// x = n ln2 + r (ln2 split in fdlibm's hi / lo parts), |r| <= ln2 / 2, the
+// degree-12 Taylor polynomial (truncation below 2e-16 relative) and a scale by
+// 2^n; 0 below -1100.  Within 2 ulp of a correctly rounded exp over the
+// normal range (checked against the C library on 2e7 points).
+__device__ __forceinline__ double d_exp_nonpos(double x)
+{
+    x = fmax(x, -1100.0);
+    const double n = rint(x * 1.4426950408889634);
+    double r = fma(-n, 6.93147180369123816490e-01, x);
+    r = fma(-n, 1.90821492927058770002e-10, r);
+    double p = 1.0 / 479001600.0;
+    p = fma(p, r, 1.0 / 39916800.0);
+    p = fma(p, r, 1.0 / 3628800.0);
+    p = fma(p, r, 1.0 / 362880.0);
+    p = fma(p, r, 1.0 / 40320.0);
+    p = fma(p, r, 1.0 / 5040.0);
+    p = fma(p, r, 1.0 / 720.0);
+    p = fma(p, r, 1.0 / 120.0);
+    p = fma(p, r, 1.0 / 24.0);
+    p = fma(p, r, 1.0 / 6.0);
+    p = fma(p, r, 0.5);
+    p = fma(p, r, 1.0);
+    p = fma(p, r, 1.0);
+    return ldexp(p, (int)n);
+}
+
+// a / b by the hardware reciprocal, two Newton steps and one residual
+// correction: the division sequence without the operand scaling and fix-up
+// for denormal, infinite or zero divisors, which the build's divisors (face
+// diffusion averages above 1e-200, Bernoulli denominators in (0, 1], pivots
+// above 1e-300 in magnitude) never are.  Within an ulp of the IEEE quotient.
+__device__ __forceinline__ double d_div_normal(double a, double b)
+{
+    double r = __builtin_amdgcn_rcp(b);
+    double e = fma(-b, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-b, r, 1.0);
+    r = fma(r, e, r);
+    const double q = a * r;
+    return fma(r, fma(-b, q, a), q);
+}
 
 // 2x2 projective maps of the pivot recurrence: m <- m o (matrix product),
 // and renormalisation by a power of two (exact; the map is unchanged).
@@ -593,7 +637,7 @@ __device__ __forceinline__ void pm_row_scan(int gl, int rl, double &m00, double 
 // group's lanes (renormalised by powers of two, so exact up to rounding of
 // the products); inside the lane the reference recurrence is applied.
 template <int R, int G>
-__device__ __forceinline__ void factorize(int gl, int n, const SrcRows &src, int o, Factor<R, G> &F, int &bad)
+__device__ __forceinline__ void factorize(int gl, int n, const double (&sv)[R], Factor<R, G> &F, int &bad)
 {
     double m00 = 1.0, m01 = 0.0, m10 = 0.0, m11 = 1.0;
 #pragma unroll
@@ -602,11 +646,9 @@ __device__ __forceinline__ void factorize(int gl, int n, const SrcRows &src, int
         const double n00 = -c * m10, n01 = -c * m11;
         const double n10 = a * m00 + b * m10, n11 = a * m01 + b * m11;
         m00 = n00; m01 = n01; m10 = n10; m11 = n11;
-        const double mx = fmax(fmax(fabs(m00), fabs(m01)), fmax(fabs(m10), fabs(m11)));
-        if (mx > 0.0 && isfinite(mx)) {
-            const int e = __builtin_amdgcn_frexp_exp(mx);
-            m00 = ldexp(m00, -e); m01 = ldexp(m01, -e); m10 = ldexp(m10, -e); m11 = ldexp(m11, -e);
-        }
+        // renormalised every second row: two rows grow the entries by at most
+        // the product of two rows' coefficient sums, far inside the range
+        if ((k & 1) || k == R - 1) pm_renorm(m00, m01, m10, m11);
     }
     // inclusive prefix over the group's lanes, P_l = M_l ... M_0: Hillis-Steele
     // levels inside 16-lane rows (DPP), then the rows of a wider group take the
@@ -653,13 +695,11 @@ __device__ __forceinline__ void factorize(int gl, int n, const SrcRows &src, int
         const double a = F.al[k], b = F.iv[k], c = F.g[k];
         const double den = b + a * cprev;
         if (!(fabs(den) > 1.0e-300)) bad |= ERR_PIVOT;
-        const double inv = 1.0 / den;
+        const double inv = d_div_normal(1.0, den);
         const double cp = (base + k == n - 1) ? 0.0 : -c * inv;
         F.iv[k] = inv;
         F.al[k] = a * inv;
-        const double qv = src.q ? src.q[o + k] : 0.0;
-        const double sv = src.inj ? src.inj[o + k] * src.ka * src.kb + qv : qv;
-        F.vs[k] = inv * (src.scale * sv);
+        F.vs[k] = inv * sv[k];
         F.g[k] = -cp;
         A *= F.al[k];
         Bq *= F.g[k];
@@ -844,14 +884,14 @@ __device__ __forceinline__ Face d_face2(double Bl, double Br, double Cl, double 
     f.wm = 0.0;
     f.w = 0.0;
     if (fabs(Cl) > 1.0e-200) {
-        const double w = f.bm / cm * h;
+        const double w = d_div_normal(f.bm, cm) * h;
         const double aw = fabs(w);
-        const double h2 = exp(-0.5 * aw);
+        const double h2 = d_exp_nonpos(-0.5 * aw);
         const double E = h2 * h2;
         const bool small = aw < 0.1;
         const double w2 = w * w;
         const double den = small ? h2 * (1.0 + w2 * (1.0 / 24.0) + (w2 * w2) * (1.0 / 1920.0)) : 1.0 - E;
-        const double big = (small ? 1.0 : aw) / den;
+        const double big = d_div_normal(small ? 1.0 : aw, den);
         const double sm = big * E;
         f.wp = (w > 0.0) ? big : sm;
         f.wm = (w > 0.0) ? sm : big;
@@ -934,6 +974,11 @@ __device__ __forceinline__ void build_factor(int gl, int n, const SpGrid &sg, co
 #pragma clang fp contract(off)
     const int base = gl * R;
     const int o = run_opaque(0);
+    // the run's sources first, so their loads overlap the row build (absent
+    // rows point at the grid's zero row: no branches)
+    double sv[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) sv[k] = src.scale * (src.inj[o + k] * src.ka * src.kb + src.q[o + k]);
     double B0, C0, BL, CL;
     row_BC<ELEC>(sg, rc, base, radp[o], B0, C0);
     row_BC<ELEC>(sg, rc, base + R - 1, radp[o + R - 1], BL, CL);
@@ -968,7 +1013,7 @@ __device__ __forceinline__ void build_factor(int gl, int n, const SpGrid &sg, co
         Bk = Br;
         Ck = Cr;
     }
-    factorize<R, G>(gl, n, src, o, F, bad);
+    factorize<R, G>(gl, n, sv, F, bad);
 }
 
 // ---------------------------------------------------------------------------
@@ -998,7 +1043,9 @@ __device__ __forceinline__ void seg_radp_e(const SpGrid &sg, const CkSlot &S, in
             const int j = base + k;
             const double pe = sg.x[sg.ix(j)];
             const double pe2 = sg.p2[sg.ix(j)];
-            const double ric = (nseg > 1) ? d_lerp(S.ric_c[j], S.ric_n[j], alpha) : S.ric_c[j];
+            // both rows loaded unconditionally (ric_n is always a valid row)
+            // alpha is 0 without interpolation; ric_n is always a valid row
+            const double ric = d_lerp(S.ric_c[j], S.ric_n[j], alpha);
             const double coul = coul_coeff * (1.0 + (sg.lnp[sg.ix(j)] + lning) * (1.0 / 74.8)) * sg.ib2[sg.ix(j)];
             radp[k] = synch_coeff * pe2 + ric + coul + adia * pe;
         }
@@ -1092,11 +1139,11 @@ enum { CK_MODE_E = 0,      // electron_only: protons are exactly zero
 };
 
 // LDS doubles per lane group of the fused kernel: loss rates (padded), the
-// parked spectrum (PK), and in mode EP the contraction's protons / electron
-// sources.
+// parked spectrum (PK), in mode EP the contraction's protons / electron
+// sources, and the tracer's backgrounds at two snapshots.
 __host__ __device__ constexpr int fused_lds_doubles(int N, int R, int MODE, bool PK)
 {
-    return N + N / R + 1 + (PK ? (N / R) * (R + 1) : 0) + (MODE == CK_MODE_EP ? N : 0);
+    return N + N / R + 1 + (PK ? (N / R) * (R + 1) : 0) + (MODE == CK_MODE_EP ? N : 0) + 2 * CK_NBG;
 }
 
 // Grid tables copied to LDS per block (electron set, proton set or both).
@@ -1160,6 +1207,13 @@ __device__ __forceinline__ void snapshot_update(const CkGrid &g, const SpGrid &s
     double *const own_x = PK ? s_grp + (N + N / R + 1) + gl * (R + 1) : nullptr;
     double *const s_row = s_grp + (N + N / R + 1) + (PK ? G * (R + 1) : 0);
     double *const own_src = s_row + base;
+    double *const s_bg = s_row + (MODE == CK_MODE_EP ? N : 0);
+    // the tracer's backgrounds at this snapshot and the next, one round trip
+    // for the group, then read from LDS by every segment
+    for (int i = gl; i < 2 * CK_NBG; i += G) s_bg[i] = (i < CK_NBG ? S.bgc[i] : S.bgn[i - CK_NBG])[t];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
     for (int iseg = 0; iseg < nseg; ++iseg) {
         const int fb = (int)(((long long)iseg * nsub) / nseg);
@@ -1169,10 +1223,10 @@ __device__ __forceinline__ void snapshot_update(const CkGrid &g, const SpGrid &s
         const double alpha = (nseg > 1) ? ((double)iseg + 0.5) / (double)nseg : 0.0;
         SegScalars q;
         {
+            // alpha is 0 (the current snapshot's value) without interpolation
             double v[CK_NBG];
 #pragma unroll
-            for (int f = 0; f < CK_NBG; ++f)
-                v[f] = (nseg > 1) ? d_lerp(S.bgc[f][t], S.bgn[f][t], alpha) : S.bgc[f][t];
+            for (int f = 0; f < CK_NBG; ++f) v[f] = d_lerp(s_bg[f], s_bg[CK_NBG + f], alpha);
             q.n_gas = v[CK_BG_NGAS];
             q.kbt = v[CK_BG_KBT];
             q.bfield = v[CK_BG_B];
@@ -1237,7 +1291,7 @@ __device__ __forceinline__ void snapshot_update(const CkGrid &g, const SpGrid &s
             seg_radp_p<N, R, G>(sgp, gl, q, s_rad);
             RunCoef rp = rc;
             rp.dtT_fac = dt * (0.5 * K_C * q.n_gas) * K_GYR;
-            const SrcRows sp = {S.qpi ? S.qpi + roff : nullptr, nullptr, 0.0, 0.0, dt};
+            const SrcRows sp = {S.qpi ? S.qpi + roff : g.zeros + base, g.zeros + base, 0.0, 0.0, dt};
             species_segment<false, N, R, G>(gl, sgp, steps, windowed, ls, le, rp, dfac_on, dfac_off, mode_on, own_rad,
                                             sp, own_x, xp, bad);
         }
@@ -1271,7 +1325,7 @@ __device__ __forceinline__ void snapshot_update(const CkGrid &g, const SpGrid &s
             for (int k = 0; k < R; ++k) own_src[k] = dt * src[k];
             park_rows<R>(own_x, xe);
             seg_radp_e<R>(sge, S, base, nseg, alpha, q, s_grp + gl * (R + 1));
-            const SrcRows se = {own_src, nullptr, 0.0, 0.0, 1.0};
+            const SrcRows se = {own_src, g.zeros + base, 0.0, 0.0, 1.0};
             species_segment<true, N, R, G>(gl, sge, steps, windowed, ls, le, rc, dfac_on, dfac_off, mode_on, own_rad,
                                            se, own_x, xe, bad);
         } else if (ELEC) {
@@ -1279,10 +1333,10 @@ __device__ __forceinline__ void snapshot_update(const CkGrid &g, const SpGrid &s
             // the proton contraction done for all tracers at once outside the
             // kernel (seg_inje rows)
             const double *inj =
-                (MODE == CK_MODE_E_SEC) ? S.seg_inje + (size_t)(row0 + iseg) * N + base : nullptr;
+                (MODE == CK_MODE_E_SEC) ? S.seg_inje + (size_t)(row0 + iseg) * N + base : g.zeros + base;
             park_rows<R>(own_x, xe);
             seg_radp_e<R>(sge, S, base, nseg, alpha, q, s_grp + gl * (R + 1));
-            const SrcRows se = {S.qepri ? S.qepri + roff : nullptr, inj, K_GYR, q.n_gas, dt};
+            const SrcRows se = {S.qepri ? S.qepri + roff : g.zeros + base, inj, K_GYR, q.n_gas, dt};
             species_segment<true, N, R, G>(gl, sge, steps, windowed, ls, le, rc, dfac_on, dfac_off, mode_on, own_rad,
                                            se, own_x, xe, bad);
         }
@@ -1314,7 +1368,8 @@ __device__ __forceinline__ void k_fused_body(const CkGrid &g, const CkSlot *__re
         __syncthreads();
     }
     const SpGrid sge = LG ? sp_grid(g, true, &s_tab[0][0], NP, PS) : sp_grid(g, true, nullptr, 0, -1);
-    const SpGrid sgp = LG ? sp_grid(g, false, &s_tab[ELEC ? GT_NE : 0][0], NP, PS) : sp_grid(g, false, nullptr, 0, -1);
+    const SpGrid sgp =
+        LG ? sp_grid(g, false, &s_tab[0][0] + ((LG && ELEC) ? GT_NE * NP : 0), NP, PS) : sp_grid(g, false, nullptr, 0, -1);
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int grp = lane / G;

@@ -53,6 +53,7 @@ typedef struct {
      * exp(-E / E_pmax) (rebuilt when the parameters change); electron 1 / beta^2
      * and ln p */
     const double *p_Ep, *p_i2mk, *p_ib2, *p_expE, *e_ib2, *e_lnp;
+    const double *zeros; /* n zeros: the rows of an absent source */
     const double *pow10; /* 10^k for k = -40..40 (index k + 40), as the host's pow gives them */
     double ppm1, ppp1, epm1, epp1;
 } CkGrid;

@@ -368,6 +368,7 @@ __global__ void k_grid_loss_tables(CkGrid g, double inv_epmax)
     const double pe2 = g.e_p2[j];
     const_cast<double *>(g.e_ib2)[j] = 1.0 / (pe2 / (1.0 + pe2));
     const_cast<double *>(g.e_lnp)[j] = log(g.ep[j]);
+    const_cast<double *>(g.zeros)[j] = 0.0;
 }
 
 extern "C" int ck_grid_loss_tables(CkGrid g, double epmax, void *stream)

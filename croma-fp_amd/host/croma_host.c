@@ -207,7 +207,7 @@ static void derived_tables(int n, const double *x, const double *dx, double *h, 
     }
 }
 
-#define CH_NTAB 27 /* 21 uploaded, 6 segment-build factor tables built on the device */
+#define CH_NTAB 28 /* 21 uploaded, 6 segment-build factor tables and a zero row built on the device */
 #define CH_NPOW10 81 /* 10^k, k = -40..40, for the nsub ladder on the device */
 
 static int ctx_upload_grid(CromaHipContext *c)
@@ -268,6 +268,7 @@ static int ctx_upload_grid(CromaHipContext *c)
     c->dgrid.p_expE = d + 24 * (size_t)n;
     c->dgrid.e_ib2 = d + 25 * (size_t)n;
     c->dgrid.e_lnp = d + 26 * (size_t)n;
+    c->dgrid.zeros = d + 27 * (size_t)n;
     c->dgrid.pow10 = d + CH_NTAB * (size_t)n;
     c->dgrid.ppm1 = c->g.ppm1;
     c->dgrid.ppp1 = c->g.ppp1;
