@@ -14,8 +14,12 @@
 //   k_nsub_scan   one block: exclusive scans of the block row totals and of
 //                 the histogram in descending bucket order
 //   k_nsub_place  one thread per tracer: segment offsets and the task slot
-// Within one cost bucket the task order follows the atomics; results do not
-// depend on it (each tracer's update is independent of its queue position).
+// Both per-tracer kernels count in a block-local LDS histogram and touch the
+// global one once per non-empty bucket of the block: most tracers of a
+// workload share a few buckets, and one global atomic per tracer on them
+// serialises.  Within one cost bucket the task order follows the atomics;
+// results do not depend on it (each tracer's update is independent of its
+// queue position).
 #include <hip/hip_runtime.h>
 
 #include "croma_device.h"
@@ -29,8 +33,11 @@ __global__ __launch_bounds__(NS_BLOCK) void k_nsub_count(CkGrid g, const CkSlot 
                                                          unsigned long long *__restrict__ cells)
 {
     __shared__ int s_wsum[NS_BLOCK / 64];
+    __shared__ int s_hist[CK_NSUB_HIST];
     const int i = blockIdx.x * NS_BLOCK + threadIdx.x;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int b = threadIdx.x; b < CK_NSUB_HIST; b += NS_BLOCK) s_hist[b] = 0;
+    __syncthreads();
     int rows = 0;
     unsigned long long cs = 0;
     if (i < ntr) {
@@ -50,7 +57,12 @@ __global__ __launch_bounds__(NS_BLOCK) void k_nsub_count(CkGrid g, const CkSlot 
         }
         const int k = d_cost_bucket(cost);
         key[i] = k;
-        atomicAdd(&hist[k], 1);
+        atomicAdd(&s_hist[k], 1);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < CK_NSUB_HIST; b += NS_BLOCK) {
+        const int c = s_hist[b];
+        if (c) atomicAdd(&hist[b], c);
     }
     // sum nsub: one atomic per wave
     for (int o = 32; o > 0; o >>= 1) cs += __shfl_xor(cs, o, 64);
@@ -119,7 +131,20 @@ __global__ __launch_bounds__(NS_BLOCK) void k_nsub_place(const CkSlot *__restric
                                                          const int *__restrict__ segloc, const int *__restrict__ blk,
                                                          int *__restrict__ hist, CkTask *__restrict__ tasks)
 {
+    __shared__ int s_hist[CK_NSUB_HIST];
     const int i = blockIdx.x * NS_BLOCK + threadIdx.x;
+    for (int b = threadIdx.x; b < CK_NSUB_HIST; b += NS_BLOCK) s_hist[b] = 0;
+    __syncthreads();
+    // rank of the tracer among the block's tracers of its bucket
+    const int k = (i < ntr) ? key[i] : 0;
+    const int local = (i < ntr) ? atomicAdd(&s_hist[k], 1) : 0;
+    __syncthreads();
+    // the block's range in each non-empty bucket
+    for (int b = threadIdx.x; b < CK_NSUB_HIST; b += NS_BLOCK) {
+        const int c = s_hist[b];
+        if (c) s_hist[b] = atomicAdd(&hist[b], c);
+    }
+    __syncthreads();
     if (i >= ntr) return;
     if (split) {
         int off = blk[blockIdx.x] + segloc[i];
@@ -130,7 +155,7 @@ __global__ __launch_bounds__(NS_BLOCK) void k_nsub_place(const CkSlot *__restric
                                      S.prm.coeff_interp_max_segments);
         }
     }
-    const int pos = atomicAdd(&hist[key[i]], 1);
+    const int pos = s_hist[k] + local;
     CkTask t;
     t.slot = 0;
     t.tracer = i;
