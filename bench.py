@@ -179,14 +179,16 @@ def emission_bench(ch, ntracer=1_000_000, n=256, nbins=64, seed=11):
         table, ms_tab = ctx.synch_kernel_table(nbins, ntheta, nlogb, logb_min, inv_dlogb, fx["logx"], fx["logfx"],
                                                fx["fx"], float(fx["xmin"][0]), logy.ravel(), pw)
         bfield = 10.0 ** rng.uniform(-8.0, -5.0, ntracer)
-        ms = []
-        for _ in range(2):
-            _, t = ctx.emission_synch(spec, bfield, table, nbins, nlogb, logb_min, inv_dlogb)
-            ms.append(t)
-        t = ms[-1]
-        res["synch"] = {"fp64": {"ms": t, "hbm_gbs": (byts + 8.0 * ntracer) / (t * 1e-3) / 1e9,
-                                 "hbm_frac": (byts + 8.0 * ntracer) / (t * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                                 "table_build_ms": ms_tab}}
+        res["synch"] = {}
+        for mname, mode in (("fp64", ch.EMIS_FP64), ("fp32", ch.EMIS_FP32)):
+            ms = []
+            for _ in range(2):
+                _, t = ctx.emission_synch(spec, bfield, table, nbins, nlogb, logb_min, inv_dlogb, mode=mode)
+                ms.append(t)
+            t = ms[-1]
+            res["synch"][mname] = {"ms": t, "hbm_gbs": (byts + 8.0 * ntracer) / (t * 1e-3) / 1e9,
+                                   "hbm_frac": (byts + 8.0 * ntracer) / (t * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                   "table_build_ms": ms_tab}
         out["kernels"] = res
         out["bytes_per_call"] = byts
         out["flops_per_call"] = flops

@@ -252,13 +252,14 @@ class Context:
              "synch_kernel_table")
         return table, ms.value
 
-    def emission_synch(self, cre, b_field, table, nfreq, nlogb, logb_min, inv_dlogb):
+    def emission_synch(self, cre, b_field, table, nfreq, nlogb, logb_min, inv_dlogb, mode=EMIS_FP64):
         nt = len(b_field)
         eps = np.zeros(nt * nfreq)
         ms = C.c_double()
-        _chk(lib().croma_hip_emission_synch(C.c_void_p(self.h), nt, int(nfreq), int(nlogb), C.c_double(logb_min),
-                                            C.c_double(inv_dlogb), _p(f64(table)), _p(f64(cre)),
-                                            _p(f64(b_field)), _p(eps), C.byref(ms)), "emission_synch")
+        _chk(lib().croma_hip_emission_synch_mode(C.c_void_p(self.h), nt, int(nfreq), int(nlogb),
+                                                 C.c_double(logb_min), C.c_double(inv_dlogb), _p(f64(table)),
+                                                 _p(f64(cre)), _p(f64(b_field)), _p(eps), int(mode), C.byref(ms)),
+             "emission_synch")
         return eps.reshape(nt, nfreq), ms.value
 
     def nsub(self, dt_snap, n_gas, kbt, b_field, divv, ric_lo, ric_hi, l_turb, dv):

@@ -500,6 +500,20 @@ __global__ __launch_bounds__(256) void k_synch_logt(int nlogb, int nfreq, int n,
     logt[o] = v > 0.0 ? log(v) : __builtin_nan("");
 }
 
+// The same table as float log2 (FP32 mode), layout [ib][je][nf]: -inf for a
+// zero entry, NaN for a negative one.
+__global__ __launch_bounds__(256) void k_synch_log2t32(int nlogb, int nfreq, int n, const double *__restrict__ table,
+                                                       float *__restrict__ lt)
+{
+    const size_t o = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (o >= (size_t)nlogb * nfreq * n) return;
+    const int nf = (int)(o % nfreq);
+    const int je = (int)((o / nfreq) % n);
+    const size_t ib = o / ((size_t)nfreq * n);
+    const double v = table[(ib * nfreq + nf) * n + je];
+    lt[o] = v > 0.0 ? (float)log2(v) : (v == 0.0 ? -__builtin_inff() : __builtin_nanf(""));
+}
+
 // Synchrotron: one wave per tracer, lanes over frequencies.  The kernel is
 // interpolated in log B between the table planes ib, ib+1 as the reference
 // does (log-linear when both entries are positive, else linear); the sum
@@ -519,15 +533,33 @@ __device__ __forceinline__ int em_synch_plane(double bval, double logb_min, doub
 
 // Tracers grouped by table plane (counting sort): waves that run one after
 // the other read the same two planes, which then stay in L2.
+// Up to EM_LDS_HIST planes the per-tracer kernels count in a block-local LDS
+// histogram and add to the global one once per non-empty plane of the block
+// (one global atomic per tracer serialises on the planes many tracers share).
+#define EM_LDS_HIST 2048
 __global__ __launch_bounds__(256) void k_synch_plane_count(int ntr, const double *__restrict__ b_field,
                                                            double logb_min, double inv_dlogb, int nlogb,
                                                            int *__restrict__ key, int *__restrict__ hist)
 {
+    __shared__ int s_hist[EM_LDS_HIST];
+    const bool local = nlogb <= EM_LDS_HIST;
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= ntr) return;
-    const int ib = em_synch_plane(b_field[i], logb_min, inv_dlogb, nlogb);
-    key[i] = ib;
-    atomicAdd(&hist[ib], 1);
+    if (local) {
+        for (int b = threadIdx.x; b < nlogb; b += 256) s_hist[b] = 0;
+        __syncthreads();
+    }
+    if (i < ntr) {
+        const int ib = em_synch_plane(b_field[i], logb_min, inv_dlogb, nlogb);
+        key[i] = ib;
+        atomicAdd(local ? &s_hist[ib] : &hist[ib], 1);
+    }
+    if (local) {
+        __syncthreads();
+        for (int b = threadIdx.x; b < nlogb; b += 256) {
+            const int c = s_hist[b];
+            if (c) atomicAdd(&hist[b], c);
+        }
+    }
 }
 
 __global__ __launch_bounds__(256) void k_synch_plane_scan(int nh, int *__restrict__ hist)
@@ -555,12 +587,26 @@ __global__ __launch_bounds__(256) void k_synch_plane_scan(int nh, int *__restric
     }
 }
 
-__global__ __launch_bounds__(256) void k_synch_plane_place(int ntr, const int *__restrict__ key,
+__global__ __launch_bounds__(256) void k_synch_plane_place(int ntr, int nlogb, const int *__restrict__ key,
                                                            int *__restrict__ hist, int *__restrict__ perm)
 {
+    __shared__ int s_hist[EM_LDS_HIST];
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= ntr) return;
-    perm[atomicAdd(&hist[key[i]], 1)] = i;
+    if (nlogb > EM_LDS_HIST) {
+        if (i < ntr) perm[atomicAdd(&hist[key[i]], 1)] = i;
+        return;
+    }
+    for (int b = threadIdx.x; b < nlogb; b += 256) s_hist[b] = 0;
+    __syncthreads();
+    const int k = i < ntr ? key[i] : 0;
+    const int rank = i < ntr ? atomicAdd(&s_hist[k], 1) : 0;
+    __syncthreads();
+    for (int b = threadIdx.x; b < nlogb; b += 256) {
+        const int c = s_hist[b];
+        if (c) s_hist[b] = atomicAdd(&hist[b], c);
+    }
+    __syncthreads();
+    if (i < ntr) perm[s_hist[k] + rank] = i;
 }
 
 template <typename T>
@@ -650,6 +696,107 @@ __global__ __launch_bounds__(256) void k_emis_synch(int ntr, int n, int nfreq, i
     }
 }
 
+// FP32 mode of the same sum (emissivities within 1e-4).  Every term
+// w K = sign(w) 2^(log2|w| + log2 K) is accumulated relative to the running
+// largest exponent of the lane's sum (rescaled once per block of 8 bins), so
+// no term leaves float's range however far the spectrum and the kernel span:
+// log2|w| per bin in LDS (w = N dp), log2 K interpolated on the float log2
+// table, the hardware exp2, and the FP64 result A acc 2^E.
+__global__ __launch_bounds__(256) void k_emis_synch32(int ntr, int n, int nfreq, int nlogb, double logb_min,
+                                                      double inv_dlogb, const double *__restrict__ table,
+                                                      const float *__restrict__ lt, const int *__restrict__ perm,
+                                                      const double *__restrict__ cre, const double *__restrict__ dp,
+                                                      const double *__restrict__ b_field, double *__restrict__ eps)
+{
+    __shared__ float s_lw[4][512];
+    __shared__ float s_sg[4][512];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int slot = blockIdx.x * 4 + w;
+    if (slot >= ntr) return;
+    const int it = perm ? perm[slot] : slot;
+    const double *cl = cre + (size_t)it * n;
+    int jend = n - 1;
+    for (int base = 2; base < n - 1; base += 64) {
+        const int je = base + lane;
+        const bool stop = je < n - 1 && !(cl[je + 1] > 0.0);
+        const unsigned long long m = __ballot(stop);
+        if (m) {
+            jend = base + __ffsll((long long)m) - 1;
+            break;
+        }
+    }
+    for (int je = lane; je < n; je += 64) {
+        const double v = cl[je] * dp[je];
+        s_lw[w][je] = v != 0.0 ? (float)log2(fabs(v)) : -__builtin_inff();
+        s_sg[w][je] = v > 0.0 ? 1.0f : (v < 0.0 ? -1.0f : 0.0f);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    double bval = b_field[it];
+    if (!(bval > 0.0) || !isfinite(bval)) bval = 1.0e-30;
+    const double u = (log10(bval / 1.0e-6) - logb_min) * inv_dlogb;
+    int ib = (int)floor(u);
+    if (ib < 0) ib = 0;
+    if (ib > nlogb - 2) ib = nlogb - 2;
+    double tt = u - (double)ib;
+    if (tt < 0.0) tt = 0.0;
+    if (tt > 1.0) tt = 1.0;
+    const float t = (float)tt;
+    const double A = sqrt(3.0) * pow(4.8e-10, 3.0) * b_field[it] / (2.0 * 9.11e-28 * 2.999e10 * 2.999e10) /
+                     (4.0 * 3.14159265358979323846);
+    const float ninf = -__builtin_inff();
+    // one plane's entry zero: the linear interpolation is t K1 or (1 - t) K0
+    const float l2t = tt > 0.0 ? (float)log2(tt) : ninf;
+    const float l2u = tt < 1.0 ? (float)log2(1.0 - tt) : ninf;
+    for (int nf = lane; nf < nfreq; nf += 64) {
+        const float *l0p = lt + (size_t)ib * n * nfreq + nf;
+        const float *l1p = l0p + (size_t)n * nfreq;
+        const double *r0 = table + ((size_t)ib * nfreq + nf) * n;
+        const double *r1 = table + ((size_t)(ib + 1) * nfreq + nf) * n;
+        float E = ninf, acc = 0.0f;
+        for (int j0 = 2; j0 < jend; j0 += 8) {
+            float a[8], b[8], x[8], sg[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int je = j0 + q < jend ? j0 + q : j0;
+                a[q] = l0p[(size_t)je * nfreq];
+                b[q] = l1p[(size_t)je * nfreq];
+            }
+            float mx = ninf;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int je = j0 + q;
+                x[q] = ninf;
+                sg[q] = 0.0f;
+                if (je < jend) {
+                    // log-linear where both entries are positive, else linear
+                    const bool pa = a[q] > ninf, pb = b[q] > ninf;
+                    float lk = (pa && pb) ? fmaf(t, b[q] - a[q], a[q]) : (pa ? l2u + a[q] : (pb ? l2t + b[q] : ninf));
+                    float sk = 1.0f;
+                    if (__builtin_isnan(a[q]) || __builtin_isnan(b[q])) {
+                        // a negative entry: the interpolation in FP64
+                        const double K = (1.0 - tt) * r0[je] + tt * r1[je];
+                        lk = K != 0.0 ? (float)log2(fabs(K)) : ninf;
+                        sk = K < 0.0 ? -1.0f : 1.0f;
+                    }
+                    x[q] = s_lw[w][je] + lk;
+                    sg[q] = s_sg[w][je] * sk;
+                }
+                mx = fmaxf(mx, x[q]);
+            }
+            if (mx > E) {
+                acc *= __builtin_amdgcn_exp2f(E - mx);
+                E = mx;
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                if (x[q] > ninf) acc = fmaf(sg[q], __builtin_amdgcn_exp2f(x[q] - E), acc);
+        }
+        eps[(size_t)it * nfreq + nf] = (E > ninf) ? A * (double)acc * exp2((double)E) : 0.0;
+    }
+}
+
 extern "C" int ck_emission_synch(int ntr, int n, int nfreq, int nlogb, double logb_min, double inv_dlogb,
                                  const double *table, double *logt, int *work, const double *cre, const double *dp,
                                  const double *b_field, double *eps, int fp64, void *stream)
@@ -665,18 +812,22 @@ extern "C" int ck_emission_synch(int ntr, int n, int nfreq, int nlogb, double lo
         hipLaunchKernelGGL(k_synch_plane_count, g1, dim3(256), 0, st, ntr, b_field, logb_min, inv_dlogb, nlogb, key,
                            hist);
         hipLaunchKernelGGL(k_synch_plane_scan, dim3(1), dim3(256), 0, st, nlogb, hist);
-        hipLaunchKernelGGL(k_synch_plane_place, g1, dim3(256), 0, st, ntr, key, hist, perm);
+        hipLaunchKernelGGL(k_synch_plane_place, g1, dim3(256), 0, st, ntr, nlogb, key, hist, perm);
     }
-    if (fp64 && logt) {
-        const size_t tot = (size_t)nlogb * nfreq * n;
+    const size_t tot = (size_t)nlogb * nfreq * n;
+    if (fp64 && logt)
         hipLaunchKernelGGL(k_synch_logt, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, nlogb, nfreq, n,
                            table, logt);
+    if (!fp64) {
+        if (!logt) return -1;
+        float *lt = (float *)logt;
+        hipLaunchKernelGGL(k_synch_log2t32, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, nlogb, nfreq, n,
+                           table, lt);
+        hipLaunchKernelGGL(k_emis_synch32, grid, dim3(256), 0, st, ntr, n, nfreq, nlogb, logb_min, inv_dlogb, table,
+                           (const float *)lt, (const int *)perm, cre, dp, b_field, eps);
+        return hipGetLastError() == hipSuccess ? 0 : -1;
     }
-    if (fp64)
-        hipLaunchKernelGGL(k_emis_synch<double>, grid, dim3(256), 0, st, ntr, n, nfreq, nlogb, logb_min, inv_dlogb,
-                           table, (const double *)logt, (const int *)perm, cre, dp, b_field, eps);
-    else
-        hipLaunchKernelGGL(k_emis_synch<float>, grid, dim3(256), 0, st, ntr, n, nfreq, nlogb, logb_min, inv_dlogb,
-                           table, (const double *)nullptr, (const int *)perm, cre, dp, b_field, eps);
+    hipLaunchKernelGGL(k_emis_synch<double>, grid, dim3(256), 0, st, ntr, n, nfreq, nlogb, logb_min, inv_dlogb,
+                       table, (const double *)logt, (const int *)perm, cre, dp, b_field, eps);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }

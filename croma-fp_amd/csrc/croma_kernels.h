@@ -232,8 +232,9 @@ int ck_synch_table(int n, int nfreq, int ntheta, int nlogb, double logb_min, dou
                    double xmin, const double *logx_tab, const double *logfx_tab, const double *fx_tab,
                    const double *logy, const double *pitch_weight, double *table, void *stream);
 /* Synchrotron: per-tracer log-B interpolated table contraction (VALU, FP64 or FP32).
- * logt (FP64 only; NULL computes logarithms per element): workspace of
- * nlogb * nfreq * n doubles, filled here with the table's logarithms.
+ * logt: workspace of nlogb * nfreq * n doubles, filled here with the table's
+ * logarithms (FP64: natural log, NULL computes them per element; FP32: float
+ * log2 in its first half, required).
  * work (NULL: tracers in input order): 2 * ntr + nlogb ints, used to run the
  * tracers grouped by table plane. */
 int ck_emission_synch(int ntr, int n, int nfreq, int nlogb, double logb_min, double inv_dlogb,

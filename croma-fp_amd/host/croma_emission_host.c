@@ -235,12 +235,21 @@ int croma_hip_emission_synch(const CromaHipContext *ctx, int ntracer, int nfreq,
                              double inv_dlogb, const double *table, const double *cre, const double *b_field,
                              double *eps, double *elapsed_ms)
 {
+    return croma_hip_emission_synch_mode(ctx, ntracer, nfreq, nlogb, logb_min, inv_dlogb, table, cre, b_field, eps,
+                                         CROMA_HIP_EMIS_FP64, elapsed_ms);
+}
+
+int croma_hip_emission_synch_mode(const CromaHipContext *ctx, int ntracer, int nfreq, int nlogb, double logb_min,
+                                  double inv_dlogb, const double *table, const double *cre, const double *b_field,
+                                  double *eps, int mode, double *elapsed_ms)
+{
     const ChGrid *g;
     const size_t nt_tab = (size_t)nlogb * nfreq;
     char *d = 0, *p;
     void *st = 0, *e0 = 0, *e1 = 0;
     int n, rc = 0;
-    if (!ctx || ntracer <= 0 || nfreq <= 0 || nlogb < 2 || !table || !cre || !b_field || !eps)
+    if (!ctx || ntracer <= 0 || nfreq <= 0 || nlogb < 2 || !table || !cre || !b_field || !eps ||
+        (mode != CROMA_HIP_EMIS_FP64 && mode != CROMA_HIP_EMIS_FP32))
         return CROMA_HIP_EINVAL;
     g = ch_ctx_grid(ctx);
     n = g->n;
@@ -275,7 +284,7 @@ int croma_hip_emission_synch(const CromaHipContext *ctx, int ntracer, int nfreq,
             ck_h2d(d_dp, g->edp, (size_t)n * 8, st) || ck_h2d(d_b, b_field, (size_t)ntracer * 8, st) ||
             ck_event_record(e0, st) ||
             ck_emission_synch(ntracer, n, nfreq, nlogb, logb_min, inv_dlogb, d_tab, d_logt, d_work, d_cre, d_dp, d_b,
-                              d_eps, 1, st) ||
+                              d_eps, mode == CROMA_HIP_EMIS_FP64, st) ||
             ck_event_record(e1, st) || ck_d2h(eps, d_eps, (size_t)ntracer * nfreq * 8, st) || ck_stream_sync(st)) {
             rc = CROMA_HIP_EDEVICE;
             ch_set_error("synch emission: %s", ck_last_error());

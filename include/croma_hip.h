@@ -265,6 +265,15 @@ int croma_hip_synch_kernel_table(const CromaHipContext *ctx, int nfreq, int nthe
 int croma_hip_emission_synch(const CromaHipContext *ctx, int ntracer, int nfreq, int nlogb,
                              double logb_min, double inv_dlogb, const double *table, const double *cre,
                              const double *b_field, double *eps, double *elapsed_ms);
+/* The same with a precision mode: CROMA_HIP_EMIS_FP64 (as above) or
+ * CROMA_HIP_EMIS_FP32 (terms as 2^(log2|N dp| + log2 K) summed in FP32
+ * relative to a running largest exponent, float log2 table, hardware exp2;
+ * emissivities within 1e-4).  Replaces
+ * prepare_synch_emission_cuda_batch (fp_cuda_backend.h:97-100) in its FP32
+ * form. */
+int croma_hip_emission_synch_mode(const CromaHipContext *ctx, int ntracer, int nfreq, int nlogb,
+                                  double logb_min, double inv_dlogb, const double *table, const double *cre,
+                                  const double *b_field, double *eps, int mode, double *elapsed_ms);
 
 /* ---- output and checkpoint files (tracer_fp_output.c, tracer_fp_restart.c)
  * The reference's byte layouts, float64 native-endian:

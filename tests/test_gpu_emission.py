@@ -82,7 +82,11 @@ def test_hadronic_matches_oracle(ctx128, mode):
     assert err < TOL[mode]
 
 
-def test_synch_matches_oracle(ctx128):
+@pytest.mark.parametrize("mode", [ch.EMIS_FP64, ch.EMIS_FP32])
+def test_synch_matches_oracle(ctx128, mode):
+    """FP64 and FP32 modes against the oracle on the compiled reference's
+    table; the FP32 mode also on spectra spanning 1e-300..1e+280 in scale
+    (per-tracer power-of-two scaling keeps them inside float's range)."""
     d = np.load(os.path.join(GOLD, "ref_n128.npz"))
     nfreq, _, nlogb = (int(v) for v in d["sy_dims"])
     inv_dlogb = float(d["sy_inv_dlogb"][0])
@@ -91,12 +95,42 @@ def test_synch_matches_oracle(ctx128):
     nt = 130
     cre = (o.ep[None, :] ** -2.4) * np.exp(-o.ep[None, :] / 10 ** rng.uniform(3, 5, nt)[:, None])
     cre[5, 70:] = 0.0  # truncated spectrum: the sum stops at the first empty successor
+    cre[7] = 0.0  # an empty spectrum
+    cre[9] *= 1e-300
+    cre[11] *= 1e280
     b = 10 ** rng.uniform(-8.5, -4.5, nt)
     ref = po.emission_synch(128, nfreq, nlogb, -6.0, inv_dlogb, d["sy_table"], cre.ravel(), o.edp, b)
-    eps, _ = ctx128.emission_synch(cre, b, d["sy_table"], nfreq, nlogb, -6.0, inv_dlogb)
+    eps, ms = ctx128.emission_synch(cre, b, d["sy_table"], nfreq, nlogb, -6.0, inv_dlogb, mode=mode)
+    assert np.all(eps[7] == 0.0)
     err = rel(eps, ref.reshape(eps.shape))
-    print(f"[emission] synch max rel err {err:.3e}", flush=True)
-    assert err < 1e-12
+    print(f"[emission] synch mode={mode} max rel err {err:.3e} ({ms:.3f} ms)", flush=True)
+    assert err < TOL[mode]
+
+
+@pytest.mark.parametrize("n", [64, 256, 512])
+def test_synch_fp32_matches_fp64_on_device_tables(n):
+    """FP32 against FP64 at every spectrum length, on a device-built table
+    (32 log-B planes x 64 frequencies) and 2000 tracers: 1e-4."""
+    fx = np.load(os.path.join(GOLD, "fx_table.npz"))
+    o = po.Oracle(n)
+    logy, pw = po.synch_logy(o, 64, 7.0, 11.0, 16)
+    rng = np.random.default_rng(17)
+    nt = 2000
+    q = rng.uniform(2.0, 3.5, nt)
+    cre = (o.ep[None, :] ** -q[:, None]) * np.exp(-o.ep[None, :] / 10 ** rng.uniform(2, 6, nt)[:, None])
+    cre *= 10 ** rng.uniform(-5, 5, nt)[:, None]
+    b = 10 ** rng.uniform(-8.0, -5.0, nt)
+    ctx = ch.Context(n, ch.default_params())
+    try:
+        tab, _ = ctx.synch_kernel_table(64, 16, 32, -2.0, 8.0, fx["logx"], fx["logfx"], fx["fx"],
+                                        float(fx["xmin"][0]), logy, pw)
+        e64, _ = ctx.emission_synch(cre, b, tab, 64, 32, -2.0, 8.0, mode=ch.EMIS_FP64)
+        e32, ms = ctx.emission_synch(cre, b, tab, 64, 32, -2.0, 8.0, mode=ch.EMIS_FP32)
+    finally:
+        ctx.close()
+    err = rel(e32, e64)
+    print(f"[emission] synch n={n} FP32 vs FP64 max rel err {err:.3e} ({ms:.3f} ms)", flush=True)
+    assert err < 1e-4
 
 
 @pytest.mark.parametrize("n", [64, 256, 512])
