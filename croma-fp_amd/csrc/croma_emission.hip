@@ -176,58 +176,99 @@ __global__ __launch_bounds__(256) void k_emis_stream(int ntr, int nbins, int ld_
     __syncthreads();
     const int r = lane & 15, q = lane >> 4;
     const int ntiles = (ntr + 15) / 16;
+    // the row streams in chunks of KC x 32 bins, the next chunk's loads in
+    // flight while the current one converts and multiplies (registers for two
+    // chunks instead of the whole row: more waves per SIMD)
+    constexpr int KC = KS >= 4 ? 4 : (KS >= 2 ? 2 : 1);
+    constexpr int NCH = KS / KC;
+    constexpr int NONE = -100000;
     for (int tile = blockIdx.x * 4 + w; tile < ntiles; tile += gridDim.x * 4) {
         const int i = tile * 16 + r;
         const bool live = i < ntr;
         const double *row = spectra + (size_t)(live ? i : 0) * K + 2 * q;
-        double2 a[KS][4];
-#pragma unroll
-        for (int ks = 0; ks < KS; ++ks)
-#pragma unroll
-            for (int m = 0; m < 4; ++m)
-                a[ks][m] = live ? *reinterpret_cast<const double2 *>(row + 32 * ks + 8 * m) : make_double2(0.0, 0.0);
-        // largest binary exponent of the row (frexp convention): the four
-        // lanes of row r are r, r + 16, r + 32, r + 48
-        int e = -100000;
-#pragma unroll
-        for (int ks = 0; ks < KS; ++ks)
-#pragma unroll
-            for (int m = 0; m < 4; ++m) {
-                const double v0 = fabs(a[ks][m].x), v1 = fabs(a[ks][m].y);
-                if (v0 > 0.0 && isfinite(v0)) e = max(e, __builtin_amdgcn_frexp_exp(v0));
-                if (v1 > 0.0 && isfinite(v1)) e = max(e, __builtin_amdgcn_frexp_exp(v1));
-            }
-        e = max(e, __shfl_xor(e, 16, 64));
-        e = max(e, __shfl_xor(e, 32, 64));
-        const int sh = e == -100000 ? 0 : e;
         floatx4 acc[4];
 #pragma unroll
         for (int c = 0; c < 4; ++c) acc[c] = floatx4{0.f, 0.f, 0.f, 0.f};
+        // each row's sum is kept scaled by 2^-s_row, s_row the largest binary
+        // exponent (frexp convention) of the row's entries seen so far; the
+        // four lanes of row r are r, r + 16, r + 32, r + 48
+        int s_row = NONE;
+        double2 nxt[KC][4];
 #pragma unroll
-        for (int ks = 0; ks < KS; ++ks) {
-            bf16x8 ah, al;
+        for (int kk = 0; kk < KC; ++kk)
 #pragma unroll
-            for (int m = 0; m < 4; ++m) {
-                const float f0 = (float)ldexp(a[ks][m].x, -sh), f1 = (float)ldexp(a[ks][m].y, -sh);
-                const uint16_t h0 = em_bf16_bits(f0), h1 = em_bf16_bits(f1);
-                ah[2 * m] = __builtin_bit_cast(__bf16, h0);
-                ah[2 * m + 1] = __builtin_bit_cast(__bf16, h1);
-                if (SPLIT) {
-                    al[2 * m] = __builtin_bit_cast(__bf16, em_bf16_bits(f0 - em_bf16_val(h0)));
-                    al[2 * m + 1] = __builtin_bit_cast(__bf16, em_bf16_bits(f1 - em_bf16_val(h1)));
+            for (int m = 0; m < 4; ++m)
+                nxt[kk][m] = live ? *reinterpret_cast<const double2 *>(row + 32 * kk + 8 * m) : make_double2(0.0, 0.0);
+#pragma unroll 1
+        for (int ch = 0; ch < NCH; ++ch) {
+            double2 a[KC][4];
+#pragma unroll
+            for (int kk = 0; kk < KC; ++kk)
+#pragma unroll
+                for (int m = 0; m < 4; ++m) a[kk][m] = nxt[kk][m];
+            if (ch + 1 < NCH) {
+#pragma unroll
+                for (int kk = 0; kk < KC; ++kk)
+#pragma unroll
+                    for (int m = 0; m < 4; ++m)
+                        nxt[kk][m] = live ? *reinterpret_cast<const double2 *>(row + 32 * (KC * (ch + 1) + kk) + 8 * m)
+                                          : make_double2(0.0, 0.0);
+            }
+            int e = NONE;
+#pragma unroll
+            for (int kk = 0; kk < KC; ++kk)
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    const double v0 = fabs(a[kk][m].x), v1 = fabs(a[kk][m].y);
+                    if (v0 > 0.0 && isfinite(v0)) e = max(e, __builtin_amdgcn_frexp_exp(v0));
+                    if (v1 > 0.0 && isfinite(v1)) e = max(e, __builtin_amdgcn_frexp_exp(v1));
+                }
+            e = max(e, __shfl_xor(e, 16, 64));
+            e = max(e, __shfl_xor(e, 32, 64));
+            const int s_new = max(s_row, e);
+            if (ch > 0) {
+                // rows whose scale rose: the partial sums move to the new scale
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int rr = 4 * q + j;
+                    const int so = __shfl(s_row, rr, 64), sn = __shfl(s_new, rr, 64);
+                    if (so != sn && so != NONE) {
+                        const float f = ldexpf(1.0f, max(so - sn, -160));
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) acc[c][j] *= f;
+                    }
                 }
             }
+            s_row = s_new;
+            const int sh = s_row == NONE ? 0 : s_row;
 #pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                const bf16x8 bh = *reinterpret_cast<const bf16x8 *>(&b_hi[ks][q][16 * c + r][0]);
-                acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[c], 0, 0, 0);
-                if (SPLIT) {
-                    const bf16x8 bl = *reinterpret_cast<const bf16x8 *>(&b_lo[ks][q][16 * c + r][0]);
-                    acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc[c], 0, 0, 0);
-                    acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[c], 0, 0, 0);
+            for (int kk = 0; kk < KC; ++kk) {
+                const int ks = KC * ch + kk;
+                bf16x8 ah, al;
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    const float f0 = (float)ldexp(a[kk][m].x, -sh), f1 = (float)ldexp(a[kk][m].y, -sh);
+                    const uint16_t h0 = em_bf16_bits(f0), h1 = em_bf16_bits(f1);
+                    ah[2 * m] = __builtin_bit_cast(__bf16, h0);
+                    ah[2 * m + 1] = __builtin_bit_cast(__bf16, h1);
+                    if (SPLIT) {
+                        al[2 * m] = __builtin_bit_cast(__bf16, em_bf16_bits(f0 - em_bf16_val(h0)));
+                        al[2 * m + 1] = __builtin_bit_cast(__bf16, em_bf16_bits(f1 - em_bf16_val(h1)));
+                    }
+                }
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const bf16x8 bh = *reinterpret_cast<const bf16x8 *>(&b_hi[ks][q][16 * c + r][0]);
+                    acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[c], 0, 0, 0);
+                    if (SPLIT) {
+                        const bf16x8 bl = *reinterpret_cast<const bf16x8 *>(&b_lo[ks][q][16 * c + r][0]);
+                        acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc[c], 0, 0, 0);
+                        acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[c], 0, 0, 0);
+                    }
                 }
             }
         }
+        const int sh = s_row == NONE ? 0 : s_row;
         // D layout: lane holds rows 4 (lane / 16) + j of the tile, column lane % 16
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
