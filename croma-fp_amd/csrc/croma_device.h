@@ -401,10 +401,11 @@ __device__ __forceinline__ double d_lerp(double cur, double nxt, double alpha)
 
 // ---------------------------------------------------------------------------
 // Lane-group cross-lane helpers.  G lanes per group, groups aligned in the
-// wave.  Groups that fit a 16-lane row shift with DPP (row_shr / row_shl, no
-// LDS round trip); wider groups use ds_bpermute.  A lane whose
-// source lies outside its group reads a neighbouring group's (finite) value;
-// every caller multiplies such values by a zero coefficient.
+// wave.  Shifts inside 16-lane rows use DPP (row_shr / row_shl), across rows
+// DPP row_bcast, wave shifts or readlane (the per-segment group sum of the
+// eta cap uses __shfl_xor).  A lane whose
+// source lies outside its group reads a neighbouring group's (finite) value,
+// or 0, and every caller discards it or multiplies it by a zero coefficient.
 // ---------------------------------------------------------------------------
 template <int CTRL>
 __device__ __forceinline__ double dpp64(double v)
@@ -414,46 +415,6 @@ __device__ __forceinline__ double dpp64(double v)
     return __hiloint2double(hi, lo);
 }
 
-// value of lane (l - S) inside the group segment (S a power of two < G)
-template <int G, int S>
-__device__ __forceinline__ double sh_up(double v)
-{
-    if constexpr (G <= 16) return dpp64<0x110 + S>(v);
-    else return __shfl_up(v, (unsigned)S, G);
-}
-
-// value of lane (l + S)
-template <int G, int S>
-__device__ __forceinline__ double sh_down(double v)
-{
-    if constexpr (G <= 16) return dpp64<0x100 + S>(v);
-    else return __shfl_down(v, (unsigned)S, G);
-}
-
-// lane (l - 1) / (l + 1) across row boundaries of the group
-template <int G>
-__device__ __forceinline__ double sh_up1(double v)
-{
-    if constexpr (G <= 16) return dpp64<0x111>(v);
-    else return __shfl_up(v, 1u, G);
-}
-template <int G>
-__device__ __forceinline__ double sh_down1(double v)
-{
-    if constexpr (G <= 16) return dpp64<0x101>(v);
-    else return __shfl_down(v, 1u, G);
-}
-
-template <int G>
-__device__ __forceinline__ double g_up(double v, int s)
-{
-    return __shfl_up(v, (unsigned)s, G);
-}
-template <int G>
-__device__ __forceinline__ double g_down(double v, int s)
-{
-    return __shfl_down(v, (unsigned)s, G);
-}
 template <int G>
 __device__ __forceinline__ double g_sum(double v)
 {
