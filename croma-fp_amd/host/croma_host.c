@@ -176,6 +176,11 @@ struct CromaHipContext {
     int *h_npmin;
     double *d_fqe;
     void *blas; /* vendor GEMM handle for the split secondary contraction */
+    /* the last IC row a window built (device, n doubles) and its (z, lo, hi):
+     * the next window's first snapshot is usually that z */
+    double *d_ic_last;
+    double ic_last_zb[3];
+    int ic_last_valid;
     /* plain-path workspace */
     CkPlainWs pws;
     void *pws_mem;
@@ -392,6 +397,7 @@ void croma_hip_context_destroy(CromaHipContext *c)
     ck_free(c->d_counter);
     ck_free(c->d_err);
     ck_free(c->d_fqe);
+    ck_free(c->d_ic_last);
     ck_free(c->pws_mem);
     ck_blas_destroy(c->blas);
     free(c->h_fqe);
@@ -1177,8 +1183,24 @@ static int upload_window(CromaHipBatch *b, WinBuf *w)
         (host && w->split && ck_h2d(w->d_segoff, w->h_segoff, (size_t)q * ntr * sizeof(int), c->stream)) ||
         ck_event_record(w->ev_up, c->stream))
         return dev_fail("window upload");
-    if (ck_ic_rows(n, c->dgrid.ep, c->g.epm1, c->g.epp1, w->nz, w->d_zb, w->d_ric, 0, c->d_err, c->stream))
-        return dev_fail("IC rows");
+    {
+        /* a row for the same (z, lo, hi) as the previous window's last one is
+         * copied rather than rebuilt (the rows depend on z and the grid only) */
+        const int reuse = c->ic_last_valid && w->nz > 0 && c->ic_last_zb[0] == w->h_zb[0] &&
+                          c->ic_last_zb[1] == w->h_zb[1] && c->ic_last_zb[2] == w->h_zb[2];
+        if (!c->d_ic_last && !(c->d_ic_last = (double *)ck_malloc((size_t)n * sizeof(double))))
+            return dev_fail("IC row cache alloc");
+        if ((reuse && ck_d2d(w->d_ric, c->d_ic_last, (size_t)n * sizeof(double), c->stream)) ||
+            (w->nz > reuse && ck_ic_rows(n, c->dgrid.ep, c->g.epm1, c->g.epp1, w->nz - reuse, w->d_zb + 3 * reuse,
+                                         w->d_ric + (size_t)reuse * n, 0, c->d_err, c->stream)))
+            return dev_fail("IC rows");
+        if (w->nz > 0) {
+            if (ck_d2d(c->d_ic_last, w->d_ric + (size_t)(w->nz - 1) * n, (size_t)n * sizeof(double), c->stream))
+                return dev_fail("IC row cache");
+            memcpy(c->ic_last_zb, w->h_zb + 3 * (w->nz - 1), sizeof(c->ic_last_zb));
+            c->ic_last_valid = 1;
+        }
+    }
     return 0;
 }
 
