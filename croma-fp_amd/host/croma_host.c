@@ -413,6 +413,7 @@ int croma_hip_context_set_params(CromaHipContext *c, const CromaHipParams *prm)
     if (!c || !valid_params(prm)) return CROMA_HIP_EINVAL;
     c->uprm = *prm;
     ch_params_to_ck(prm, &c->prm);
+    c->ic_last_valid = 0;
     /* the D_pp cutoff table depends on E_pmax */
     if (c->prm.epmax != old_epmax &&
         (ck_stream_sync(c->stream) || ck_grid_loss_tables(c->dgrid, c->prm.epmax, c->stream) ||
@@ -751,14 +752,10 @@ typedef struct {
     int *d_nsub, *d_non;
     CkTask *d_tasks;
     CkSlot *d_slot;
-    double *d_ric;    /* [cap + 1][n]: IC cooling rows at the window's redshifts (built on the device) */
-    double *d_zb;     /* [cap + 1][3]: z, CMB e0 bounds */
     int *h_nsub, *h_non;     /* pinned [cap][ntr] */
     CkTask *h_tasks;         /* pinned [ntr] */
     CkSlot *h_slot;          /* pinned [cap] */
-    double *h_zb;            /* pinned [cap + 1][3] */
     int count;               /* snapshots prepared */
-    int nz;                  /* IC rows to build */
     int *d_segoff, *h_segoff; /* split launches: [cap][ntr] first segment row per tracer */
     long segrows;            /* segment rows of the window */
     long segq;               /* segrows rounded up to the batch's GEMM row quantum */
@@ -788,6 +785,10 @@ struct CromaHipBatch {
     void *ev_a, *ev_b;
     void **ev_k; /* per-launch timing events, 2 per window */
     int ev_k_cap;
+    /* IC cooling rows of every redshift a run visits, row q at z[icr_s0 + q],
+     * built on the device in one launch at the start of the run */
+    double *d_icr, *d_icz, *h_icz; /* [icr_cap][n], [icr_cap][3] (z, CMB e0 bounds), pinned copy */
+    int icr_cap, icr_s0;
     /* plain path */
     int *d_idx;
     int *h_idx;
@@ -800,8 +801,6 @@ static void win_free(CromaHipContext *c, WinBuf *w)
     ck_free(w->d_non);
     ck_free(w->d_tasks);
     ck_free(w->d_slot);
-    ck_free(w->d_ric);
-    ck_free(w->d_zb);
     ck_free(w->d_segoff);
     ck_free(w->d_key);
     ck_free(w->d_segloc);
@@ -812,12 +811,11 @@ static void win_free(CromaHipContext *c, WinBuf *w)
     ck_host_free(w->h_non);
     ck_host_free(w->h_tasks);
     ck_host_free(w->h_slot);
-    ck_host_free(w->h_zb);
     ck_event_destroy(w->ev_up);
     memset(w, 0, sizeof(*w));
 }
 
-static int win_reserve(CromaHipContext *c, WinBuf *w, int cap, int ntr, int n)
+static int win_reserve(CromaHipContext *c, WinBuf *w, int cap, int ntr)
 {
     if (w->cap >= cap) return 0;
     if (w->ev_up) ck_event_sync(w->ev_up);
@@ -827,13 +825,10 @@ static int win_reserve(CromaHipContext *c, WinBuf *w, int cap, int ntr, int n)
     w->d_non = (int *)ck_malloc((size_t)cap * ntr * sizeof(int));
     w->d_tasks = (CkTask *)ck_malloc((size_t)ntr * sizeof(CkTask));
     w->d_slot = (CkSlot *)ck_malloc((size_t)cap * sizeof(CkSlot));
-    w->d_ric = (double *)ck_malloc((size_t)(cap + 1) * n * sizeof(double));
-    w->d_zb = (double *)ck_malloc((size_t)(cap + 1) * 3 * sizeof(double));
     w->h_nsub = (int *)ck_host_alloc((size_t)cap * ntr * sizeof(int));
     w->h_non = (int *)ck_host_alloc((size_t)cap * ntr * sizeof(int));
     w->h_tasks = (CkTask *)ck_host_alloc((size_t)ntr * sizeof(CkTask));
     w->h_slot = (CkSlot *)ck_host_alloc((size_t)cap * sizeof(CkSlot));
-    w->h_zb = (double *)ck_host_alloc((size_t)(cap + 1) * 3 * sizeof(double));
     w->d_segoff = (int *)ck_malloc((size_t)cap * ntr * sizeof(int));
     w->h_segoff = (int *)ck_host_alloc((size_t)cap * ntr * sizeof(int));
     w->d_key = (int *)ck_malloc((size_t)ntr * sizeof(int));
@@ -841,8 +836,8 @@ static int win_reserve(CromaHipContext *c, WinBuf *w, int cap, int ntr, int n)
     w->d_blk = (int *)ck_malloc(((size_t)ntr / 256 + 2) * sizeof(int));
     w->d_hist = (int *)ck_malloc((size_t)CK_NSUB_HIST * sizeof(int));
     w->ev_up = ck_event_create();
-    if (!w->d_nsub || !w->d_non || !w->d_tasks || !w->d_slot || !w->d_ric || !w->d_zb || !w->h_nsub ||
-        !w->h_non || !w->h_tasks || !w->h_slot || !w->h_zb || !w->ev_up || !w->d_segoff || !w->h_segoff ||
+    if (!w->d_nsub || !w->d_non || !w->d_tasks || !w->d_slot || !w->h_nsub || !w->h_non ||
+        !w->h_tasks || !w->h_slot || !w->ev_up || !w->d_segoff || !w->h_segoff ||
         !w->d_key || !w->d_segloc || !w->d_blk || !w->d_hist) {
         win_free(c, w);
         return dev_fail("window buffers");
@@ -870,6 +865,9 @@ void croma_hip_batch_destroy(CromaHipBatch *b)
     ck_free(b->d_segrec);
     ck_free(b->d_seginje);
     ck_free(b->d_cells);
+    ck_free(b->d_icr);
+    ck_free(b->d_icz);
+    ck_host_free(b->h_icz);
     free(b->h_bg);
     free(b->src_set);
     free(b->hist);
@@ -915,7 +913,7 @@ CromaHipBatch *croma_hip_batch_create(CromaHipContext *c, int ntr, int species, 
         !b->ev_a || !b->ev_b || !b->d_cells)
         goto fail;
     for (k = 0; k < 2; k++)
-        if (win_reserve(c, &b->win[k], 1, ntr, n)) goto fail;
+        if (win_reserve(c, &b->win[k], 1, ntr)) goto fail;
     if (b->protons && ch_ctx_fqe(c) != 0) goto fail;
     return b;
 fail:
@@ -1075,14 +1073,9 @@ static int prepare_window(CromaHipBatch *b, WinBuf *w, int s_first, int count, i
     w->split = split;
     w->dev_nsub = dev_nsub;
     w->segrows = 0;
-    /* IC rows at z_first .. z_last (+ z_last+1 when the last snapshot has a
-     * successor) are built on the device; the host only evaluates the two
-     * end bins nsub needs (same arithmetic as the reference row). */
-    w->nz = count + ((s_first + count < nsnap_total) ? 1 : 0);
-    for (q = 0; q < w->nz; q++) {
-        w->h_zb[3 * q] = z[s_first + q];
-        ch_ic_bounds(z[s_first + q], &w->h_zb[3 * q + 1], &w->h_zb[3 * q + 2]);
-    }
+    /* the IC rows come from the run's table (run_ic_rows); the host only
+     * evaluates the two end bins nsub needs (same arithmetic as the
+     * reference row) */
     for (q = 0; q < count; q++) {
         const int s = s_first + q;
         const int al = (s + 1 < nsnap_total) ? 1 : 0;
@@ -1126,8 +1119,8 @@ static int prepare_window(CromaHipBatch *b, WinBuf *w, int s_first, int count, i
             S->bgc[f] = b->d_bg + ((size_t)s * CK_NBG + f) * ntr;
             S->bgn[f] = b->d_bg + ((size_t)sn * CK_NBG + f) * ntr;
         }
-        S->ric_c = w->d_ric + (size_t)q * n;
-        S->ric_n = al ? w->d_ric + (size_t)(q + 1) * n : S->ric_c;
+        S->ric_c = b->d_icr + (size_t)(s - b->icr_s0) * n;
+        S->ric_n = al ? S->ric_c + n : S->ric_c;
         S->nsub = w->d_nsub + (size_t)q * ntr;
         S->n_on = w->d_non + (size_t)q * ntr;
         S->fqe_t = b->protons ? c->d_fqe : 0;
@@ -1170,37 +1163,59 @@ static int prepare_window(CromaHipBatch *b, WinBuf *w, int s_first, int count, i
     return 0;
 }
 
+/* IC cooling rows at z[s0] .. z[s1 - 1] (+ z[s1] when snapshot s1 - 1 has a
+ * successor) for the whole run, in one launch: one row alone is a few hundred
+ * blocks that cannot fill the device, so the rows of a run are built
+ * together.  A first row at the (z, lo, hi) of the previous run's last row is
+ * copied rather than rebuilt (the rows depend on z and the grid only). */
+static int run_ic_rows(CromaHipBatch *b, int s0, int s1, int nsnap_total, const double *z)
+{
+    CromaHipContext *c = b->ctx;
+    const int n = c->g.n, nz = s1 - s0 + ((s1 < nsnap_total) ? 1 : 0);
+    int q, reuse;
+    if (nz > b->icr_cap) {
+        ck_stream_sync(c->stream);
+        ck_free(b->d_icr);
+        ck_free(b->d_icz);
+        ck_host_free(b->h_icz);
+        b->d_icr = (double *)ck_malloc((size_t)nz * n * sizeof(double));
+        b->d_icz = (double *)ck_malloc((size_t)nz * 3 * sizeof(double));
+        b->h_icz = (double *)ck_host_alloc((size_t)nz * 3 * sizeof(double));
+        b->icr_cap = (b->d_icr && b->d_icz && b->h_icz) ? nz : 0;
+        if (!b->icr_cap) return dev_fail("IC row table");
+    }
+    if (!c->d_ic_last && !(c->d_ic_last = (double *)ck_malloc((size_t)n * sizeof(double))))
+        return dev_fail("IC row cache");
+    for (q = 0; q < nz; q++) {
+        b->h_icz[3 * q] = z[s0 + q];
+        ch_ic_bounds(z[s0 + q], &b->h_icz[3 * q + 1], &b->h_icz[3 * q + 2]);
+    }
+    reuse = c->ic_last_valid && c->ic_last_zb[0] == b->h_icz[0] && c->ic_last_zb[1] == b->h_icz[1] &&
+            c->ic_last_zb[2] == b->h_icz[2];
+    if (ck_h2d(b->d_icz, b->h_icz, (size_t)nz * 3 * sizeof(double), c->stream) ||
+        (reuse && ck_d2d(b->d_icr, c->d_ic_last, (size_t)n * sizeof(double), c->stream)) ||
+        (nz > reuse && ck_ic_rows(n, c->dgrid.ep, c->g.epm1, c->g.epp1, nz - reuse, b->d_icz + 3 * reuse,
+                                  b->d_icr + (size_t)reuse * n, 0, c->d_err, c->stream)) ||
+        ck_d2d(c->d_ic_last, b->d_icr + (size_t)(nz - 1) * n, (size_t)n * sizeof(double), c->stream))
+        return dev_fail("IC rows");
+    memcpy(c->ic_last_zb, b->h_icz + 3 * (nz - 1), sizeof(c->ic_last_zb));
+    c->ic_last_valid = 1;
+    b->icr_s0 = s0;
+    return 0;
+}
+
 static int upload_window(CromaHipBatch *b, WinBuf *w)
 {
     CromaHipContext *c = b->ctx;
-    const int ntr = b->ntr, n = c->g.n, q = w->count;
+    const int ntr = b->ntr, q = w->count;
     const int host = !w->dev_nsub;
     if ((host && ck_h2d(w->d_nsub, w->h_nsub, (size_t)q * ntr * sizeof(int), c->stream)) ||
         (host && ck_h2d(w->d_non, w->h_non, (size_t)q * ntr * sizeof(int), c->stream)) ||
         (host && ck_h2d(w->d_tasks, w->h_tasks, (size_t)ntr * sizeof(CkTask), c->stream)) ||
         ck_h2d(w->d_slot, w->h_slot, (size_t)q * sizeof(CkSlot), c->stream) ||
-        ck_h2d(w->d_zb, w->h_zb, (size_t)w->nz * 3 * sizeof(double), c->stream) ||
         (host && w->split && ck_h2d(w->d_segoff, w->h_segoff, (size_t)q * ntr * sizeof(int), c->stream)) ||
         ck_event_record(w->ev_up, c->stream))
         return dev_fail("window upload");
-    {
-        /* a row for the same (z, lo, hi) as the previous window's last one is
-         * copied rather than rebuilt (the rows depend on z and the grid only) */
-        const int reuse = c->ic_last_valid && w->nz > 0 && c->ic_last_zb[0] == w->h_zb[0] &&
-                          c->ic_last_zb[1] == w->h_zb[1] && c->ic_last_zb[2] == w->h_zb[2];
-        if (!c->d_ic_last && !(c->d_ic_last = (double *)ck_malloc((size_t)n * sizeof(double))))
-            return dev_fail("IC row cache alloc");
-        if ((reuse && ck_d2d(w->d_ric, c->d_ic_last, (size_t)n * sizeof(double), c->stream)) ||
-            (w->nz > reuse && ck_ic_rows(n, c->dgrid.ep, c->g.epm1, c->g.epp1, w->nz - reuse, w->d_zb + 3 * reuse,
-                                         w->d_ric + (size_t)reuse * n, 0, c->d_err, c->stream)))
-            return dev_fail("IC rows");
-        if (w->nz > 0) {
-            if (ck_d2d(c->d_ic_last, w->d_ric + (size_t)(w->nz - 1) * n, (size_t)n * sizeof(double), c->stream))
-                return dev_fail("IC row cache");
-            memcpy(c->ic_last_zb, w->h_zb + 3 * (w->nz - 1), sizeof(c->ic_last_zb));
-            c->ic_last_valid = 1;
-        }
-    }
     return 0;
 }
 
@@ -1529,7 +1544,7 @@ int croma_hip_batch_run(CromaHipBatch *b, int s0, int s1, int nsnap_total, const
     }
     nwin = (s1 - s0 + wsz - 1) / wsz;
     for (k = 0; k < 2; k++)
-        if ((rc = win_reserve(c, &b->win[k], wsz, b->ntr, c->g.n)) != 0) return rc;
+        if ((rc = win_reserve(c, &b->win[k], wsz, b->ntr)) != 0) return rc;
     if (path == CROMA_HIP_PATH_FUSED && b->ev_k_cap < 2 * nwin) {
         int need = 2 * nwin, j;
         void **ne = (void **)realloc(b->ev_k, (size_t)need * sizeof(void *));
@@ -1549,6 +1564,7 @@ int croma_hip_batch_run(CromaHipBatch *b, int s0, int s1, int nsnap_total, const
     t_start = ch_now_s();
     if (ck_event_record(b->ev_a, c->stream)) return dev_fail("event");
     if (dev_nsub && ck_fill_i32((int *)b->d_cells, 0, 2, c->stream)) return dev_fail("cell counter");
+    if ((rc = run_ic_rows(b, s0, s1, nsnap_total, z)) != 0) return rc;
     rc = prepare_window(b, &b->win[0], s0, (s1 - s0 < wsz) ? s1 - s0 : wsz, nsnap_total, z, dt_snap, el, split,
                         dev_nsub, &ps);
     if (rc) return rc;
