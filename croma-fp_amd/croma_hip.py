@@ -188,6 +188,10 @@ class Context:
         """True: repeated snapshot windows replay a captured hipGraph of their launches."""
         _chk(lib().croma_hip_context_set_graphs(C.c_void_p(self.h), int(bool(enable))), "set_graphs")
 
+    def set_pipeline(self, enable):
+        """True: split windows run the electron pass beside the next window's proton pass."""
+        _chk(lib().croma_hip_context_set_pipeline(C.c_void_p(self.h), int(bool(enable))), "set_pipeline")
+
     def get_tile(self, protons=False):
         t = Tile()
         _chk(lib().croma_hip_context_get_tile(C.c_void_p(self.h), int(protons), C.byref(t)), "get_tile")

@@ -153,6 +153,7 @@ struct CromaHipContext {
     CkGrid dgrid;
     double *d_grid_mem;
     void *stream;
+    void *stream_e; /* split windows: the electron pass, beside the next window's proton pass */
     int *d_counter;
     int *d_err;
     CromaHipTile tile[2]; /* [protons] */
@@ -187,6 +188,7 @@ struct CromaHipContext {
     int pws_cap;
     /* window launches replayed as hipGraphs */
     int use_graphs;
+    int pipeline; /* split windows: electron pass beside the next proton pass */
     /* batch runs estimate nsub and order the queue on the device */
     int nsub_device;
     void **retired; /* graph executables replaced while their last launch may still run */
@@ -354,6 +356,7 @@ CromaHipContext *croma_hip_context_create(const CromaHipGridSpec *gs, const Crom
     }
     default_tiles(c);
     c->use_graphs = 1;
+    c->pipeline = 0;
     c->nsub_device = 1;
     return c;
 }
@@ -402,6 +405,7 @@ void croma_hip_context_destroy(CromaHipContext *c)
     ck_blas_destroy(c->blas);
     free(c->h_fqe);
     free(c->h_npmin);
+    if (c->stream_e) ck_stream_destroy(c->stream_e);
     if (c->stream) ck_stream_destroy(c->stream);
     pthread_mutex_destroy(&c->ic_lock);
     free(c);
@@ -443,6 +447,15 @@ int croma_hip_context_set_nsub_device(CromaHipContext *c, int enable)
     if (!c || (enable != 0 && enable != 1)) return CROMA_HIP_EINVAL;
     ck_stream_sync(c->stream);
     c->nsub_device = enable;
+    return 0;
+}
+
+int croma_hip_context_set_pipeline(CromaHipContext *c, int enable)
+{
+    if (!c || (enable != 0 && enable != 1)) return CROMA_HIP_EINVAL;
+    ck_stream_sync(c->stream);
+    if (c->stream_e) ck_stream_sync(c->stream_e);
+    c->pipeline = enable;
     return 0;
 }
 
@@ -698,8 +711,8 @@ typedef struct {
 
 typedef int (*ch_issue_fn)(void *arg);
 
-static int graph_issue(CromaHipContext *c, ChGraph *g, const long key[CH_GKEY], ch_issue_fn fn, void *arg,
-                       GraphStats *gs)
+static int graph_issue(CromaHipContext *c, void *stream, ChGraph *g, const long key[CH_GKEY], ch_issue_fn fn,
+                       void *arg, GraphStats *gs)
 {
     void *ex;
     int rc;
@@ -714,13 +727,13 @@ static int graph_issue(CromaHipContext *c, ChGraph *g, const long key[CH_GKEY], 
     }
     if (g->state == -1) return fn(arg);
     if (g->state == 1) {
-        if (ck_graph_begin(c->stream)) {
+        if (ck_graph_begin(stream)) {
             g->state = -1;
             gs->failures++;
             return fn(arg);
         }
         rc = fn(arg);
-        ex = ck_graph_end(c->stream);
+        ex = ck_graph_end(stream);
         if (rc || !ex) {
             /* nothing captured ran: issue the window as plain launches */
             ck_graph_destroy(ex);
@@ -732,7 +745,7 @@ static int graph_issue(CromaHipContext *c, ChGraph *g, const long key[CH_GKEY], 
         g->state = 2;
         gs->captures++;
     }
-    if (ck_graph_launch(g->exec, c->stream)) return dev_fail("graph launch");
+    if (ck_graph_launch(g->exec, stream)) return dev_fail("graph launch");
     gs->launches++;
     return 0;
 }
@@ -762,6 +775,10 @@ typedef struct {
     int split;               /* the window runs as proton pass + GEMM + electron pass */
     void *ev_up;
     ChGraph graph;           /* launches of the last window shape run from this buffer set */
+    /* pipelined split windows: the GEMM and the electron pass have graphs
+     * of their own (they are issued after waits on the other stream) */
+    ChGraph graph_g, graph_e;
+    void *ev_g, *ev_e;       /* GEMM done (library stream), electron pass done (electron stream) */
     int dev_nsub;            /* nsub, n_on, order and segment rows are built on the device */
     int *d_key, *d_segloc, *d_blk, *d_hist; /* their work buffers */
 } WinBuf;
@@ -797,6 +814,8 @@ struct CromaHipBatch {
 static void win_free(CromaHipContext *c, WinBuf *w)
 {
     graph_reset(c, &w->graph);
+    graph_reset(c, &w->graph_g);
+    graph_reset(c, &w->graph_e);
     ck_free(w->d_nsub);
     ck_free(w->d_non);
     ck_free(w->d_tasks);
@@ -812,6 +831,8 @@ static void win_free(CromaHipContext *c, WinBuf *w)
     ck_host_free(w->h_tasks);
     ck_host_free(w->h_slot);
     ck_event_destroy(w->ev_up);
+    ck_event_destroy(w->ev_g);
+    ck_event_destroy(w->ev_e);
     memset(w, 0, sizeof(*w));
 }
 
@@ -820,6 +841,7 @@ static int win_reserve(CromaHipContext *c, WinBuf *w, int cap, int ntr)
     if (w->cap >= cap) return 0;
     if (w->ev_up) ck_event_sync(w->ev_up);
     ck_stream_sync(c->stream);
+    if (c->stream_e) ck_stream_sync(c->stream_e);
     win_free(c, w);
     w->d_nsub = (int *)ck_malloc((size_t)cap * ntr * sizeof(int));
     w->d_non = (int *)ck_malloc((size_t)cap * ntr * sizeof(int));
@@ -836,8 +858,10 @@ static int win_reserve(CromaHipContext *c, WinBuf *w, int cap, int ntr)
     w->d_blk = (int *)ck_malloc(((size_t)ntr / 256 + 2) * sizeof(int));
     w->d_hist = (int *)ck_malloc((size_t)CK_NSUB_HIST * sizeof(int));
     w->ev_up = ck_event_create();
+    w->ev_g = ck_event_create();
+    w->ev_e = ck_event_create();
     if (!w->d_nsub || !w->d_non || !w->d_tasks || !w->d_slot || !w->h_nsub || !w->h_non ||
-        !w->h_tasks || !w->h_slot || !w->ev_up || !w->d_segoff || !w->h_segoff ||
+        !w->h_tasks || !w->h_slot || !w->ev_up || !w->ev_g || !w->ev_e || !w->d_segoff || !w->h_segoff ||
         !w->d_key || !w->d_segloc || !w->d_blk || !w->d_hist) {
         win_free(c, w);
         return dev_fail("window buffers");
@@ -851,6 +875,7 @@ void croma_hip_batch_destroy(CromaHipBatch *b)
     int k;
     if (!b) return;
     if (b->ctx && b->ctx->stream) ck_stream_sync(b->ctx->stream);
+    if (b->ctx && b->ctx->stream_e) ck_stream_sync(b->ctx->stream_e);
     for (k = 0; k < 2; k++) win_free(b->ctx, &b->win[k]);
     if (b->ctx) flush_retired(b->ctx);
     ck_free(b->d_crp);
@@ -1219,32 +1244,39 @@ static int upload_window(CromaHipBatch *b, WinBuf *w)
     return 0;
 }
 
+/* A fused pass fills the device (cu_count * waves_per_simd blocks of four
+ * waves), or half of it when it runs beside the other species' pass of the
+ * neighbouring window; the two passes then take their tasks from separate
+ * queue counters. */
 static int launch_mode(CromaHipContext *c, int protons, int mode, const CkSlot *d_slot, const CkTask *d_tasks,
-                       int ntasks)
+                       int ntasks, int shared, int *counter, void *stream)
 {
     const CromaHipTile *t = &c->tile[protons ? 1 : 0];
-    const int nblocks = c->cu_count * t->waves_per_simd;
-    if (ck_fill_i32(c->d_counter, 0, 1, c->stream)) return dev_fail("counter reset");
+    const int nfull = c->cu_count * t->waves_per_simd;
+    const int nblocks = shared ? (nfull + 1) / 2 : nfull;
+    if (ck_fill_i32(counter, 0, 1, stream)) return dev_fail("counter reset");
     if (ck_launch_fused(c->g.n, t->rows_per_lane, t->waves_per_simd, nblocks, mode, c->dgrid, d_slot, d_tasks,
-                        ntasks, c->d_counter, c->d_err, c->stream))
+                        ntasks, counter, c->d_err, stream))
         return dev_fail("fused launch");
     return 0;
 }
 
 static int launch_tasks(CromaHipContext *c, int protons, const CkSlot *d_slot, const CkTask *d_tasks, int ntasks)
 {
-    return launch_mode(c, protons, protons ? 1 : 0, d_slot, d_tasks, ntasks);
+    return launch_mode(c, protons, protons ? 1 : 0, d_slot, d_tasks, ntasks, 0, c->d_counter, c->stream);
 }
+
+/* queue counter of the electron pass of a pipelined split window (its own
+ * 128-byte line of the counter block) */
+#define CH_CTR_E 32
 
 /* Split two-species window: proton pass recording the segment-start rows,
  * one GEMM of all rows against the secondary weights (rows jp0..n-1), then
  * the electron pass. */
-static int launch_split(CromaHipBatch *b, WinBuf *w)
+static int launch_gemm(CromaHipBatch *b, WinBuf *w)
 {
     CromaHipContext *c = b->ctx;
     const int n = c->g.n, jp0 = c->jp0;
-    int rc = launch_mode(c, 1, 2, w->d_slot, w->d_tasks, b->ntr);
-    if (rc) return rc;
     /* rows segrows..segq-1 (the rounding to the row quantum, which keeps the
      * GEMM shape and so the launch graph fixed while segrows drifts) hold
      * stale rows whose products are never read */
@@ -1255,12 +1287,23 @@ static int launch_split(CromaHipBatch *b, WinBuf *w)
         return dev_fail("secondary GEMM");
     if (w->segq > 0 && jp0 >= n && ck_fill_f64(b->d_seginje, 0.0, (size_t)w->segq * n, c->stream))
         return dev_fail("secondary clear");
-    return launch_mode(c, 1, 3, w->d_slot, w->d_tasks, b->ntr);
+    return 0;
+}
+
+static int launch_split(CromaHipBatch *b, WinBuf *w)
+{
+    CromaHipContext *c = b->ctx;
+    int rc = launch_mode(c, 1, 2, w->d_slot, w->d_tasks, b->ntr, 0, c->d_counter, c->stream);
+    if (!rc) rc = launch_gemm(b, w);
+    if (!rc) rc = launch_mode(c, 1, 3, w->d_slot, w->d_tasks, b->ntr, 0, c->d_counter, c->stream);
+    return rc;
 }
 
 typedef struct {
     CromaHipBatch *b;
     WinBuf *w;
+    int pipe;   /* pipelined split window: this issue is the proton pass only */
+    int shared; /* the pass runs beside the other species' pass */
 } WinIssue;
 
 static int issue_window(void *arg)
@@ -1268,21 +1311,39 @@ static int issue_window(void *arg)
     WinIssue *wi = (WinIssue *)arg;
     CromaHipBatch *b = wi->b;
     WinBuf *w = wi->w;
-    if (w->dev_nsub && ck_nsub_order(b->ctx->dgrid, w->d_slot, w->count, b->ntr, b->protons, w->split, w->d_tasks,
-                                     w->d_key, w->d_segloc, w->d_blk, w->d_hist, b->d_cells, b->ctx->stream))
+    CromaHipContext *c = b->ctx;
+    if (w->dev_nsub && ck_nsub_order(c->dgrid, w->d_slot, w->count, b->ntr, b->protons, w->split, w->d_tasks,
+                                     w->d_key, w->d_segloc, w->d_blk, w->d_hist, b->d_cells, c->stream))
         return dev_fail("device nsub");
-    if (wi->w->split) return launch_split(b, wi->w);
-    return launch_tasks(b->ctx, b->protons, wi->w->d_slot, wi->w->d_tasks, b->ntr);
+    if (wi->pipe) return launch_mode(c, 1, 2, w->d_slot, w->d_tasks, b->ntr, wi->shared, c->d_counter, c->stream);
+    if (w->split) return launch_split(b, w);
+    return launch_tasks(c, b->protons, w->d_slot, w->d_tasks, b->ntr);
 }
 
-static int launch_window(CromaHipBatch *b, WinBuf *w, GraphStats *gs)
+static int issue_gemm(void *arg)
+{
+    WinIssue *wi = (WinIssue *)arg;
+    return launch_gemm(wi->b, wi->w);
+}
+
+static int issue_epass(void *arg)
+{
+    WinIssue *wi = (WinIssue *)arg;
+    CromaHipContext *c = wi->b->ctx;
+    return launch_mode(c, 1, 3, wi->w->d_slot, wi->w->d_tasks, wi->b->ntr, wi->shared, c->d_counter + CH_CTR_E,
+                       c->stream_e);
+}
+
+static int launch_window(CromaHipBatch *b, WinBuf *w, WinBuf *prev, int last, GraphStats *gs)
 {
     CromaHipContext *c = b->ctx;
     const CromaHipTile *t = &c->tile[b->protons ? 1 : 0];
+    const int pipe = w->split && c->pipeline;
     WinIssue wi;
     long key[CH_GKEY];
+    int rc;
     memset(key, 0, sizeof(key));
-    key[0] = w->split;
+    key[0] = w->split | (pipe << 1) | ((pipe && prev) << 2);
     key[1] = w->segq;
     key[2] = t->rows_per_lane;
     key[3] = t->waves_per_simd;
@@ -1292,7 +1353,31 @@ static int launch_window(CromaHipBatch *b, WinBuf *w, GraphStats *gs)
     key[7] = b->ntr;
     wi.b = b;
     wi.w = w;
-    return graph_issue(c, &w->graph, key, issue_window, &wi, gs);
+    wi.pipe = pipe;
+    wi.shared = prev != 0;
+    if (!pipe) return graph_issue(c, c->stream, &w->graph, key, issue_window, &wi, gs);
+    /* Pipelined split window w on the library stream L and the electron
+     * stream E:
+     *   L: nsub(w), proton pass(w)    beside E: electron pass(w-1)
+     *   L: wait E done(w-1), GEMM(w), record G(w)
+     *   E: wait G(w), electron pass(w), record E done(w)
+     * The proton pass of w+1 writes the segment rows only after GEMM(w) read
+     * them (both on L); GEMM(w+1) overwrites the GEMM rows only after the
+     * electron pass of w read them (the wait on E); the upload of w+1 into
+     * the buffer set of w-1 follows the wait on its electron pass.  The last
+     * window's electron pass runs alone and L waits for it. */
+    if (!c->stream_e && !(c->stream_e = ck_stream_create())) return dev_fail("electron stream");
+    if ((rc = graph_issue(c, c->stream, &w->graph, key, issue_window, &wi, gs)) != 0) return rc;
+    if (prev && ck_stream_wait_event(c->stream, prev->ev_e)) return dev_fail("stream wait");
+    if ((rc = graph_issue(c, c->stream, &w->graph_g, key, issue_gemm, &wi, gs)) != 0) return rc;
+    if (ck_event_record(w->ev_g, c->stream) || ck_stream_wait_event(c->stream_e, w->ev_g))
+        return dev_fail("stream wait");
+    wi.shared = !last;
+    key[0] |= (long)wi.shared << 3;
+    if ((rc = graph_issue(c, c->stream_e, &w->graph_e, key, issue_epass, &wi, gs)) != 0) return rc;
+    if (ck_event_record(w->ev_e, c->stream_e) || (last && ck_stream_wait_event(c->stream, w->ev_e)))
+        return dev_fail("stream wait");
+    return 0;
 }
 
 /* Plain path: reference GPU semantics (per bucket of equal (nsub, n_on),
@@ -1574,7 +1659,7 @@ int croma_hip_batch_run(CromaHipBatch *b, int s0, int s1, int nsnap_total, const
         if ((rc = upload_window(b, w)) != 0) return rc;
         if (path == CROMA_HIP_PATH_FUSED) {
             if (ck_event_record(b->ev_k[2 * iw], c->stream)) return dev_fail("event");
-            rc = launch_window(b, w, &gst);
+            rc = launch_window(b, w, iw ? &b->win[(iw + 1) & 1] : 0, s + cnt >= s1, &gst);
             if (!rc && ck_event_record(b->ev_k[2 * iw + 1], c->stream)) return dev_fail("event");
         } else {
             rc = run_plain_snapshot(b, w);
@@ -1808,7 +1893,7 @@ int croma_hip_execute_snapshot(
             xi.d_slot = d_slot;
             xi.d_tasks = d_tasks;
             xi.ntr = ntr;
-            if ((rc = graph_issue(c, &c->x_graph, key, issue_tasks, &xi, &gst)) != 0) goto done;
+            if ((rc = graph_issue(c, c->stream, &c->x_graph, key, issue_tasks, &xi, &gst)) != 0) goto done;
             if (times) {
                 times->graph_launches += gst.launches;
                 times->graph_captures += gst.captures;

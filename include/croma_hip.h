@@ -130,6 +130,13 @@ int croma_hip_context_get_tile(const CromaHipContext *ctx, int protons, CromaHip
  * (fill, fused kernel, GEMM, fused kernel); 0: plain stream launches. No
  * reference counterpart (the reference launches per bucket and segment). */
 int croma_hip_context_set_graphs(CromaHipContext *ctx, int enable);
+/* 1: in batch runs with the split secondary GEMM, the electron pass of each
+ * snapshot window runs on a second stream beside the proton pass of the next
+ * window, each on half the device; 0 (default): the passes of a window run
+ * in turn on the full device.  The spectra are the same either way; which is
+ * faster depends on the tile.  No reference counterpart (the reference runs
+ * one species pass at a time). */
+int croma_hip_context_set_pipeline(CromaHipContext *ctx, int enable);
 /* 1 (default): batch runs estimate nsub (tracer_fp_nsub.c:94-282), order the
  * queue by cost and assign the split segment rows on the device, without a
  * host pass over the tracers; 0: on the host (the same integers). */

@@ -4,7 +4,10 @@
    hipGraph.  The same windows run with graphs on and off must give
    bit-identical spectra, on every replay, with host copies and
    synchronisations of the program's own between the replays.
-2. croma_hip_execute_snapshot (the parameter list of
+2. Pipelined split windows (the electron pass of a window beside the next
+   window's proton pass, on a second stream) give the same spectra as the
+   passes run in turn, graphs on and off, over runs of several windows.
+3. croma_hip_execute_snapshot (the parameter list of
    tracer_fp_batch_execute_parallel, tracer_fp_batch.c:829-876) called per
    snapshot through the C ABI, as the reference driver calls it
    (include/tracer_fp_solve.inc:631-673), against the oracle's same entry
@@ -36,6 +39,33 @@ def _batch(ctx, w):
         if s in w.qpi or s in w.qepri:
             b.set_sources(s, w.qpi.get(s), w.qepri.get(s))
     return b
+
+
+@pytest.mark.parametrize("graphs", [True, False])
+def test_pipelined_split_windows_equal_serial(graphs):
+    n, nt, ns = 128, 96, 8
+    w = small_workload(n, nt, ns, 3, 1, dsa_frac=0.3, seed=29)
+    prm = hip_params(w)
+    cp, cs = ch.Context(n, prm), ch.Context(n, prm)
+    cp.set_pipeline(True)
+    for c in (cp, cs):
+        c.set_graphs(graphs)
+        c.set_tile(8, 2, 1, secondary_gemm=1)
+    bp, bs = _batch(cp, w), _batch(cs, w)
+    try:
+        el = 0.0
+        # windows 0..3, then 3 single-window runs, then 4..7 in one run
+        for s0, s1 in ((0, 3), (3, 4), (4, 8)):
+            print(f"[pipeline] graphs={graphs} snapshots {s0}..{s1 - 1}", flush=True)
+            tp = bp.run(s0, s1, ns, w.z, w.dt_snap, elapsed=el)
+            ts = bs.run(s0, s1, ns, w.z, w.dt_snap, elapsed=el)
+            assert tp.cell_steps == ts.cell_steps
+            for a, b in zip(bp.get_state(), bs.get_state()):
+                assert np.array_equal(a, b), f"snapshots {s0}..{s1 - 1}"
+            el += float(np.sum(w.dt_snap[s0:s1]))
+    finally:
+        for o in (bp, bs, cp, cs):
+            o.close()
 
 
 @pytest.mark.parametrize("case", GRAPH_CASES, ids=[f"n{c[0]}-sp{c[3]}-gemm{c[8]}" for c in GRAPH_CASES])
