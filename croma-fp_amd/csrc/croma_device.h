@@ -539,17 +539,17 @@ __device__ __forceinline__ double d_exp_nonpos(double x)
     return ldexp(p, (int)n);
 }
 
-// a / b by the hardware reciprocal, two Newton steps and one residual
-// correction: the division sequence without the operand scaling and fix-up
-// for denormal, infinite or zero divisors, which the build's divisors (face
-// diffusion averages above 1e-200, Bernoulli denominators in (0, 1], pivots
-// above 1e-300 in magnitude) never are.  Within an ulp of the IEEE quotient.
+// a / b by the hardware reciprocal, one Newton step and one residual
+// correction (the step squares the reciprocal's relative error, the
+// correction squares the quotient's): the division sequence without the
+// operand scaling and fix-up for denormal, infinite or zero divisors, which
+// the build's divisors (face diffusion averages above 1e-200, Bernoulli
+// denominators in (0, 1], pivots above 1e-300 in magnitude) never are.
+// Within an ulp of the IEEE quotient.
 __device__ __forceinline__ double d_div_normal(double a, double b)
 {
     double r = __builtin_amdgcn_rcp(b);
-    double e = fma(-b, r, 1.0);
-    r = fma(r, e, r);
-    e = fma(-b, r, 1.0);
+    const double e = fma(-b, r, 1.0);
     r = fma(r, e, r);
     const double q = a * r;
     return fma(r, fma(-b, q, a), q);
