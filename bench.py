@@ -315,8 +315,8 @@ def main():
     split = bool(w.protons and tile[3])
     out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(f"k_fused_{args.config}_traffic"),
-                       "kernel": ("k_fused proton pass + secondary GEMM + k_fused electron pass (one snapshot)"
-                                  if split else "k_fused (one snapshot)"),
+                       "kernel": ("k_fused proton pass + secondary GEMM + k_fused electron pass"
+                                  if split else "k_fused") + f" (one launch window of {tile[2]} snapshot(s))",
                        "avg_launch_ms": kern_ms, "alg_bytes_per_launch": alg_bytes,
                        "fp64_valu": {"achieved_tflops": flops / (kern_ms * 1e-3) / 1e12, "peak_tflops": FP64_PEAK_TFLOPS,
                                      "frac": flops / (kern_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}}

@@ -2051,6 +2051,32 @@ int croma_hip_batch_autotune(CromaHipBatch *b, int s, int nsnap_total, const dou
             bt = cand[k];
         }
     }
+    /* snapshots per task: one launch per snapshot against one task over two
+     * snapshots, timed on snapshots s and s + 1 (the second needs its
+     * successor resident unless it is the last) */
+    if (s + 2 <= nsnap_total && s + 2 <= b->nsnap_res && (s + 2 == nsnap_total || s + 2 < b->nsnap_res)) {
+        double cms[2] = {0.0, 0.0};
+        int ch;
+        for (ch = 0; ch < 2; ch++) {
+            c->tile[b->protons ? 1 : 0] = bt;
+            c->tile[b->protons ? 1 : 0].chain = ch + 1;
+            for (rep = 0; rep < 2; rep++) {
+                CromaHipTimes tm;
+                memset(&tm, 0, sizeof(tm));
+                if (ck_d2d(b->d_cre, d_save, m * sizeof(double), c->stream) ||
+                    (b->protons && ck_d2d(b->d_crp, d_save + m, m * sizeof(double), c->stream))) {
+                    rc = dev_fail("autotune state restore");
+                    goto out;
+                }
+                if ((rc = croma_hip_batch_run(b, s, s + 2, nsnap_total, z, dt_snap, elapsed, CROMA_HIP_PATH_FUSED,
+                                              &tm)) != 0)
+                    goto out;
+                cms[ch] = tm.kernel_ms;
+                if (cms[ch] > 100.0) break;
+            }
+        }
+        bt.chain = (cms[1] < 0.99 * cms[0]) ? 2 : 1;
+    }
 out:
     if (d_save) {
         if (ck_d2d(b->d_cre, d_save, m * sizeof(double), c->stream) ||

@@ -231,8 +231,11 @@ int croma_hip_batch_sync(CromaHipBatch *b);
 /* Autotune the fused kernel's tile on the batch's own snapshot s: every
  * candidate (rows per lane x waves per SIMD, x split secondary for two
  * species) advances snapshot s from the current state, which is restored
- * after each trial and at the end.  Stores the fastest tile in the context
- * and returns it.  No reference counterpart (the reference has no tiles). */
+ * after each trial and at the end.  Then, where snapshot s + 1 is resident
+ * with its successor, the winner advances snapshots s and s + 1 with one
+ * and with two snapshots per queue task, and keeps two if that is more than
+ * 1% faster.  Stores the fastest tile in the context and returns it.  No
+ * reference counterpart (the reference has no tiles). */
 int croma_hip_batch_autotune(CromaHipBatch *b, int isnap, int nsnap_total, const double *z,
                              const double *dt_snap, double elapsed_gyr, CromaHipTile *best);
 

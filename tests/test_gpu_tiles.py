@@ -74,6 +74,7 @@ def test_batch_autotune_restores_state():
         print("[tiles] batch autotune", flush=True)
         tile = b.autotune(0, w.nsnap, w.z, w.dt_snap)
         assert tile == ctx.get_tile(True)
+        assert tile[2] in (1, 2)
         crp, cre = b.get_state()
         assert np.array_equal(crp, w.crp0) and np.array_equal(cre, w.cre0)
         b.run(0, ns, ns, w.z, w.dt_snap)
