@@ -19,8 +19,9 @@ from workloads import hip_params, rel_err
 pytestmark = pytest.mark.gpu
 
 
-def _run_full(w, s1):
+def _run_full(w, s1, tile):
     ctx = ch.Context(w.n, hip_params(w))
+    ctx.set_tile(tile[0], tile[1], tile[2], secondary_gemm=tile[3])
     b = ch.Batch(ctx, w.ntracer, w.species, w.nsnap)
     try:
         b.set_state(w.crp0 if w.protons else None, w.cre0, mass=w.mass)
@@ -48,11 +49,12 @@ def _oracle_nsub_total(w, s1):
     return total
 
 
-@pytest.mark.parametrize("name,s1,nsample", [("c2", 1, 48), ("c1", 3, 48)])
-def test_full_size_sample_and_invariants(name, s1, nsample):
+# the bench's tiles: rows per lane, waves per SIMD, snapshots per task, split GEMM
+@pytest.mark.parametrize("name,s1,nsample,tile", [("c2", 2, 48, (16, 2, 2, 0)), ("c1", 3, 48, (8, 2, 2, 1))])
+def test_full_size_sample_and_invariants(name, s1, nsample, tile):
     w = synth.make_workload(seed=2026, **synth.CONFIGS[name])
-    print(f"[fullsize] {name}: {w.ntracer} tracers x {w.n} bins, snapshots 0..{s1 - 1}", flush=True)
-    crp, cre, t = _run_full(w, s1)
+    print(f"[fullsize] {name}: {w.ntracer} tracers x {w.n} bins, snapshots 0..{s1 - 1}, tile {tile}", flush=True)
+    crp, cre, t = _run_full(w, s1, tile)
     print(f"  device done: {t.cell_steps} tracer-substeps", flush=True)
     assert t.cell_steps == _oracle_nsub_total(w, s1)
     for a in (cre, crp):

@@ -57,6 +57,27 @@ def test_all_tiles_match_oracle(refs, case):
     print(f"  worst rel err {worst:.3e}", flush=True)
 
 
+@pytest.mark.parametrize("sp,gemm", [(3, 1), (3, 0), (1, 0)])
+def test_two_snapshot_tasks_match_oracle(sp, gemm):
+    """Two snapshots per queue task over five snapshots (windows of 2, 2, 1),
+    as the workload autotuner picks for the bench workloads."""
+    n, nt, ns = 128, 96, 5
+    w = small_workload(n, nt, ns, sp, 1, dsa_frac=0.2, seed=37)
+    crp_ref, cre_ref = run_oracle.run_workload(w, run_oracle.oracle_params(w))
+    ctx = ch.Context(n, hip_params(w))
+    try:
+        ctx.set_tile(8, 2, 2, secondary_gemm=gemm)
+        print(f"[tiles] two snapshots per task, species={sp} gemm={gemm}", flush=True)
+        crp, cre, _ = run_hip(w, ctx, path=ch.PATH_FUSED)
+    finally:
+        ctx.close()
+    e_err, e_ok = rel_err(cre, cre_ref)
+    assert e_ok, f"electron max rel err {e_err:.3e}"
+    if sp == 3:
+        p_err, p_ok = rel_err(crp, crp_ref)
+        assert p_ok, f"proton max rel err {p_err:.3e}"
+
+
 def test_batch_autotune_restores_state():
     """The workload autotuner times every tile on the batch's own snapshot and
     leaves the state as it found it; the chosen tile then meets the parity bar."""
