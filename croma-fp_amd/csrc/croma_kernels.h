@@ -139,7 +139,9 @@ int ck_graph_destroy(void *graph_exec);
 void *ck_blas_create(void *stream);
 void ck_blas_destroy(void *handle);
 int ck_dgemm_nn(void *handle, int m, int n, int k, const double *A, int lda, const double *B, int ldb, double *C,
-                int ldc);
+                int ldc, int solution);
+int ck_dgemm_tune(void *handle, int m, int n, int k, const double *A, int lda, const double *B, int ldb, double *C,
+                  int ldc, void *stream, int *best, float *best_ms, float *default_ms);
 
 /* Inverse-Compton cooling rows on the device: for each of nz redshifts,
  * zb[3 iz .. 3 iz + 2] = {z, e0min, e0max} (CMB bounds from the host),

@@ -177,6 +177,8 @@ struct CromaHipContext {
     int *h_npmin;
     double *d_fqe;
     void *blas; /* vendor GEMM handle for the split secondary contraction */
+    int gemm_sol;       /* its solution for gemm_sol_rows rows (0: the library's choice) */
+    long gemm_sol_rows;
     /* the last IC row a window built (device, n doubles) and its (z, lo, hi):
      * the next window's first snapshot is usually that z */
     double *d_ic_last;
@@ -1283,7 +1285,7 @@ static int launch_gemm(CromaHipBatch *b, WinBuf *w)
     if (!c->blas) return dev_fail("GEMM handle");
     if (w->segq > 0 && jp0 < n &&
         ck_dgemm_nn(c->blas, n, (int)w->segq, n - jp0, c->d_fqe + (size_t)jp0 * n, n, b->d_segrec + jp0, n,
-                    b->d_seginje, n))
+                    b->d_seginje, n, w->segq == c->gemm_sol_rows ? c->gemm_sol : 0))
         return dev_fail("secondary GEMM");
     if (w->segq > 0 && jp0 >= n && ck_fill_f64(b->d_seginje, 0.0, (size_t)w->segq * n, c->stream))
         return dev_fail("secondary clear");
@@ -1350,7 +1352,7 @@ static int launch_window(CromaHipBatch *b, WinBuf *w, WinBuf *prev, int last, Gr
     key[4] = (long)(size_t)b->d_segrec;
     key[5] = b->protons;
     key[6] = c->jp0 | ((long)w->dev_nsub << 20) | ((long)w->count << 21);
-    key[7] = b->ntr;
+    key[7] = b->ntr | ((long)(w->segq == c->gemm_sol_rows ? c->gemm_sol : 0) << 32);
     wi.b = b;
     wi.w = w;
     wi.pipe = pipe;
@@ -2076,6 +2078,20 @@ int croma_hip_batch_autotune(CromaHipBatch *b, int s, int nsnap_total, const dou
             }
         }
         bt.chain = (cms[1] < 0.99 * cms[0]) ? 2 : 1;
+    }
+    /* the split GEMM's solution for the batch's row capacity (every window
+     * runs the GEMM over all of it) */
+    if (b->protons && bt.secondary_gemm && b->segrows_cap > 0 && c->blas && c->jp0 < c->g.n &&
+        b->segrows_cap <= 0x7fffffffL) {
+        const int n = c->g.n, jp0 = c->jp0;
+        int sol = 0;
+        if (ck_dgemm_tune(c->blas, n, (int)b->segrows_cap, n - jp0, c->d_fqe + (size_t)jp0 * n, n, b->d_segrec + jp0,
+                          n, b->d_seginje, n, c->stream, &sol, 0, 0)) {
+            rc = dev_fail("GEMM solution tuning");
+            goto out;
+        }
+        c->gemm_sol = sol;
+        c->gemm_sol_rows = b->segrows_cap;
     }
 out:
     if (d_save) {

@@ -593,7 +593,7 @@ static int sv_launch_class(CromaHipServer *s, int ci)
                                  ntasks, k->d_ctrl, k->d_ctrl + 1, k->stream);
             if (!rc && segrows > 0 && jp0 < n)
                 rc = ck_dgemm_nn(k->blas, n, (int)segrows, n - jp0, ch_ctx_d_fqe(c) + (size_t)jp0 * n, n,
-                                 k->d_segrec + jp0, n, k->d_seginje, n);
+                                 k->d_segrec + jp0, n, k->d_seginje, n, 0);
             if (!rc && segrows > 0 && jp0 >= n) rc = ck_fill_f64(k->d_seginje, 0.0, (size_t)segrows * n, k->stream);
             if (!rc) rc = ck_fill_i32(k->d_ctrl, 0, 1, k->stream);
             if (!rc)
