@@ -234,8 +234,11 @@ int croma_hip_batch_sync(CromaHipBatch *b);
  * after each trial and at the end.  Then, where snapshot s + 1 is resident
  * with its successor, the winner advances snapshots s and s + 1 with one
  * and with two snapshots per queue task, and keeps two if that is more than
- * 1% faster.  Stores the fastest tile in the context and returns it.  No
- * reference counterpart (the reference has no tiles). */
+ * 1% faster.  For a split tile it then times the vendor library's
+ * deterministic FP64 GEMM solutions at the batch's segment-row capacity and
+ * keeps the fastest for this context.  Stores the fastest tile in the
+ * context and returns it.  No reference counterpart (the reference has no
+ * tiles). */
 int croma_hip_batch_autotune(CromaHipBatch *b, int isnap, int nsnap_total, const double *z,
                              const double *dt_snap, double elapsed_gyr, CromaHipTile *best);
 
