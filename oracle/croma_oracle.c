@@ -1712,3 +1712,15 @@ void orc_emission_synch(int n, int ntracer, int nfreq, int nlogb, double logb_mi
         }
     }
 }
+
+/* ---------------------------------------------------------------------- */
+/* Pieces of the hadronic kernels, exported so the tests can check them    */
+/* against sum rules (pion / lepton counting) rather than against a copy   */
+/* of their own formulas.                                                  */
+double orc_hadron_dsigma(int kind, double epi, double ep) { return orc_dsigma(kind, epi, ep); }
+double orc_hadron_fpi(double epi, double ep) { return orc_FpiQ(0, epi, ep); }
+double orc_hadron_fe(double x) { return orc_fe(x); }
+double orc_hadron_fnue(double x) { return orc_fnue(x); }
+double orc_hadron_Fe(double ee, double epi) { return orc_Fe(ee, epi); }
+double orc_hadron_ekernel(int kind, double e, double p) { return orc_ekernel(kind, e, p); }
+double orc_hadron_numu1(double e, double p) { return orc_numu1_kernel(e, p); }

@@ -188,6 +188,21 @@ void orc_emission_synch(int n, int ntracer, int nfreq, int nlogb, double logb_mi
                         double inv_dlogb, const double *table, const double *cre,
                         const double *dp, const double *b_field, double *eps);
 
+
+/* Pieces of the hadronic kernels (HADRONIC.c:139-498) for the sum-rule
+ * checks: dsigma (pion production spectrum x cross section, :291-300), the
+ * QGSJET pion spectrum FpiQ (:326-336), the Kelner decay spectra fe / fnue
+ * (:302-322), the Brunetti-Blasi muon-decay electron spectrum Fe (:500-533),
+ * eKernel for nu_e (kind 1) / nu_mu from muon decay (kind 2) (:179-235) and
+ * numu1Kernel (:239-289). */
+double orc_hadron_dsigma(int kind, double epi, double ep);
+double orc_hadron_fpi(double epi, double ep);
+double orc_hadron_fe(double x);
+double orc_hadron_fnue(double x);
+double orc_hadron_Fe(double ee, double epi);
+double orc_hadron_ekernel(int kind, double e, double p);
+double orc_hadron_numu1(double e, double p);
+
 #ifdef __cplusplus
 }
 #endif

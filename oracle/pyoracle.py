@@ -67,6 +67,13 @@ def lib():
         _lib.orc_nu_kernel.argtypes = [C.c_double] * 2
         _lib.orc_ic_emissivity_kernel.argtypes = [C.c_double] * 3
         _lib.orc_ctx_j_pp.argtypes = [C.c_void_p]
+        for name, args in (("orc_hadron_dsigma", [C.c_int, C.c_double, C.c_double]),
+                           ("orc_hadron_fpi", [C.c_double] * 2), ("orc_hadron_fe", [C.c_double]),
+                           ("orc_hadron_fnue", [C.c_double]), ("orc_hadron_Fe", [C.c_double] * 2),
+                           ("orc_hadron_ekernel", [C.c_int, C.c_double, C.c_double]),
+                           ("orc_hadron_numu1", [C.c_double] * 2)):
+            getattr(_lib, name).restype = C.c_double
+            getattr(_lib, name).argtypes = args
     return _lib
 
 
