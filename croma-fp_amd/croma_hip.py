@@ -406,6 +406,21 @@ def solve_cc_batch(a, b, c, source, state, dt):
     return out, ms.value
 
 
+def solve_cc_batch_noalloc(nsys, nrow, dt, a_ptr, b_ptr, c_ptr, source_ptr, state_in_ptr, state_out_ptr, stream=0):
+    """Allocation-free batched solve on caller-owned DEVICE arrays (addresses
+    as integers, e.g. torch tensors' data_ptr()); returns the library's code
+    (0, or raises on error)."""
+    L = lib()
+    fn = L.croma_hip_solve_cc_batch_noalloc
+    fn.argtypes = [C.c_int, C.c_int, C.c_double] + [C.c_void_p] * 6 + [C.c_void_p]
+    fn.restype = C.c_int
+    _chk(fn(nsys, nrow, dt, a_ptr, b_ptr, c_ptr, source_ptr, state_in_ptr, state_out_ptr, stream or None),
+         "solve_cc_batch_noalloc")
+
+
+NOALLOC_MAX_CELLS = 4 * 1024 * 1024
+
+
 class Server:
     """Request loop (croma_hip_serve_*): chunks of tracers are submitted at any
     time, admitted against an HBM budget and advanced together on the device."""

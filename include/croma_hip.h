@@ -249,6 +249,24 @@ int croma_hip_solve_cc_batch(int nsys, int nrow, double dt, const double *a, con
                              const double *c, const double *source, const double *state_init,
                              double *state_out, double *elapsed_ms);
 
+/* Allocation-free variant of croma_hip_solve_cc_batch.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same kernel and arithmetic, but a, b, c, source, state_init
+ * and state_out are DEVICE pointers ([nsys * nrow] cell-major FP64, caller
+ * owned; state_out may equal state_init), the work runs on `stream` (a
+ * hipStream_t; NULL is the null stream) and the call returns once that
+ * stream has drained.  The solver's scratch (the Thomas c' workspace and the
+ * pivot-failure word) is a static device array of the library, one per GPU,
+ * held until the process ends: nsys * nrow may not exceed
+ * CROMA_HIP_NOALLOC_MAX_CELLS (32 MiB of c'), else CROMA_HIP_EINVAL.  Calls
+ * that overlap on one GPU share that array and race.  Returns 0,
+ * CROMA_HIP_EPIVOT, CROMA_HIP_EINVAL or CROMA_HIP_EDEVICE. */
+#define CROMA_HIP_NOALLOC_MAX_CELLS (4 * 1024 * 1024)
+int croma_hip_solve_cc_batch_noalloc(int nsys, int nrow, double dt, const double *a, const double *b,
+                                     const double *c, const double *source, const double *state_init,
+                                     double *state_out, void *stream);
+
 /* ---- emission (prepare_{synch,gamma,neutrino}_emission_cuda_batch,
  * fp_cuda_backend.h:97-110, plus IC which the reference runs on the CPU) */
 #define CROMA_HIP_EMIS_FP32 0        /* FP32 VALU contraction */
