@@ -19,12 +19,13 @@ def oracle_params(w, **kw):
 
 
 def run_workload(w, prm=None, s0=0, s1=None, nthreads=0, oracle=None, crp=None, cre=None, return_nsub=False,
-                 secondary=None, ic_rows=None):
+                 secondary=None, ic_rows=None, dis=None):
     """Advance the workload's spectra over snapshots [s0, s1); returns (crp, cre[, nsub list]).
 
     secondary: precomputed (fqe_t, np_min_qe) of the grid; ic_rows: dict
     z -> (row, m1, p1) of precomputed IC cooling rows (both are per-grid or
-    per-snapshot constants shared by every tracer)."""
+    per-snapshot constants shared by every tracer); dis: per-tracer
+    disable_adiabatic flags [ntracer] (uint8) or None."""
     o = oracle or po.Oracle(w.n, w.pmin, w.pmax, w.pemin, w.pemax)
     prm = prm or oracle_params(w)
     s1 = w.nsnap if s1 is None else s1
@@ -57,7 +58,7 @@ def run_workload(w, prm=None, s0=0, s1=None, nthreads=0, oracle=None, crp=None, 
         qpi = w.qpi.get(s, zeros)
         qepri = w.qepri.get(s, zeros)
         o.execute_snapshot(prm, ns, non, allow, elapsed, w.dt_snap[s], bgc, row, m1, p1, bgn, rown, m1n, p1n, qpi,
-                           qepri, fqe, npmin, w.mass, None, crp, cre, nthreads=nthreads)
+                           qepri, fqe, npmin, w.mass, dis, crp, cre, nthreads=nthreads)
         elapsed += w.dt_snap[s]
     if return_nsub:
         return crp, cre, nsubs
