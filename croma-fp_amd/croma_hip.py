@@ -184,6 +184,11 @@ class Context:
         t = Tile(rows_per_lane, waves_per_simd, chain, secondary_gemm)
         _chk(lib().croma_hip_context_set_tile(C.c_void_p(self.h), int(protons), C.byref(t)), "set_tile")
 
+    def set_coeff_precision(self, fp32):
+        """FP32 (True) or FP64 (False, default) coefficient evaluation of the fused path."""
+        _chk(lib().croma_hip_context_set_coeff_precision(C.c_void_p(self.h), 1 if fp32 else 0),
+             "set_coeff_precision")
+
     def set_graphs(self, enable):
         """True: repeated snapshot windows replay a captured hipGraph of their launches."""
         _chk(lib().croma_hip_context_set_graphs(C.c_void_p(self.h), int(bool(enable))), "set_graphs")

@@ -71,6 +71,25 @@ __device__ __forceinline__ double d_b_coulomb_p_tab(double nicm, double kBT, dou
     return K_GYR * K_C / (K_MP * K_GEV) * sum * nicm * ib2 * thermal * 3.5e-29;
 }
 
+// FP32 form of d_b_coulomb_p_tab (the FP32-coefficient instances).
+__device__ __forceinline__ float d_coulomb_kernelf(float x)
+{
+    if (x >= 40.0f) return (float)(0.5 * 1.7724538509055160273);
+    const float sx = sqrtf(x);
+    return (float)(0.5 * 1.7724538509055160273) * erff(sx) - sx * __expf(-x);
+}
+
+__device__ __forceinline__ float d_b_coulomb_p_tabf(float nicm, float kBT, float ikbt, float Ep, float i2mk, float ib2)
+{
+#pragma clang fp contract(off)
+    const float xp = Ep * ikbt;
+    const float xe = xp * (float)(K_ME / K_MP);
+    const float sum = d_coulomb_kernelf(xe) + d_coulomb_kernelf(xp) * (float)(K_ME / K_MP);
+    const float thermal = 1.0f - kBT * i2mk;
+    if (thermal < 0.0f) return 0.0f;
+    return (float)(K_GYR * K_C / (K_MP * K_GEV) * 3.5e-29) * sum * nicm * ib2 * thermal;
+}
+
 // Integer powers below are products (pow with an integer exponent differs
 // from them by an ulp or two; these enter t_acc, and through it D_pp, only
 // multiplicatively).
@@ -771,14 +790,18 @@ __device__ __forceinline__ void apply_solve(int gl, const Factor<R, G> &F, doubl
 // Per-species view of the per-row grid tables of the build and the loss
 // rates: the device tables (row j at j), or their copies in LDS (row j at
 // j + (j >> ps), the padding of the rate rows).  ps is a literal at every
-// construction, so the index arithmetic folds away.
-struct SpGrid {
-    const double *x, *h, *inv_h, *inv_dx, *two_over_p, *p2, *expcut, *ib2;
-    const double *lnp;                     // electrons only
-    const double *expE, *sig, *Ep, *i2mk;  // protons only
-    int ps;                                // < 0: device tables
+// construction, so the index arithmetic folds away.  T is the coefficient
+// type: double, or float in the FP32-coefficient instances (whose tables are
+// float copies in LDS).
+template <typename T>
+struct SpGridT {
+    const T *x, *h, *inv_h, *inv_dx, *two_over_p, *p2, *expcut, *ib2;
+    const T *lnp;                     // electrons only
+    const T *expE, *sig, *Ep, *i2mk;  // protons only
+    int ps;                           // < 0: device tables
     __device__ __forceinline__ int ix(int j) const { return ps < 0 ? j : j + (j >> ps); }
 };
+using SpGrid = SpGridT<double>;
 
 // Tables of one species, in the order of their LDS copies: the common set,
 // then ln p (electrons, GT_NE tables) or the proton factors (GT_NP tables).
@@ -804,14 +827,19 @@ __device__ __forceinline__ const double *sp_table(const CkGrid &g, bool elec, in
     }
 }
 
-// tab: the device tables (ps < 0) or the species' LDS copies, np doubles apart
-__device__ __forceinline__ SpGrid sp_grid(const CkGrid &g, bool elec, const double *tab, int np, int ps)
+// tab: the device tables (ps < 0, T = double only) or the species' LDS
+// copies, np elements apart
+template <typename T>
+__device__ __forceinline__ SpGridT<T> sp_grid(const CkGrid &g, bool elec, const T *tab, int np, int ps)
 {
-    SpGrid s;
-    const double *t[GT_NP];
+    SpGridT<T> s;
+    const T *t[GT_NP];
 #pragma unroll
-    for (int i = 0; i < GT_NP; ++i)
-        t[i] = (i >= (elec ? GT_NE : GT_NP)) ? nullptr : (ps < 0) ? sp_table(g, elec, i) : tab + (size_t)i * np;
+    for (int i = 0; i < GT_NP; ++i) {
+        if (i >= (elec ? GT_NE : GT_NP)) t[i] = nullptr;
+        else if constexpr (sizeof(T) == sizeof(double)) t[i] = (ps < 0) ? sp_table(g, elec, i) : tab + (size_t)i * np;
+        else t[i] = tab + (size_t)i * np;
+    }
     s.x = t[GT_X]; s.h = t[GT_H]; s.inv_h = t[GT_INVH]; s.inv_dx = t[GT_INVDX]; s.two_over_p = t[GT_2OP];
     s.p2 = t[GT_P2]; s.expcut = t[GT_EXPCUT]; s.ib2 = t[GT_IB2];
     s.lnp = elec ? t[GT_LNP] : nullptr;
@@ -834,18 +862,35 @@ struct Face {
     double bm, Fh, wp, wm, w;
 };
 
-__device__ __forceinline__ Face d_face2(double Bl, double Br, double Cl, double Cr, double h, double inv_h)
+// FP32 coefficients (T = float): the face averages, diffusion flux factor
+// and Peclet number w come from float rows; the Bernoulli weights of w are
+// still evaluated in FP64 (the stability-critical part of the weights).
+template <typename T>
+__device__ __forceinline__ Face d_face2(T Bl, T Br, T Cl, T Cr, T h, T inv_h)
 {
 #pragma clang fp contract(off)
     Face f;
-    f.bm = 0.5 * (Bl + Br);
-    const double cm = 0.5 * (Cl + Cr);
-    f.Fh = cm * inv_h;
+    double wv;
+    bool live;
+    if constexpr (sizeof(T) == sizeof(double)) {
+        f.bm = 0.5 * (Bl + Br);
+        const double cm = 0.5 * (Cl + Cr);
+        f.Fh = cm * inv_h;
+        live = fabs(Cl) > 1.0e-200;
+        wv = live ? d_div_normal(f.bm, cm) * h : 0.0;
+    } else {
+        const float bm = 0.5f * (Bl + Br);
+        const float cm = 0.5f * (Cl + Cr);
+        f.bm = (double)bm;
+        f.Fh = (double)(cm * inv_h);
+        live = fabs((double)Cl) > 1.0e-200;
+        wv = live ? (double)(bm * __builtin_amdgcn_rcpf(cm) * h) : 0.0;
+    }
     f.wp = 0.0;
     f.wm = 0.0;
     f.w = 0.0;
-    if (fabs(Cl) > 1.0e-200) {
-        const double w = d_div_normal(f.bm, cm) * h;
+    if (live) {
+        const double w = wv;
         const double aw = fabs(w);
         const double h2 = d_exp_nonpos(-0.5 * aw);
         const double E = h2 * h2;
@@ -914,23 +959,41 @@ struct RunCoef {
     double dtT_fac;  // dt * 0.5 c n_gas Gyr (protons), 0 for electrons
 };
 
-template <bool ELEC>
-__device__ __forceinline__ void row_BC(const SpGrid &sg, const RunCoef &rc, int j, double radp, double &B, double &C)
+template <bool ELEC, typename T>
+__device__ __forceinline__ void row_BC(const SpGridT<T> &sg, const RunCoef &rc, int j, T radp, T &B, T &C)
 {
 #pragma clang fp contract(off)
     const int i = sg.ix(j);
-    double D;
-    if (!rc.model) D = sg.p2[i] * rc.dfac;
-    else if (ELEC) D = sg.p2[i] * rc.dfac * sg.expcut[i];
-    else D = sg.p2[i] * rc.dfac * sg.expE[i] * sg.expcut[i];
+    const T dfac = (T)rc.dfac;
+    T D;
+    if (!rc.model) D = sg.p2[i] * dfac;
+    else if (ELEC) D = sg.p2[i] * dfac * sg.expcut[i];
+    else D = sg.p2[i] * dfac * sg.expE[i] * sg.expcut[i];
     C = D;
     B = radp - sg.two_over_p[i] * D;
 }
 
+// lane l + 1 / l - 1 across the wave for the row inputs of the build (one
+// 32-bit DPP move for float rows, two for double)
+template <typename T>
+__device__ __forceinline__ T wave_dn1_t(T v)
+{
+    if constexpr (sizeof(T) == sizeof(double)) return wave_dn1(v);
+    else return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xF, 0xF, true));
+}
+template <typename T>
+__device__ __forceinline__ T wave_up1_t(T v)
+{
+    if constexpr (sizeof(T) == sizeof(double)) return wave_up1(v);
+    else return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF, true));
+}
+
 // Builds the rows of the lane into F (al = a, iv = b, g = c) and factorises.
-template <bool ELEC, int R, int G>
-__device__ __forceinline__ void build_factor(int gl, int n, const SpGrid &sg, const RunCoef &rc,
-                                             const double *radp, const SrcRows &src, Factor<R, G> &F, int &bad)
+// The rows' coefficients (B, C and the face inputs) are of type T; the
+// face weights, the rows a, b, c and the factorisation are FP64.
+template <bool ELEC, int R, int G, typename T>
+__device__ __forceinline__ void build_factor(int gl, int n, const SpGridT<T> &sg, const RunCoef &rc,
+                                             const T *radp, const SrcRows &src, Factor<R, G> &F, int &bad)
 {
 #pragma clang fp contract(off)
     const int base = gl * R;
@@ -940,37 +1003,38 @@ __device__ __forceinline__ void build_factor(int gl, int n, const SpGrid &sg, co
     double sv[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) sv[k] = src.scale * (src.inj[o + k] * src.ka * src.kb + src.q[o + k]);
-    double B0, C0, BL, CL;
-    row_BC<ELEC>(sg, rc, base, radp[o], B0, C0);
-    row_BC<ELEC>(sg, rc, base + R - 1, radp[o + R - 1], BL, CL);
+    T B0, C0, BL, CL;
+    row_BC<ELEC, T>(sg, rc, base, radp[o], B0, C0);
+    row_BC<ELEC, T>(sg, rc, base + R - 1, radp[o + R - 1], BL, CL);
     // neighbour lanes' edge rows (DPP across the wave; a group's edge lanes
     // read the next group's values and do not use them)
-    const double Bn = wave_dn1(B0), Cn = wave_dn1(C0);
-    const double Bp = wave_up1(BL), Cp = wave_up1(CL);
+    const T Bn = wave_dn1_t(B0), Cn = wave_dn1_t(C0);
+    const T Bp = wave_up1_t(BL), Cp = wave_up1_t(CL);
     Face fl = {0.0, 0.0, 0.0, 0.0, 0.0};
-    if (gl > 0) fl = d_face2(Bp, B0, Cp, C0, sg.h[sg.ix(base - 1)], sg.inv_h[sg.ix(base - 1)]);
-    double Cl = Cp;
-    double Bk = B0, Ck = C0;
+    if (gl > 0) fl = d_face2<T>(Bp, B0, Cp, C0, sg.h[sg.ix(base - 1)], sg.inv_h[sg.ix(base - 1)]);
+    double Cl = (double)Cp;
+    T Bk = B0, Ck = C0;
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         const int j = base + k;
         const bool last = (j == n - 1);
-        double Br, Cr;
+        T Br, Cr;
         if (k == R - 1) { Br = Bn; Cr = Cn; }
         else if (k == R - 2) { Br = BL; Cr = CL; }
-        else row_BC<ELEC>(sg, rc, j + 1, radp[o + k + 1], Br, Cr);
+        else row_BC<ELEC, T>(sg, rc, j + 1, radp[o + k + 1], Br, Cr);
         Face fr = {0.0, 0.0, 0.0, 0.0, 0.0};
-        if (!last) fr = d_face2(Bk, Br, Ck, Cr, sg.h[sg.ix(j)], sg.inv_h[sg.ix(j)]);
+        if (!last) fr = d_face2<T>(Bk, Br, Ck, Cr, sg.h[sg.ix(j)], sg.inv_h[sg.ix(j)]);
         const int kind = (j == 0) ? 1 : (last ? 2 : 0);
-        const double t1 = rc.dt * sg.inv_dx[sg.ix(j)];
-        const double dtT = ELEC ? 0.0 : rc.dtT_fac * sg.sig[sg.ix(j)];
+        const double t1 = rc.dt * (double)sg.inv_dx[sg.ix(j)];
+        double dtT = 0.0;
+        if constexpr (!ELEC) dtT = (double)((T)rc.dtT_fac * sg.sig[sg.ix(j)]);
         double a, b, c;
-        d_cc_row2(kind, t1, Ck, dtT, fl, fr, Cl, a, b, c);
+        d_cc_row2(kind, t1, (double)Ck, dtT, fl, fr, Cl, a, b, c);
         F.al[k] = a;
         F.iv[k] = b;
         F.g[k] = c;
         fl = fr;
-        Cl = Ck;
+        Cl = (double)Ck;
         Bk = Br;
         Ck = Cr;
     }
@@ -986,28 +1050,28 @@ struct SegScalars {
 
 // Electron loss rates of the lane's rows for one segment
 // (prepare_cre_losses_1d, fp_shared_core.c:780-842).
-template <int R>
-__device__ __forceinline__ void seg_radp_e(const SpGrid &sg, const CkSlot &S, int base, int nseg, double alpha,
-                                           const SegScalars &q, double *radp)
+template <int R, typename T>
+__device__ __forceinline__ void seg_radp_e(const SpGridT<T> &sg, const CkSlot &S, int base, int nseg, double alpha,
+                                           const SegScalars &q, T *radp)
 {
 #pragma clang fp contract(off)
     {
         const double b_micro = q.bfield * 1.0e+6;
         const double b_ratio = b_micro / 3.2;
-        const double synch_coeff = 4.8e-4 * b_ratio * b_ratio * 9.11e-28 * 3.0e+10 * 3.1536e+16;
-        const double coul_coeff = 3.05e-29 * q.n_gas * K_C * K_GYR / (K_ME * K_GEV);
-        const double adia = 1.0 / 3.0 * q.divv;
+        const T synch_coeff = (T)(4.8e-4 * b_ratio * b_ratio * 9.11e-28 * 3.0e+10 * 3.1536e+16);
+        const T coul_coeff = (T)(3.05e-29 * q.n_gas * K_C * K_GYR / (K_ME * K_GEV));
+        const T adia = (T)(1.0 / 3.0 * q.divv);
         // ln(p / n_gas) as ln p (table) + ln(1 / n_gas), one log per segment
-        const double lning = log(1.0 / q.n_gas);
+        const T lning = (T)log(1.0 / q.n_gas);
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             const int j = base + k;
-            const double pe = sg.x[sg.ix(j)];
-            const double pe2 = sg.p2[sg.ix(j)];
+            const T pe = sg.x[sg.ix(j)];
+            const T pe2 = sg.p2[sg.ix(j)];
             // both rows loaded unconditionally (ric_n is always a valid row)
             // alpha is 0 without interpolation; ric_n is always a valid row
-            const double ric = d_lerp(S.ric_c[j], S.ric_n[j], alpha);
-            const double coul = coul_coeff * (1.0 + (sg.lnp[sg.ix(j)] + lning) * (1.0 / 74.8)) * sg.ib2[sg.ix(j)];
+            const T ric = (T)d_lerp(S.ric_c[j], S.ric_n[j], alpha);
+            const T coul = coul_coeff * ((T)1.0 + (sg.lnp[sg.ix(j)] + lning) * (T)(1.0 / 74.8)) * sg.ib2[sg.ix(j)];
             radp[k] = synch_coeff * pe2 + ric + coul + adia * pe;
         }
     }
@@ -1021,17 +1085,25 @@ __device__ __forceinline__ void seg_radp_e(const SpGrid &sg, const CkSlot &S, in
 // branch for each of its R rows, in the strided order the wave takes it
 // ceil(rows / G) times.  s_tr holds N + N / R doubles (row j at j + j / R:
 // one pad slot per block of R rows against LDS bank conflicts on the reads).
-template <int N, int R, int G>
-__device__ __forceinline__ void seg_radp_p(const SpGrid &sg, int gl, const SegScalars &q, double *s_tr)
+template <int N, int R, int G, typename T>
+__device__ __forceinline__ void seg_radp_p(const SpGridT<T> &sg, int gl, const SegScalars &q, T *s_tr)
 {
 #pragma clang fp contract(off)
-    const double adia = 1.0 / 3.0 * q.divv;
-    const double ikbt = 1.0 / q.kbt;
 #pragma unroll
     for (int m = 0; m < R; ++m) {
         const int j = m * G + gl;
         const int i = sg.ix(j);
-        s_tr[j + j / R] = d_b_coulomb_p_tab(q.n_gas, q.kbt, ikbt, sg.Ep[i], sg.i2mk[i], sg.ib2[i]) + adia * sg.x[i];
+        if constexpr (sizeof(T) == sizeof(double)) {
+            const double adia = 1.0 / 3.0 * q.divv;
+            const double ikbt = 1.0 / q.kbt;
+            s_tr[j + j / R] =
+                d_b_coulomb_p_tab(q.n_gas, q.kbt, ikbt, sg.Ep[i], sg.i2mk[i], sg.ib2[i]) + adia * sg.x[i];
+        } else {
+            const float adia = (float)(1.0 / 3.0 * q.divv);
+            s_tr[j + j / R] = d_b_coulomb_p_tabf((float)q.n_gas, (float)q.kbt, (float)(1.0 / q.kbt), sg.Ep[i],
+                                                 sg.i2mk[i], sg.ib2[i]) +
+                              adia * sg.x[i];
+        }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -1060,10 +1132,10 @@ __device__ __forceinline__ void unpark_rows(const double *own, double (&x)[R])
 // All runs of one species over one segment: up to three runs of constant
 // coefficients (before / inside / after the on-window, tracer_fp_cpu.c:150-157).
 // On entry the spectrum is parked (park_rows); on exit it is in x.
-template <bool ELEC, int N, int R, int G>
-__device__ __forceinline__ void species_segment(int gl, const SpGrid &sg, int steps, bool windowed, int ls, int le,
+template <bool ELEC, int N, int R, int G, typename T>
+__device__ __forceinline__ void species_segment(int gl, const SpGridT<T> &sg, int steps, bool windowed, int ls, int le,
                                                 RunCoef rc, double dfac_on, double dfac_off, bool mode_on,
-                                                const double *radp, const SrcRows &src, double *xpark,
+                                                const T *radp, const SrcRows &src, double *xpark,
                                                 double (&x)[R], int &bad)
 {
     int s0 = 0;
@@ -1080,7 +1152,7 @@ __device__ __forceinline__ void species_segment(int gl, const SpGrid &sg, int st
         rc.dfac = on ? dfac_on : dfac_off;
         if (s0 > 0) park_rows<R>(xpark, x);
         Factor<R, G> F;
-        build_factor<ELEC, R, G>(gl, N, sg, rc, radp, src, F, bad);
+        build_factor<ELEC, R, G, T>(gl, N, sg, rc, radp, src, F, bad);
         unpark_rows<R>(xpark, x);
         for (int s = s0; s < s1; ++s) apply_solve<R, G>(gl, F, x);
         s0 = s1;
@@ -1096,7 +1168,8 @@ __device__ __forceinline__ void species_segment(int gl, const SpGrid &sg, int st
 enum { CK_MODE_E = 0,      // electron_only: protons are exactly zero
        CK_MODE_EP = 1,     // both species, secondaries contracted in the kernel
        CK_MODE_P_REC = 2,  // protons only; records the segment-start proton rows
-       CK_MODE_E_SEC = 3   // electrons only; secondaries from the contracted rows
+       CK_MODE_E_SEC = 3,  // electrons only; secondaries from the contracted rows
+       CK_MODE_F32 = CK_LAUNCH_F32  // added to a mode: FP32 coefficient evaluation (losses, D_pp, B, C, face inputs)
 };
 
 // LDS doubles per lane group of the fused kernel: loss rates (padded), the
@@ -1114,28 +1187,31 @@ __host__ __device__ constexpr int fused_grid_tables(int MODE)
 }
 
 // Whether the blocks that the register budget lets share a CU (OCC waves per
-// SIMD, WPB waves per block) fit its 160 KiB of LDS with the grid tables (LG)
-// and the parked spectra (PK).
-__host__ __device__ constexpr bool fused_lds_fits(int N, int R, int WPB, int MODE, int OCC, bool LG, bool PK)
+// SIMD, WPB waves per block) fit its 160 KiB of LDS with the grid tables (LG,
+// TB bytes per entry: 8 for FP64 coefficients, 4 for FP32) and the parked
+// spectra (PK).
+__host__ __device__ constexpr bool fused_lds_fits(int N, int R, int WPB, int MODE, int OCC, bool LG, bool PK,
+                                                  int TB = 8)
 {
-    return (size_t)8 * (WPB * (64 / (N / R)) * fused_lds_doubles(N, R, MODE, PK) +
-                        (LG ? fused_grid_tables(MODE) * (N + N / R + 1) : 0)) *
+    return ((size_t)8 * WPB * (64 / (N / R)) * fused_lds_doubles(N, R, MODE, PK) +
+            (LG ? (size_t)TB * fused_grid_tables(MODE) * (N + N / R + 1) : 0)) *
                (OCC * 4 / WPB > 0 ? OCC * 4 / WPB : 1) <=
            (size_t)160 * 1024;
 }
 // LDS use of an instance: the grid tables take precedence over parking (the
 // tables cut the build's load latency, parking its register pressure).
-__host__ __device__ constexpr bool fused_use_lg(int N, int R, int WPB, int MODE, int OCC)
+// FP32-coefficient instances exist only where their float tables fit.
+__host__ __device__ constexpr bool fused_use_lg(int N, int R, int WPB, int MODE, int OCC, int TB = 8)
 {
-    return fused_lds_fits(N, R, WPB, MODE, OCC, true, false);
+    return fused_lds_fits(N, R, WPB, MODE, OCC, true, false, TB);
 }
-__host__ __device__ constexpr bool fused_use_pk(int N, int R, int WPB, int MODE, int OCC)
+__host__ __device__ constexpr bool fused_use_pk(int N, int R, int WPB, int MODE, int OCC, int TB = 8)
 {
-    return fused_lds_fits(N, R, WPB, MODE, OCC, fused_use_lg(N, R, WPB, MODE, OCC), true);
+    return fused_lds_fits(N, R, WPB, MODE, OCC, fused_use_lg(N, R, WPB, MODE, OCC, TB), true, TB);
 }
 
-template <int N, int R, int MODE, bool PK>
-__device__ __forceinline__ void snapshot_update(const CkGrid &g, const SpGrid &sge, const SpGrid &sgp,
+template <int N, int R, int MODE, bool PK, typename T>
+__device__ __forceinline__ void snapshot_update(const CkGrid &g, const SpGridT<T> &sge, const SpGridT<T> &sgp,
                                                 const CkSlot &S, int t, int gl, double *s_grp, double (&xe)[R],
                                                 double (&xp)[R], int &bad)
 {
@@ -1163,8 +1239,9 @@ __device__ __forceinline__ void snapshot_update(const CkGrid &g, const SpGrid &s
     const bool dis = S.dis ? (S.dis[t] != 0) : false;
     const bool sec = (MODE == CK_MODE_EP) && S.fqe_t;
     const int row0 = (MODE == CK_MODE_P_REC || MODE == CK_MODE_E_SEC) ? S.seg_off[t] : 0;
-    double *const s_rad = s_grp;
-    const double *const own_rad = s_grp + gl * (R + 1);
+    // loss rates in the coefficient type (float rates use the region's first half)
+    T *const s_rad = (T *)s_grp;
+    const T *const own_rad = (T *)s_grp + gl * (R + 1);
     double *const own_x = PK ? s_grp + (N + N / R + 1) + gl * (R + 1) : nullptr;
     double *const s_row = s_grp + (N + N / R + 1) + (PK ? G * (R + 1) : 0);
     double *const own_src = s_row + base;
@@ -1249,11 +1326,11 @@ __device__ __forceinline__ void snapshot_update(const CkGrid &g, const SpGrid &s
                 for (int k = 0; k < R; ++k) rec[k] = xp[k];
             }
             park_rows<R>(own_x, xp);
-            seg_radp_p<N, R, G>(sgp, gl, q, s_rad);
+            seg_radp_p<N, R, G, T>(sgp, gl, q, s_rad);
             RunCoef rp = rc;
             rp.dtT_fac = dt * (0.5 * K_C * q.n_gas) * K_GYR;
             const SrcRows sp = {S.qpi ? S.qpi + roff : g.zeros + base, g.zeros + base, 0.0, 0.0, dt};
-            species_segment<false, N, R, G>(gl, sgp, steps, windowed, ls, le, rp, dfac_on, dfac_off, mode_on, own_rad,
+            species_segment<false, N, R, G, T>(gl, sgp, steps, windowed, ls, le, rp, dfac_on, dfac_off, mode_on, own_rad,
                                             sp, own_x, xp, bad);
         }
         if (ELEC && MODE == CK_MODE_EP) {
@@ -1285,9 +1362,9 @@ __device__ __forceinline__ void snapshot_update(const CkGrid &g, const SpGrid &s
 #pragma unroll
             for (int k = 0; k < R; ++k) own_src[k] = dt * src[k];
             park_rows<R>(own_x, xe);
-            seg_radp_e<R>(sge, S, base, nseg, alpha, q, s_grp + gl * (R + 1));
+            seg_radp_e<R, T>(sge, S, base, nseg, alpha, q, (T *)s_grp + gl * (R + 1));
             const SrcRows se = {own_src, g.zeros + base, 0.0, 0.0, 1.0};
-            species_segment<true, N, R, G>(gl, sge, steps, windowed, ls, le, rc, dfac_on, dfac_off, mode_on, own_rad,
+            species_segment<true, N, R, G, T>(gl, sge, steps, windowed, ls, le, rc, dfac_on, dfac_off, mode_on, own_rad,
                                            se, own_x, xe, bad);
         } else if (ELEC) {
             // mode E_SEC: the same source as the in-kernel contraction, with
@@ -1296,15 +1373,15 @@ __device__ __forceinline__ void snapshot_update(const CkGrid &g, const SpGrid &s
             const double *inj =
                 (MODE == CK_MODE_E_SEC) ? S.seg_inje + (size_t)(row0 + iseg) * N + base : g.zeros + base;
             park_rows<R>(own_x, xe);
-            seg_radp_e<R>(sge, S, base, nseg, alpha, q, s_grp + gl * (R + 1));
+            seg_radp_e<R, T>(sge, S, base, nseg, alpha, q, (T *)s_grp + gl * (R + 1));
             const SrcRows se = {S.qepri ? S.qepri + roff : g.zeros + base, inj, K_GYR, q.n_gas, dt};
-            species_segment<true, N, R, G>(gl, sge, steps, windowed, ls, le, rc, dfac_on, dfac_off, mode_on, own_rad,
+            species_segment<true, N, R, G, T>(gl, sge, steps, windowed, ls, le, rc, dfac_on, dfac_off, mode_on, own_rad,
                                            se, own_x, xe, bad);
         }
     }
 }
 
-template <int N, int R, int WPB, int MODE, bool LG, bool PK>
+template <int N, int R, int WPB, int MODE, bool LG, bool PK, typename T>
 __device__ __forceinline__ void k_fused_body(const CkGrid &g, const CkSlot *__restrict__ slots,
                                              const CkTask *__restrict__ tasks, int ntasks,
                                              int *__restrict__ counter, int *__restrict__ err)
@@ -1316,7 +1393,8 @@ __device__ __forceinline__ void k_fused_body(const CkGrid &g, const CkSlot *__re
     __shared__ double s_lds[WPB][GPW][fused_lds_doubles(N, R, MODE, PK)];
     constexpr int NP = N + N / R + 1;
     constexpr int PS = LogG<R>::v;
-    __shared__ double s_tab[LG ? fused_grid_tables(MODE) : 1][LG ? NP : 1];
+    static_assert(LG || sizeof(T) == sizeof(double), "FP32-coefficient instances need their tables in LDS");
+    __shared__ T s_tab[LG ? fused_grid_tables(MODE) : 1][LG ? NP : 1];
     if constexpr (LG) {
         // the launch's grid tables, row j at j + j / R: electron set first
         constexpr int NT = fused_grid_tables(MODE);
@@ -1324,13 +1402,14 @@ __device__ __forceinline__ void k_fused_body(const CkGrid &g, const CkSlot *__re
             const int tb = i / N, j = i - tb * N;
             const bool el = ELEC && tb < GT_NE;
             const int ti = el ? tb : (ELEC ? tb - GT_NE : tb);
-            s_tab[tb][j + (j >> PS)] = sp_table(g, el, ti)[j];
+            s_tab[tb][j + (j >> PS)] = (T)sp_table(g, el, ti)[j];
         }
         __syncthreads();
     }
-    const SpGrid sge = LG ? sp_grid(g, true, &s_tab[0][0], NP, PS) : sp_grid(g, true, nullptr, 0, -1);
-    const SpGrid sgp =
-        LG ? sp_grid(g, false, &s_tab[0][0] + ((LG && ELEC) ? GT_NE * NP : 0), NP, PS) : sp_grid(g, false, nullptr, 0, -1);
+    const SpGridT<T> sge =
+        LG ? sp_grid<T>(g, true, &s_tab[0][0], NP, PS) : sp_grid<T>(g, true, nullptr, 0, -1);
+    const SpGridT<T> sgp = LG ? sp_grid<T>(g, false, &s_tab[0][0] + ((LG && ELEC) ? GT_NE * NP : 0), NP, PS)
+                              : sp_grid<T>(g, false, nullptr, 0, -1);
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int grp = lane / G;
@@ -1360,7 +1439,7 @@ __device__ __forceinline__ void k_fused_body(const CkGrid &g, const CkSlot *__re
             // a task advances one tracer over nslot consecutive snapshots
             for (int sl = task.slot; sl < task.slot + task.nslot; ++sl) {
                 const CkSlot &S = slots[sl];
-                snapshot_update<N, R, MODE, PK>(g, sge, sgp, S, t, gl, s_grp, xe, xp, bad);
+                snapshot_update<N, R, MODE, PK, T>(g, sge, sgp, S, t, gl, s_grp, xe, xp, bad);
                 if (ELEC && S.out_e) {
 #pragma unroll
                     for (int k = 0; k < R; ++k) S.out_e[roff + k] = xe[k];
@@ -1383,12 +1462,14 @@ __device__ __forceinline__ void k_fused_body(const CkGrid &g, const CkSlot *__re
 
 // The kernel: WPB waves per block, register budget set for OCC waves per SIMD
 // (the solve is latency-bound, so the budget trades unrolled state against
-// waves available to hide cross-lane and memory latency).
-template <int N, int R, int WPB, int MODE, int OCC>
+// waves available to hide cross-lane and memory latency).  T: the
+// coefficient type (double; float for the FP32-coefficient instances).
+template <int N, int R, int WPB, int MODE, int OCC, typename T = double>
 __global__ __launch_bounds__(WPB * 64) __attribute__((amdgpu_waves_per_eu(OCC, 8))) void k_fused(
     CkGrid g, const CkSlot *__restrict__ slots, const CkTask *__restrict__ tasks, int ntasks,
     int *__restrict__ counter, int *__restrict__ err)
 {
-    k_fused_body<N, R, WPB, MODE, fused_use_lg(N, R, WPB, MODE, OCC), fused_use_pk(N, R, WPB, MODE, OCC)>(
-        g, slots, tasks, ntasks, counter, err);
+    constexpr int TB = (int)sizeof(T);
+    k_fused_body<N, R, WPB, MODE, fused_use_lg(N, R, WPB, MODE, OCC, TB), fused_use_pk(N, R, WPB, MODE, OCC, TB),
+                 T>(g, slots, tasks, ntasks, counter, err);
 }

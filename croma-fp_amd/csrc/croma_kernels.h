@@ -14,6 +14,8 @@ extern "C" {
 #endif
 
 #define CK_MAX_N 512
+/* added to a fused launch mode: the FP32-coefficient instance */
+#define CK_LAUNCH_F32 4
 #define CK_NBG 8 /* n_gas, kbt, b_field, divv, l_turb, dv, cs, beta_pl */
 
 enum { CK_BG_NGAS = 0, CK_BG_KBT, CK_BG_B, CK_BG_DIVV, CK_BG_LTURB, CK_BG_DV, CK_BG_CS, CK_BG_BETA };
@@ -173,6 +175,7 @@ int ck_grid_loss_tables(CkGrid g, double epmax, void *stream);
 /* Fill kernel (used instead of memset so it is safe under graph capture). */
 int ck_fill_i32(int *dst, int value, size_t count, void *stream);
 int ck_fill_f64(double *dst, double value, size_t count, void *stream);
+int ck_fill_u8(unsigned char *dst, unsigned char value, size_t count, void *stream);
 
 /* Fused path: persistent cost-ordered lane-group kernel.  rows_per_lane is
  * 2, 4, 8 or 16 (n / rows_per_lane lanes per tracer, 4..64); occupancy 1 or 2
@@ -182,26 +185,33 @@ int ck_fill_f64(double *dst, double value, size_t count, void *stream);
  * kernel), 2 protons only recording segment-start rows, 3 electrons only
  * with secondaries from seg_inje; every slot of one launch must agree.  The
  * counter must be zero on entry. */
+/* 1 if the FP32-coefficient instance of this launch shape and mode (0..3) exists */
+int ck_fused_f32_supported(int n, int r, int occ, int mode);
 int ck_launch_fused(int n, int rows_per_lane, int occupancy, int nblocks, int mode,
                     CkGrid grid, const CkSlot *slots_dev, const CkTask *tasks_dev, int ntasks,
                     int *counter_dev, int *err_dev, void *stream);
 int ck_fused_supported(int n, int rows_per_lane);
 
-/* Plain path (one tracer per thread, unfused coefficient kernels, cell-major
- * a,b,c arrays).  idx[count] selects tracers of the slot (a bucket). */
+/* Plain path (one tracer per thread, unfused coefficient kernels, every
+ * per-bin array in HBM, momentum-major: bin j of the bucket's tracer i at
+ * j * count + i).  idx[count] selects tracers of the slot (a bucket). */
 typedef struct {
-    double *radp_p, *tl_p, *radp_e, *dpp_on_p, *dpp_on_e, *dpp_off_p, *dpp_off_e; /* [count*n] */
-    double *a, *b, *c;         /* [4][count*n]: p-on, e-on, p-off, e-off */
-    double *inje;              /* [count*n] */
-    double *cprime;            /* [count*n] */
+    double *radp_p, *tl_p, *radp_e, *dpp_on_p, *dpp_on_e, *dpp_off_p, *dpp_off_e; /* [n][count] */
+    double *a, *b, *c;         /* [4][n][count]: p-on, e-on, p-off, e-off */
+    double *inje;              /* [n][count] */
+    double *cprime;            /* [n][count] */
     double *bg;                /* [CK_NBG][count] interpolated backgrounds */
-    double *ric;               /* [n] interpolated IC row */
+    double *ric;               /* IC cooling rate of bin j, tracer i at j * ric_js + i * ric_is:
+                                * one row [n] (ric_js 1, ric_is 0) or per tracer [n][count] */
+    long ric_js, ric_is;
     double *xp, *xe;           /* [count*n] gathered state */
     double *qp, *qe;           /* [count*n] gathered sources */
     double *mass;              /* [count] */
     unsigned char *dis;        /* [count] */
 } CkPlainWs;
 
+/* out[i] = cur[i] + alpha (nxt[i] - cur[i]) (the reference's lerp), i < count */
+int ck_lerp_f64(size_t count, const double *cur, const double *nxt, double alpha, double *out, void *stream);
 int ck_plain_gather(int n, int count, const int *idx_dev, const CkSlot *slot_host, CkPlainWs ws,
                     void *stream);
 int ck_plain_lerp(int n, int count, const int *idx_dev, const CkSlot *slot_host, double alpha,

@@ -52,12 +52,20 @@ extern "C" int ck_fused_supported(int n, int r)
     return 1;
 }
 
+// Whether the FP32-coefficient instance of (n, r, occ, mode) exists: its
+// float grid tables must fit the LDS beside the lane groups' regions.
+extern "C" int ck_fused_f32_supported(int n, int r, int occ, int mode)
+{
+    if (!ck_fused_supported(n, r) || occ < 1 || occ > 2 || mode < 0 || mode > 3) return 0;
+    return fused_use_lg(n, r, 4, mode, occ, 4) ? 1 : 0;
+}
+
 extern "C" int ck_launch_fused(int n, int r, int occ, int nblocks, int mode, CkGrid grid, const CkSlot *slots,
                                const CkTask *tasks, int ntasks, int *counter, int *err, void *stream)
 {
     int rc = -1;
     if (!ck_fused_supported(n, r) || ntasks < 0 || nblocks <= 0) return -1;
-    if (occ < 1 || occ > 2 || mode < 0 || mode > 3) return -1;
+    if (occ < 1 || occ > 2 || mode < 0 || mode > 7) return -1;
     if (ntasks == 0) return 0;
     switch (n) {
     case 64: rc = ck_launch_fused_n64(r, occ, nblocks, mode, grid, slots, tasks, ntasks, counter, err, stream); break;
@@ -70,7 +78,11 @@ extern "C" int ck_launch_fused(int n, int r, int occ, int nblocks, int mode, CkG
 }
 
 // ---------------------------------------------------------------------------
-// Plain path kernels: one tracer per thread, cell-major arrays in HBM.
+// Plain path kernels: one tracer per thread, unfused coefficient kernels,
+// every per-bin array materialised in HBM.  The workspace is momentum-major
+// ([bin][tracer of the bucket], stride `count`), so the threads of a wave
+// (consecutive tracers) touch consecutive addresses at every bin; gather and
+// scatter transpose from and to the batch's cell-major spectra.
 // ---------------------------------------------------------------------------
 __global__ void k_plain_gather(int n, int count, const int *__restrict__ idx, const double *__restrict__ crp,
                                const double *__restrict__ cre, const double *__restrict__ qpi,
@@ -79,7 +91,7 @@ __global__ void k_plain_gather(int n, int count, const int *__restrict__ idx, co
 {
     const size_t tot = (size_t)count * n;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
-        const int c = (int)(i / n), j = (int)(i % n);
+        const int j = (int)(i / count), c = (int)(i % count);
         const size_t src = (size_t)idx[c] * n + j;
         ws.xp[i] = crp ? crp[src] : 0.0;
         ws.xe[i] = cre[src];
@@ -97,7 +109,7 @@ __global__ void k_plain_scatter(int n, int count, const int *__restrict__ idx, d
 {
     const size_t tot = (size_t)count * n;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
-        const int c = (int)(i / n), j = (int)(i % n);
+        const int j = (int)(i / count), c = (int)(i % count);
         const size_t dst = (size_t)idx[c] * n + j;
         if (crp) crp[dst] = ws.xp[i];
         cre[dst] = ws.xe[i];
@@ -119,21 +131,29 @@ __global__ void k_plain_lerp(int n, int count, const int *__restrict__ idx, BgPt
         for (int f = 0; f < CK_NBG; ++f)
             ws.bg[(size_t)f * count + i] = interp ? d_lerp(bp.c[f][t], bp.x[f][t], alpha) : bp.c[f][t];
     }
-    if (i < n) ws.ric[i] = interp ? d_lerp(ric_c[i], ric_n[i], alpha) : ric_c[i];
+    if (i < n && ric_c) ws.ric[i] = interp ? d_lerp(ric_c[i], ric_n[i], alpha) : ric_c[i];
 }
 
-// losses + D_pp (on / off) + Chang-Cooper assembly, one tracer per thread
+__global__ void k_lerp_f64(size_t count, const double *__restrict__ cur, const double *__restrict__ nxt, double alpha,
+                           double *__restrict__ out)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = d_lerp(cur[i], nxt[i], alpha);
+}
+
+// losses + D_pp (on / off) + Chang-Cooper assembly, one tracer per thread;
+// bin j of tracer i at j * count + i
 __global__ void k_plain_coeff(int n, int count, CkGrid g, CkParams prm, double dt, double t_acc_direct,
                               int protons, CkPlainWs ws, int *err)
 {
 #pragma clang fp contract(off)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
-    const double n_gas = ws.bg[0 * count + i], kbt = ws.bg[1 * count + i], bfield = ws.bg[2 * count + i];
-    const double divv = ws.dis[i] ? 0.0 : ws.bg[3 * count + i];
-    const double lturb = ws.bg[4 * count + i], dv = ws.bg[5 * count + i], cs = ws.bg[6 * count + i];
-    const double beta = ws.bg[7 * count + i];
-    const size_t off = (size_t)i * n;
+    const size_t st = (size_t)count;
+    const double n_gas = ws.bg[0 * st + i], kbt = ws.bg[1 * st + i], bfield = ws.bg[2 * st + i];
+    const double divv = ws.dis[i] ? 0.0 : ws.bg[3 * st + i];
+    const double lturb = ws.bg[4 * st + i], dv = ws.bg[5 * st + i], cs = ws.bg[6 * st + i];
+    const double beta = ws.bg[7 * st + i];
     const double b_micro = bfield * 1.0e+6;
     const double b_ratio = b_micro / 3.2;
     const double synch_coeff = 4.8e-4 * b_ratio * b_ratio * 9.11e-28 * 3.0e+10 * 3.1536e+16;
@@ -141,13 +161,14 @@ __global__ void k_plain_coeff(int n, int count, CkGrid g, CkParams prm, double d
     const double adia = 1.0 / 3.0 * divv;
     const double tl_pref = 1.0 / (0.5 * K_C * n_gas) / K_GYR;
     for (int j = 0; j < n; ++j) {
+        const size_t o = (size_t)j * st + i;
         const double pe = g.ep[j], pe2 = g.e_p2[j];
         const double beta2 = pe2 / (1.0 + pe2);
-        ws.radp_e[off + j] = synch_coeff * pe2 + ws.ric[j] + coul_coeff * (1.0 + log(pe / n_gas) / 74.8) / beta2 +
-                             adia * pe;
+        ws.radp_e[o] = synch_coeff * pe2 + ws.ric[(size_t)j * ws.ric_js + (size_t)i * ws.ric_is] + coul_coeff * (1.0 + log(pe / n_gas) / 74.8) / beta2 +
+                       adia * pe;
         if (protons) {
-            ws.radp_p[off + j] = d_b_coulomb_p(n_gas, g.pp[j], kbt) + adia * g.pp[j];
-            ws.tl_p[off + j] = tl_pref / g.p_sig[j];
+            ws.radp_p[o] = d_b_coulomb_p(n_gas, g.pp[j], kbt) + adia * g.pp[j];
+            ws.tl_p[o] = tl_pref / g.p_sig[j];
         }
     }
     double inv_on = 0.0;
@@ -157,25 +178,26 @@ __global__ void k_plain_coeff(int n, int count, CkGrid g, CkParams prm, double d
         double ecr = 0.0;
         if (protons)
             for (int j = 2; j < n - 2; ++j) {
-                const double w = g.p_ecr_w[j], v = ws.xp[off + j];
+                const double w = g.p_ecr_w[j], v = ws.xp[(size_t)j * st + i];
                 if (w > 0.0 && isfinite(v)) ecr += w * v * g.pdp[j];
             }
         for (int j = 2; j < n - 2; ++j) {
-            const double w = g.e_ecr_w[j], v = ws.xe[off + j];
+            const double w = g.e_ecr_w[j], v = ws.xe[(size_t)j * st + i];
             if (w > 0.0 && isfinite(v)) ecr += w * v * g.edp[j];
         }
         scale = d_eta_scale(prm, prm.dpp_model, lturb, dv, cs, beta, ws.mass[i], t_acc_direct, ecr);
     }
     for (int j = 0; j < n; ++j) {
+        const size_t o = (size_t)j * st + i;
         const double off_e = g.e_p2[j] * (1.0 / (4.0 * 1.0e+5));
         const double on_e = mode_on ? g.e_p2[j] * inv_on * g.e_expcut[j] : off_e;
-        ws.dpp_off_e[off + j] = off_e;
-        ws.dpp_on_e[off + j] = on_e * scale;
+        ws.dpp_off_e[o] = off_e;
+        ws.dpp_on_e[o] = on_e * scale;
         if (protons) {
             const double off_p = g.p_p2[j] * (1.0 / (4.0 * 1.0e+5));
             const double on_p = mode_on ? g.p_p2[j] * inv_on * exp(-g.pE[j] / prm.epmax) * g.p_expcut[j] : off_p;
-            ws.dpp_off_p[off + j] = off_p;
-            ws.dpp_on_p[off + j] = on_p * scale;
+            ws.dpp_off_p[o] = off_p;
+            ws.dpp_on_p[o] = on_p * scale;
         }
     }
     // assembly: sets 0 p-on, 1 e-on, 2 p-off, 3 e-off
@@ -185,24 +207,27 @@ __global__ void k_plain_coeff(int n, int count, CkGrid g, CkParams prm, double d
         if (sp == 0 && !protons) continue;
         const double *xg = sp ? g.ep : g.pp;
         const double *dxg = sp ? g.edp : g.pdp;
-        const double *radp = sp ? ws.radp_e + off : ws.radp_p + off;
-        const double *D = (set < 2) ? (sp ? ws.dpp_on_e + off : ws.dpp_on_p + off)
-                                    : (sp ? ws.dpp_off_e + off : ws.dpp_off_p + off);
-        double *a = ws.a + set * plane + off, *b = ws.b + set * plane + off, *c = ws.c + set * plane + off;
+        const double *radp = sp ? ws.radp_e + i : ws.radp_p + i;
+        const double *D = (set < 2) ? (sp ? ws.dpp_on_e + i : ws.dpp_on_p + i)
+                                    : (sp ? ws.dpp_off_e + i : ws.dpp_off_p + i);
+        double *a = ws.a + set * plane + i, *b = ws.b + set * plane + i, *c = ws.c + set * plane + i;
+        const double *tl = ws.tl_p + i;
         double Wp_prev = 0.0, Wm_prev = 0.0, w_prev = 0.0;
         for (int m = 0; m < n; ++m) {
-            const double Bm = radp[m] - 2.0 / xg[m] * D[m];
+            const size_t om = (size_t)m * st;
+            const double Dm = D[om];
+            const double Bm = radp[om] - 2.0 / xg[m] * Dm;
             double Bl = 0.0, Br = 0.0, xl = 0.0, xr = 0.0, Cl = 0.0, Cr = 0.0;
-            if (m > 0) { Bl = radp[m - 1] - 2.0 / xg[m - 1] * D[m - 1]; Cl = D[m - 1]; xl = xg[m - 1]; }
-            if (m < n - 1) { Br = radp[m + 1] - 2.0 / xg[m + 1] * D[m + 1]; Cr = D[m + 1]; xr = xg[m + 1]; }
+            if (m > 0) { Bl = radp[om - st] - 2.0 / xg[m - 1] * D[om - st]; Cl = D[om - st]; xl = xg[m - 1]; }
+            if (m < n - 1) { Br = radp[om + st] - 2.0 / xg[m + 1] * D[om + st]; Cr = D[om + st]; xr = xg[m + 1]; }
             double Wp = 0.0, Wm = 0.0, w = 0.0;
-            if (m < n - 1) d_face(Bm, Br, D[m], Cr, xg[m], xr, Wp, Wm, w);
+            if (m < n - 1) d_face(Bm, Br, Dm, Cr, xg[m], xr, Wp, Wm, w);
             const int kind = (m == 0) ? 1 : ((m == n - 1) ? 2 : 0);
-            const double dtT = sp ? dt / 1.0e+100 : dt / ws.tl_p[off + m];
+            const double dtT = sp ? dt / 1.0e+100 : dt / tl[om];
             double delta_l = 0.0;
-            if (kind == 2 && !(fabs(D[m]) > 1.0e-200) && fabs(Cl) > 1.0e-200) delta_l = d_face_delta(w_prev);
-            d_cc_row(kind, dt, xg[m], dxg[m], Bm, D[m], dtT, xl, Bl, Cl, Wp_prev, Wm_prev, delta_l, xr, Br, Cr, Wp,
-                     Wm, a[m], b[m], c[m]);
+            if (kind == 2 && !(fabs(Dm) > 1.0e-200) && fabs(Cl) > 1.0e-200) delta_l = d_face_delta(w_prev);
+            d_cc_row(kind, dt, xg[m], dxg[m], Bm, Dm, dtT, xl, Bl, Cl, Wp_prev, Wm_prev, delta_l, xr, Br, Cr, Wp,
+                     Wm, a[om], b[om], c[om]);
             Wp_prev = Wp;
             Wm_prev = Wm;
             w_prev = w;
@@ -215,30 +240,31 @@ __global__ void k_plain_secondary(int n, int count, const double *__restrict__ f
 {
     const size_t tot = (size_t)count * n;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
-        const int c = (int)(i / n), je = (int)(i % n);
+        const int je = (int)(i / count), c = (int)(i % count);
         double q = 0.0;
         if (fqe_t)
-            for (int jp = jp0; jp < n; ++jp) q += ws.xp[(size_t)c * n + jp] * fqe_t[(size_t)jp * n + je];
+            for (int jp = jp0; jp < n; ++jp) q += ws.xp[(size_t)jp * count + c] * fqe_t[(size_t)jp * n + je];
         ws.inje[i] = q * K_GYR * ws.bg[(size_t)c] + ws.qe[i];
     }
 }
 
 // Thomas with the rhs update (solve_cc_batch_cpu_timed semantics) for one
-// system, state in place; cprime scratch in HBM.
+// system of n rows at stride st, state in place; cprime scratch in HBM.
 __device__ __forceinline__ int d_thomas_cc(int n, double dt, const double *a, const double *b, const double *c,
-                                           const double *q, double *x, double *cp)
+                                           const double *q, double *x, double *cp, size_t st = 1)
 {
-    for (int j = 0; j < n; ++j) x[j] = dt * q[j] + x[j];
+    for (int j = 0; j < n; ++j) x[j * st] = dt * q[j * st] + x[j * st];
     if (fabs(b[0]) <= 1.0e-300) return ERR_PIVOT;
     cp[0] = -c[0] / b[0];
     x[0] /= b[0];
     for (int j = 1; j < n; ++j) {
-        const double den = b[j] + a[j] * cp[j - 1];
+        const size_t o = j * st;
+        const double den = b[o] + a[o] * cp[o - st];
         if (fabs(den) <= 1.0e-300) return ERR_PIVOT;
-        cp[j] = (j == n - 1) ? 0.0 : -c[j] / den;
-        x[j] = (x[j] + a[j] * x[j - 1]) / den;
+        cp[o] = (j == n - 1) ? 0.0 : -c[o] / den;
+        x[o] = (x[o] + a[o] * x[o - st]) / den;
     }
-    for (int j = n - 2; j >= 0; --j) x[j] -= cp[j] * x[j + 1];
+    for (int j = n - 2; j >= 0; --j) x[j * st] -= cp[j * st] * x[(j + 1) * st];
     return 0;
 }
 
@@ -247,16 +273,14 @@ __global__ void k_plain_thomas(int n, int count, int nstep, double dt, int windo
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
-    const size_t off = (size_t)i * n, plane = (size_t)count * n;
+    const size_t plane = (size_t)count * n, st = (size_t)count;
     int bad = 0;
     for (int s = 0; s < nstep && !bad; ++s) {
         const int on = !windowed || (s >= on_start && s < on_end);
-        const size_t ps = (on ? 0 : 2) * plane, es = (on ? 1 : 3) * plane;
+        const size_t ps = (on ? 0 : 2) * plane + i, es = (on ? 1 : 3) * plane + i;
         if (protons)
-            bad |= d_thomas_cc(n, dt, ws.a + ps + off, ws.b + ps + off, ws.c + ps + off, ws.qp + off, ws.xp + off,
-                               ws.cprime + off);
-        bad |= d_thomas_cc(n, dt, ws.a + es + off, ws.b + es + off, ws.c + es + off, ws.inje + off, ws.xe + off,
-                           ws.cprime + off);
+            bad |= d_thomas_cc(n, dt, ws.a + ps, ws.b + ps, ws.c + ps, ws.qp + i, ws.xp + i, ws.cprime + i, st);
+        bad |= d_thomas_cc(n, dt, ws.a + es, ws.b + es, ws.c + es, ws.inje + i, ws.xe + i, ws.cprime + i, st);
     }
     if (bad) atomicOr(err, bad);
 }
@@ -294,6 +318,15 @@ extern "C" int ck_plain_scatter(int n, int count, const int *idx, const CkSlot *
     return hip_ok(hipGetLastError(), "k_plain_scatter");
 }
 
+extern "C" int ck_lerp_f64(size_t count, const double *cur, const double *nxt, double alpha, double *out,
+                           void *stream)
+{
+    if (count == 0) return 0;
+    hipLaunchKernelGGL(k_lerp_f64, dim3(grid1d(count, 256)), dim3(256), 0, (hipStream_t)stream, count, cur, nxt,
+                       alpha, out);
+    return hip_ok(hipGetLastError(), "k_lerp_f64");
+}
+
 extern "C" int ck_plain_lerp(int n, int count, const int *idx, const CkSlot *S, double alpha, CkPlainWs ws,
                              void *stream)
 {
@@ -326,7 +359,7 @@ extern "C" int ck_plain_secondary(int n, int count, const double *fqe_t, int jp0
 extern "C" int ck_plain_thomas(int n, int count, int nstep, double dt, int windowed, int on_start, int on_end,
                                int protons, CkPlainWs ws, int *err, void *stream)
 {
-    hipLaunchKernelGGL(k_plain_thomas, dim3(grid1d((size_t)count, 32)), dim3(32), 0, (hipStream_t)stream, n, count,
+    hipLaunchKernelGGL(k_plain_thomas, dim3(grid1d((size_t)count, 64)), dim3(64), 0, (hipStream_t)stream, n, count,
                        nstep, dt, windowed, on_start, on_end, protons, ws, err);
     return hip_ok(hipGetLastError(), "k_plain_thomas");
 }
@@ -384,6 +417,14 @@ extern "C" int ck_fill_i32(int *dst, int v, size_t count, void *stream)
     if (count == 0) return 0;
     hipLaunchKernelGGL(k_fill<int>, dim3(grid1d(count, 256)), dim3(256), 0, (hipStream_t)stream, dst, v, count);
     return hip_ok(hipGetLastError(), "k_fill_i32");
+}
+
+extern "C" int ck_fill_u8(unsigned char *dst, unsigned char v, size_t count, void *stream)
+{
+    if (count == 0) return 0;
+    hipLaunchKernelGGL(k_fill<unsigned char>, dim3(grid1d(count, 256)), dim3(256), 0, (hipStream_t)stream, dst, v,
+                       count);
+    return hip_ok(hipGetLastError(), "k_fill_u8");
 }
 
 extern "C" int ck_fill_f64(double *dst, double v, size_t count, void *stream)

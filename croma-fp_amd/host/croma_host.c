@@ -193,6 +193,9 @@ struct CromaHipContext {
     int pipeline; /* split windows: electron pass beside the next proton pass */
     /* batch runs estimate nsub and order the queue on the device */
     int nsub_device;
+    /* coefficient evaluation in FP32 (losses, D_pp, B, C, face inputs) feeding
+     * the FP64 weights, factorisation and substeps */
+    int coeff_fp32;
     void **retired; /* graph executables replaced while their last launch may still run */
     int nretired, retired_cap;
     /* croma_hip_execute_snapshot state kept between calls */
@@ -443,6 +446,17 @@ int croma_hip_context_get_tile(const CromaHipContext *c, int protons, CromaHipTi
     *t = c->tile[protons ? 1 : 0];
     return 0;
 }
+
+int croma_hip_context_set_coeff_precision(CromaHipContext *c, int precision)
+{
+    if (!c || (precision != CROMA_HIP_COEFF_FP64 && precision != CROMA_HIP_COEFF_FP32)) return CROMA_HIP_EINVAL;
+    ck_stream_sync(c->stream);
+    if (c->stream_e) ck_stream_sync(c->stream_e);
+    c->coeff_fp32 = precision == CROMA_HIP_COEFF_FP32;
+    return 0;
+}
+
+int ch_ctx_coeff_fp32(const CromaHipContext *c) { return c->coeff_fp32; }
 
 int croma_hip_context_set_nsub_device(CromaHipContext *c, int enable)
 {
@@ -1256,9 +1270,15 @@ static int launch_mode(CromaHipContext *c, int protons, int mode, const CkSlot *
     const CromaHipTile *t = &c->tile[protons ? 1 : 0];
     const int nfull = c->cu_count * t->waves_per_simd;
     const int nblocks = shared ? (nfull + 1) / 2 : nfull;
+    if (c->coeff_fp32 && !ck_fused_f32_supported(c->g.n, t->rows_per_lane, t->waves_per_simd, mode)) {
+        ch_set_error("FP32 coefficients: no instance for n=%d, %d rows per lane, %d waves per SIMD, mode %d",
+                     c->g.n, t->rows_per_lane, t->waves_per_simd, mode);
+        return CROMA_HIP_EUNSUPPORTED;
+    }
     if (ck_fill_i32(counter, 0, 1, stream)) return dev_fail("counter reset");
-    if (ck_launch_fused(c->g.n, t->rows_per_lane, t->waves_per_simd, nblocks, mode, c->dgrid, d_slot, d_tasks,
-                        ntasks, counter, c->d_err, stream))
+    if (ck_launch_fused(c->g.n, t->rows_per_lane, t->waves_per_simd, nblocks,
+                        mode + (c->coeff_fp32 ? CK_LAUNCH_F32 : 0), c->dgrid, d_slot, d_tasks, ntasks, counter,
+                        c->d_err, stream))
         return dev_fail("fused launch");
     return 0;
 }
@@ -1350,7 +1370,7 @@ static int launch_window(CromaHipBatch *b, WinBuf *w, WinBuf *prev, int last, Gr
     key[2] = t->rows_per_lane;
     key[3] = t->waves_per_simd;
     key[4] = (long)(size_t)b->d_segrec;
-    key[5] = b->protons;
+    key[5] = b->protons | ((long)c->coeff_fp32 << 1);
     key[6] = c->jp0 | ((long)w->dev_nsub << 20) | ((long)w->count << 21);
     key[7] = b->ntr | ((long)(w->segq == c->gemm_sol_rows ? c->gemm_sol : 0) << 32);
     wi.b = b;
@@ -1426,8 +1446,32 @@ static int ensure_plain_ws(CromaHipContext *c, int count)
     TAKE(mass, (size_t)count * sizeof(double));
     TAKE(dis, (size_t)count);
 #undef TAKE
+    c->pws.ric_js = 1;
+    c->pws.ric_is = 0;
     c->pws_cap = count;
     return 0;
+}
+
+int ch_ctx_plain_ws(CromaHipContext *c, int count, CkPlainWs *out)
+{
+    const int rc = ensure_plain_ws(c, count);
+    if (!rc) *out = c->pws;
+    return rc;
+}
+
+void *ch_ctx_stream(const CromaHipContext *c) { return c->stream; }
+int *ch_ctx_d_err(const CromaHipContext *c) { return c->d_err; }
+
+/* Reads and clears the context's error word (after the stream drains):
+ * CROMA_HIP_EPIVOT / CROMA_HIP_ENONFINITE or 0. */
+int ch_ctx_take_err(CromaHipContext *c)
+{
+    int err = 0;
+    if (ck_d2h(&err, c->d_err, sizeof(int), c->stream) || ck_stream_sync(c->stream)) return dev_fail("error word");
+    if (!err) return 0;
+    if (ck_fill_i32(c->d_err, 0, 1, c->stream) || ck_stream_sync(c->stream)) return dev_fail("error word");
+    ch_set_error((err & 1) ? "Thomas pivot below 1e-300" : "non-finite spectrum");
+    return (err & 1) ? CROMA_HIP_EPIVOT : CROMA_HIP_ENONFINITE;
 }
 
 /* LSD radix sort of (key, index) pairs on 64-bit keys, 16 bits per pass. */
@@ -1980,16 +2024,29 @@ static double at_normal(unsigned long long *s)
     return sqrt(-2.0 * log(u1)) * cos(2.0 * 3.14159265358979323846 * u2);
 }
 
+/* Whether every fused launch mode a tile runs has an FP32-coefficient
+ * instance (electrons: mode 0; two species: the split passes 2 and 3, or the
+ * in-kernel contraction 1, which the eta cap also selects). */
+static int tile_f32_ok(const CromaHipContext *c, int protons, int r, int w, int g)
+{
+    const int n = c->g.n;
+    if (!protons) return ck_fused_f32_supported(n, r, w, 0);
+    if (!g || c->prm.eta_dpp_cap > 0.0) return ck_fused_f32_supported(n, r, w, 1);
+    return ck_fused_f32_supported(n, r, w, 2) && ck_fused_f32_supported(n, r, w, 3);
+}
+
 /* Candidate tiles of the autotuners: rows per lane x waves per SIMD (x the
  * split secondary contraction for two species), one snapshot per task. */
-static int tile_candidates(int n, int protons, CromaHipTile *out)
+static int tile_candidates(const CromaHipContext *c, int protons, CromaHipTile *out)
 {
     static const int rows[4] = {2, 4, 8, 16};
+    const int n = c->g.n;
     int r, w, g, k = 0;
     for (g = protons ? 1 : 0; g >= 0; g--)
         for (r = 0; r < 4; r++) {
             if (!ck_fused_supported(n, rows[r])) continue;
             for (w = 1; w <= 2; w++) {
+                if (c->coeff_fp32 && !tile_f32_ok(c, protons, rows[r], w, g)) continue;
                 out[k].rows_per_lane = rows[r];
                 out[k].waves_per_simd = w;
                 out[k].chain = 1;
@@ -2028,7 +2085,7 @@ int croma_hip_batch_autotune(CromaHipBatch *b, int s, int nsnap_total, const dou
         rc = dev_fail("autotune state copy");
         goto out;
     }
-    nc = tile_candidates(c->g.n, b->protons, cand);
+    nc = tile_candidates(c, b->protons, cand);
     for (k = 0; k < nc; k++) {
         double ms = 0.0;
         c->tile[b->protons ? 1 : 0] = cand[k];
@@ -2164,7 +2221,7 @@ int croma_hip_autotune(CromaHipContext *c, int protons, int ntracer, CromaHipTil
         for (f = 0; f < CK_NBG; f++) fl[f] = bg + ((size_t)s * CK_NBG + f) * ntracer;
         if ((rc = croma_hip_batch_set_background(b, s, fl)) != 0) goto out;
     }
-    nc = tile_candidates(n, protons, cand);
+    nc = tile_candidates(c, protons, cand);
     for (k = 0; k < nc; k++) {
         CromaHipTimes tm;
         c->tile[protons ? 1 : 0] = cand[k];

@@ -59,9 +59,14 @@ const CkGrid *ch_ctx_dgrid(const CromaHipContext *c);
 const CkParams *ch_ctx_prm(const CromaHipContext *c);
 int ch_ctx_cu_count(const CromaHipContext *c);
 CromaHipTile ch_ctx_tile(const CromaHipContext *c, int protons);
+int ch_ctx_coeff_fp32(const CromaHipContext *c); /* FP32 coefficient evaluation selected */
 int ch_ctx_fqe(CromaHipContext *c);              /* builds the secondary weights once */
 const double *ch_ctx_d_fqe(const CromaHipContext *c);
 int ch_ctx_jp0(const CromaHipContext *c);
+int ch_ctx_plain_ws(CromaHipContext *c, int count, CkPlainWs *out); /* plain-path workspace of >= count tracers */
+void *ch_ctx_stream(const CromaHipContext *c);
+int *ch_ctx_d_err(const CromaHipContext *c);
+int ch_ctx_take_err(CromaHipContext *c); /* reads and clears the error word: 0, EPIVOT or ENONFINITE */
 void ch_ctx_ic_ends(CromaHipContext *c, double z, double ends[2]); /* IC rate at bins 0, n-1 (cached) */
 int ch_cost_bucket(double cost);                 /* log-scale bucket in [0, CH_HIST) */
 

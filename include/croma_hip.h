@@ -141,6 +141,19 @@ int croma_hip_context_set_pipeline(CromaHipContext *ctx, int enable);
  * queue by cost and assign the split segment rows on the device, without a
  * host pass over the tracers; 0: on the host (the same integers). */
 int croma_hip_context_set_nsub_device(CromaHipContext *ctx, int enable);
+/* Precision of the coefficient evaluation of the fused path (the FP32 mode of
+ * north-star subsystem (a); the reference evaluates FP_Coef.c:438-849 and
+ * fp_shared_core.c:729-1081 in FP64).  CROMA_HIP_COEFF_FP64 (default):
+ * spectra within 1e-10 of the reference CPU path.  CROMA_HIP_COEFF_FP32: loss
+ * rates, D_pp rows, the drift B and diffusion C of every row and the face
+ * averages and Peclet numbers in FP32, with the Bernoulli face weights, the
+ * rows a, b, c, the Thomas factorisation and every substep in FP64; spectra
+ * then carry the FP32 coefficients' relative error (measured against the
+ * FP64 oracle in the tests).  Applies to batch runs and the drop-in; the plain
+ * path stays FP64. */
+#define CROMA_HIP_COEFF_FP64 0
+#define CROMA_HIP_COEFF_FP32 1
+int croma_hip_context_set_coeff_precision(CromaHipContext *ctx, int precision);
 /* Grid arrays (momentum_bin, FP_Coef.c:291-330); species 0 proton, 1 electron. */
 int croma_hip_context_grid(const CromaHipContext *ctx, int species, double *p, double *dp, double *E,
                            double *pm1, double *pp1);
