@@ -187,6 +187,8 @@ int ck_fill_u8(unsigned char *dst, unsigned char value, size_t count, void *stre
  * counter must be zero on entry. */
 /* 1 if the FP32-coefficient instance of this launch shape and mode (0..3) exists */
 int ck_fused_f32_supported(int n, int r, int occ, int mode);
+/* LDS layout check of a fused instance (tb: 8 FP64 / 4 FP32 tables); 0 if in bounds */
+int ck_fused_lds_check(int n, int r, int occ, int mode, int tb, long *info);
 int ck_launch_fused(int n, int rows_per_lane, int occupancy, int nblocks, int mode,
                     CkGrid grid, const CkSlot *slots_dev, const CkTask *tasks_dev, int ntasks,
                     int *counter_dev, int *err_dev, void *stream);

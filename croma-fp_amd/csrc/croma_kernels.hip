@@ -52,6 +52,46 @@ extern "C" int ck_fused_supported(int n, int r)
     return 1;
 }
 
+// Host-side check of one fused instance's LDS use.  Per lane group the
+// slice of fused_lds_doubles(N, R, MODE, PK) doubles holds, in this order,
+// the segment's loss rates (row j at j + j / R; a lane reads its own rows
+// at gl (R + 1) + k), the parked spectrum (PK; rows at gl (R + 1) + k), the
+// EP rows (MODE 1; row j at j) and the 2 CK_NBG backgrounds.  info[0..5]:
+// largest index of each region's accesses (rates, own rates, parked, EP,
+// backgrounds) and the slice size; info[6] the bytes a CU holds for the
+// blocks its register budget admits.  Returns 0 when every index lies in the
+// slice and the blocks fit the 160 KiB of LDS.
+extern "C" int ck_fused_lds_check(int n, int r, int occ, int mode, int tb, long *info)
+{
+    if (!ck_fused_supported(n, r) || occ < 1 || occ > 2 || mode < 0 || mode > 3 || !(tb == 4 || tb == 8)) return -1;
+    const int WPB = 4, G = n / r;
+    const bool LG = fused_use_lg(n, r, WPB, mode, occ, tb);
+    const bool PK = fused_use_pk(n, r, WPB, mode, occ, tb);
+    const long D = fused_lds_doubles(n, r, mode, PK);
+    const long nrates = n + n / r + 1;
+    const long rates_max = (long)(n - 1) + (n - 1) / r;
+    const long own_rates_max = (long)(G - 1) * (r + 1) + r - 1;
+    const long pk_base = nrates;
+    const long parked_max = PK ? pk_base + (long)(G - 1) * (r + 1) + r - 1 : -1;
+    const long row_base = pk_base + (PK ? (long)G * (r + 1) : 0);
+    const long ep_max = (mode == CK_MODE_EP) ? row_base + n - 1 : -1;
+    const long bg_max = row_base + (mode == CK_MODE_EP ? n : 0) + 2 * CK_NBG - 1;
+    const long blocks = (occ * 4 / WPB > 0) ? occ * 4 / WPB : 1;
+    const long bytes = (8L * WPB * (64 / G) * D + (LG ? (long)tb * fused_grid_tables(mode) * nrates : 0)) * blocks;
+    if (info) {
+        info[0] = rates_max;
+        info[1] = own_rates_max;
+        info[2] = parked_max;
+        info[3] = ep_max;
+        info[4] = bg_max;
+        info[5] = D;
+        info[6] = bytes;
+    }
+    const bool ok = rates_max < nrates && own_rates_max < nrates && parked_max < row_base && ep_max < D &&
+                    bg_max < D && bytes <= 160L * 1024 && (LG || tb == 8);
+    return ok ? 0 : 1;
+}
+
 // Whether the FP32-coefficient instance of (n, r, occ, mode) exists: its
 // float grid tables must fit the LDS beside the lane groups' regions.
 extern "C" int ck_fused_f32_supported(int n, int r, int occ, int mode)

@@ -15,6 +15,7 @@ plain one-tracer-per-thread path gives the plain rate.
 from __future__ import annotations
 
 import math
+import sys
 import threading
 import time
 
@@ -129,6 +130,9 @@ def run_stream(chm=ch, count=1000, seconds=240.0, seed=3, rate=None, budget_gb=6
             th.start()
             waiters.append(th)
             t_next += rng.exponential(1.0 / rate)
+            if len(waiters) % 100 == 0:
+                print(f"[serve] {len(waiters)} chunks submitted, {len(done)} done, "
+                      f"{time.perf_counter() - t_start:.1f} s", file=sys.stderr, flush=True)
         stop.set()
         while producer.is_alive():
             try:
