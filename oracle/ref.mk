@@ -18,8 +18,14 @@ CFLAGS_REF := -O2 -std=gnu11 -fPIC -fopenmp -ffp-contract=off -w \
               -ffunction-sections -fdata-sections -I$(REF)/include
 
 LIBS := $(foreach n,$(SIZES),$(OUT)/libcroma_ref_n$(n).so)
+LAYOUTS := $(foreach n,$(SIZES),$(OUT)/ref_layout_n$(n))
 
-all: $(LIBS)
+all: $(LIBS) $(LAYOUTS)
+
+# struct layouts of the reference's GPU-backend headers (headers only)
+$(OUT)/ref_layout_n%: $(HERE)ref_layout.c
+	@mkdir -p $(OUT)
+	gcc -std=gnu11 -w -Dnp=$* -Dnpe=$* -I$(REF)/include -o $@ $<
 
 $(OUT)/libcroma_ref_n%.so: $(SRCS) $(HERE)ref_harness.c
 	@mkdir -p $(OUT)
