@@ -399,7 +399,7 @@ def emission_bench(ch, ntracer=1_000_000, n=256, nbins=64, seed=11):
     return out
 
 
-def served_latency(ch, chunks=1000, seconds=240.0, seed=3, compare_chunks=96, plain_chunks=12, plain_max_tracers=8192):
+def served_latency(ch, chunks=1000, seconds=240.0, seed=3, compare_chunks=48, plain_chunks=6, plain_max_tracers=2048):
     """Poisson stream of mixed chunks through the serving loop, then the
     server's closed-loop capacity on the stream's first chunks against the
     same chunks run one at a time through the batch API; and, on the first
@@ -411,7 +411,7 @@ def served_latency(ch, chunks=1000, seconds=240.0, seed=3, compare_chunks=96, pl
     progress(f"served: {out['chunks']} chunks, p50 {out['p50_ms']:.1f} ms, p99 {out['p99_ms']:.1f} ms")
     specs = serve_stream.chunk_specs(compare_chunks, seed)
     out["closed_loop"] = serve_stream.run_closed_loop(specs)
-    out["sequential_fused"] = serve_stream.run_sequential(specs, path=ch.PATH_FUSED)
+    out["sequential_fused"] = serve_stream.run_sequential(specs, path=ch.PATH_FUSED, verbose=True)
     out["served_vs_sequential_fused"] = (out["closed_loop"]["throughput_substeps_per_s"] /
                                          out["sequential_fused"]["throughput_substeps_per_s"])
     progress(f"served closed loop vs sequential fused: {out['served_vs_sequential_fused']:.3f}")
@@ -419,7 +419,7 @@ def served_latency(ch, chunks=1000, seconds=240.0, seed=3, compare_chunks=96, pl
              if s["ntracer"] <= plain_max_tracers][:plain_chunks]
     ps = {"chunks": len(small), "max_tracers": plain_max_tracers,
           "closed_loop": serve_stream.run_closed_loop(small),
-          "sequential_plain": serve_stream.run_sequential(small, path=ch.PATH_PLAIN)}
+          "sequential_plain": serve_stream.run_sequential(small, path=ch.PATH_PLAIN, verbose=True)}
     ps["served_vs_plain"] = (ps["closed_loop"]["throughput_substeps_per_s"] /
                              ps["sequential_plain"]["throughput_substeps_per_s"])
     out["plain_set"] = ps

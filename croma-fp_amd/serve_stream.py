@@ -174,13 +174,13 @@ def run_closed_loop(specs, budget_gb=64.0):
             "wall_s": wall}
 
 
-def run_sequential(specs, path=ch.PATH_FUSED, params=None):
+def run_sequential(specs, path=ch.PATH_FUSED, params=None, verbose=False):
     """The same chunks one at a time through the batch API (no overlap)."""
     params = params or server_params()
     ctxs = {}
     steps, wall = 0, 0.0
     try:
-        for spec in specs:
+        for i, spec in enumerate(specs):
             w, kw = make_chunk(spec)
             if w.n not in ctxs:
                 ctxs[w.n] = ch.Context(w.n, params)
@@ -196,8 +196,12 @@ def run_sequential(specs, path=ch.PATH_FUSED, params=None):
             t = b.run(0, k, k + 1, w.z, w.dt_snap, path=path)
             b.get_state()
             b.close()
-            wall += time.perf_counter() - t0
+            dt = time.perf_counter() - t0
+            wall += dt
             steps += t.cell_steps
+            if verbose:
+                print(f"[serve] sequential {'plain' if path == ch.PATH_PLAIN else 'fused'} chunk {i + 1}/{len(specs)}: "
+                      f"{w.ntracer} tracers x {w.n} bins, {dt:.2f} s", file=sys.stderr, flush=True)
     finally:
         for c in ctxs.values():
             c.close()
