@@ -382,7 +382,8 @@ def emission_bench(ch, ntracer=1_000_000, n=256, nbins=64, seed=11):
                                                fx["fx"], float(fx["xmin"][0]), logy.ravel(), pw)
         bfield = 10.0 ** rng.uniform(-8.0, -5.0, ntracer)
         res["synch"] = {}
-        for mname, mode in (("fp64", ch.EMIS_FP64), ("fp32", ch.EMIS_FP32)):
+        for mname, mode in (("fp64", ch.EMIS_FP64), ("fp32", ch.EMIS_FP32),
+                            ("mfma_split_bf16x3", ch.EMIS_MFMA_XF32)):
             ms = []
             for _ in range(2):
                 _, t = ctx.emission_synch(spec, bfield, table, nbins, nlogb, logb_min, inv_dlogb, mode=mode)
@@ -391,6 +392,11 @@ def emission_bench(ch, ntracer=1_000_000, n=256, nbins=64, seed=11):
             res["synch"][mname] = {"ms": t, "hbm_gbs": (byts + 8.0 * ntracer) / (t * 1e-3) / 1e9,
                                    "hbm_frac": (byts + 8.0 * ntracer) / (t * 1e-3) / 1e9 / HBM_PEAK_GBS,
                                    "table_build_ms": ms_tab}
+            if mode == ch.EMIS_MFMA_XF32:
+                # the device time includes the per-call Chebyshev coefficient
+                # tables of the 511 plane intervals (a per-table cost)
+                res["synch"][mname]["includes"] = "Chebyshev coefficient tables, plane sort, contraction"
+                res["synch"][mname]["mfma_tflops_issued"] = 3.0 * 2.0 * n * nbins * 8 * ntracer / (t * 1e-3) / 1e12
         out["kernels"] = res
         out["bytes_per_call"] = byts
         out["flops_per_call"] = flops

@@ -21,6 +21,7 @@ PATH_FUSED, PATH_PLAIN = 0, 1
 SPECIES_ELECTRON, SPECIES_PROTON, SPECIES_ELECTRON_PROTON = 1, 2, 3
 DPP_ASA, DPP_TTD, DPP_OFF, DPP_DIRECT = 1, 2, 3, 4
 EMIS_FP32, EMIS_MFMA_XF32, EMIS_MFMA_BF16, EMIS_FP64 = 0, 1, 2, 3
+TABLE_GAMMA, TABLE_NEUTRINO, TABLE_IC = 0, 1, 2
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -169,6 +170,12 @@ class Context:
             lib().croma_hip_context_destroy(C.c_void_p(self.h))
             self.h = None
 
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
     def __del__(self):
         try:
             self.close()
@@ -295,6 +302,16 @@ class Context:
     def set_nsub_device(self, enable):
         """True: batch runs estimate nsub and order the queue on the device."""
         _chk(lib().croma_hip_context_set_nsub_device(C.c_void_p(self.h), int(bool(enable))), "set_nsub_device")
+
+    def emission_kernel_table(self, kind, nbins, lo, hi, z=0.0):
+        """Gamma / neutrino / IC kernel rows [nbins, n] built on the device
+        (kind TABLE_GAMMA, TABLE_NEUTRINO or TABLE_IC); returns (table, device ms)."""
+        table = np.zeros(int(nbins) * self.n)
+        ms = C.c_double()
+        _chk(lib().croma_hip_emission_kernel_table(C.c_void_p(self.h), int(kind), int(nbins), C.c_double(lo),
+                                                   C.c_double(hi), C.c_double(z), _p(table), C.byref(ms)),
+             "emission_kernel_table")
+        return table.reshape(int(nbins), self.n), ms.value
 
     def secondary_weights(self):
         fqe = np.zeros(self.n * self.n)

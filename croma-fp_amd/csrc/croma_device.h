@@ -725,9 +725,12 @@ template <int R, int G>
 __device__ __forceinline__ void apply_solve(int gl, const Factor<R, G> &F, double (&x)[R])
 {
     double t[R];
-    double L = 0.0;
+    // the lane-local prefixes start from the first row itself (al * 0 + t is
+    // t exactly for the finite multipliers of a factorised run)
+    t[0] = fma(F.iv[0], x[0], F.vs[0]);
+    double L = t[0];
 #pragma unroll
-    for (int k = 0; k < R; ++k) {
+    for (int k = 1; k < R; ++k) {
         t[k] = fma(F.iv[k], x[k], F.vs[k]);
         L = fma(F.al[k], L, t[k]);
     }
@@ -755,9 +758,9 @@ __device__ __forceinline__ void apply_solve(int gl, const Factor<R, G> &F, doubl
         d = fma(F.al[k], d, t[k]);
         t[k] = d;
     }
-    double M = 0.0;
+    double M = t[R - 1];
 #pragma unroll
-    for (int k = R - 1; k >= 0; --k) M = fma(F.g[k], M, t[k]);
+    for (int k = R - 2; k >= 0; --k) M = fma(F.g[k], M, t[k]);
     scan_bwd<G, 0, Factor<R, G>::LV>(F.Qb, M);
     if constexpr (G == 32) {
         // rows 0 and 2 take the value at the first lane of the row after

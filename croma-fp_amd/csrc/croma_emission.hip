@@ -747,13 +747,16 @@ __global__ __launch_bounds__(256) void k_emis_synch32(int ntr, int n, int nfreq,
                                                       double inv_dlogb, const double *__restrict__ table,
                                                       const float *__restrict__ lt, const int *__restrict__ perm,
                                                       const double *__restrict__ cre, const double *__restrict__ dp,
-                                                      const double *__restrict__ b_field, double *__restrict__ eps)
+                                                      const double *__restrict__ b_field, double *__restrict__ eps,
+                                                      const int *__restrict__ count)
 {
     __shared__ float s_lw[4][512];
     __shared__ float s_sg[4][512];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int slot = blockIdx.x * 4 + w;
-    if (slot >= ntr) return;
+    // count (device): the number of tracers listed in perm, when the list
+    // was made on the device
+    if (slot >= (count ? *count : ntr)) return;
     const int it = perm ? perm[slot] : slot;
     const double *cl = cre + (size_t)it * n;
     int jend = n - 1;
@@ -838,6 +841,414 @@ __global__ __launch_bounds__(256) void k_emis_synch32(int ntr, int n, int nfreq,
     }
 }
 
+// ---------------------------------------------------------------------------
+// Synchrotron on MFMA.  Between table planes p and p + 1 the reference
+// interpolates the kernel log-linearly in log B where both entries are
+// positive: K(t) = K0^(1-t) K1^t = sqrt(K0 K1) e^(b s), s = 2 t - 1,
+// b = ln(K1 / K0) / 2, else linearly: K = (K0 + K1) / 2 + s (K1 - K0) / 2.
+// With the Chebyshev expansion e^(b s) = I_0(b) + 2 sum_m I_m(b) T_m(s)
+// (I_m: modified Bessel functions) truncated after T_7, every tracer of plane
+// interval p has
+//   eps_f = A(B) sum_m T_m(s) sum_j N_j [dp_j C_m(p, j, f)],
+// a GEMM of the plane's spectra against 8 columns per frequency followed by
+// an 8-term sum.  The truncation error is 2 I_8(b) / I_0(b) < 3e-6 for
+// |b| < 1.4; b = (d ln K / d ln x) (Delta ln B) / 2 with d ln K / d ln x ~ x
+// in the exponential tail, so with table planes no coarser than 0.02 in
+// log10 B it holds for every entry with K above e^-57 of the kernel's peak
+// (the entries that reach emissivities within 25 decades of a tracer's
+// peak).  Entries with |b| > 30 (ratios beyond e^60 between adjacent planes,
+// far out in the tail) use the linear form.
+// ---------------------------------------------------------------------------
+#define EM_SY_M 8 // Chebyshev terms per (plane interval, bin, frequency)
+// Both operands are scaled so that their largest entry is 2^59 (not 1): a
+// product of the largest entries is then 2^118 and a sum of 256 of them stays
+// below FP32's 2^128, while products down to 2^-126 (244 binary orders, 73
+// decades below the largest) stay normal.  With unit maxima the products of
+// a spectrum's cutoff bins (N_j ~ 1e-38 of the row's largest) with the
+// kernel's entries would leave FP32's range although they carry frequencies
+// within 25 decades of the tracer's peak emissivity.
+#define EM_SY_OFF 59
+
+// I_0(b) .. I_7(b) by their power series (|b| <= 30).
+__device__ __forceinline__ void em_bessel_i8(double b, double (&I)[EM_SY_M])
+{
+    const double h = 0.5 * b, h2 = h * h;
+    double pm = 1.0; // h^m / m!
+#pragma unroll
+    for (int m = 0; m < EM_SY_M; ++m) {
+        if (m > 0) pm *= h / (double)m;
+        double term = pm, sum = pm;
+        for (int k = 1; k < 96; ++k) {
+            term *= h2 / ((double)k * (double)(k + m));
+            sum += term;
+            if (fabs(term) <= 1.0e-17 * fabs(sum)) break;
+        }
+        I[m] = sum;
+    }
+}
+
+// Chebyshev coefficients of the interpolated kernel at one entry, times dp.
+__device__ __forceinline__ void em_synch_cheb(double K0, double K1, double w, double (&c)[EM_SY_M])
+{
+#pragma unroll
+    for (int m = 0; m < EM_SY_M; ++m) c[m] = 0.0;
+    if (K0 > 0.0 && K1 > 0.0) {
+        const double b = 0.5 * log(K1 / K0);
+        if (fabs(b) <= 30.0) {
+            double I[EM_SY_M];
+            em_bessel_i8(b, I);
+            const double base = sqrt(K0) * sqrt(K1) * w;
+            c[0] = base * I[0];
+#pragma unroll
+            for (int m = 1; m < EM_SY_M; ++m) c[m] = 2.0 * base * I[m];
+            return;
+        }
+    }
+    c[0] = 0.5 * (K0 + K1) * w;
+    c[1] = 0.5 * (K1 - K0) * w;
+}
+
+// Pass 0: the largest binary exponent of every column (p, f * 8 + m) over its
+// bins (atomic max); pass 1: the entries scaled by their column's power of
+// two, split into bf16 hi / lo.  One thread per (p, f, j); bins 0 and 1 are
+// zero (the sum starts at bin 2).  kt [p][f * 8 + m][j], col_shift [p][f * 8 + m].
+__global__ __launch_bounds__(256) void k_synch_cheb(int pass, int n, int nfreq, int nlogb,
+                                                    const double *__restrict__ table, const double *__restrict__ dp,
+                                                    uint16_t *__restrict__ kt_hi, uint16_t *__restrict__ kt_lo,
+                                                    int *__restrict__ col_shift)
+{
+    const size_t o = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t tot = (size_t)(nlogb - 1) * nfreq * n;
+    if (o >= tot) return;
+    const int j = (int)(o % n);
+    const int f = (int)((o / n) % nfreq);
+    const int pl = (int)(o / ((size_t)n * nfreq));
+    double c[EM_SY_M];
+    if (j < 2) {
+#pragma unroll
+        for (int m = 0; m < EM_SY_M; ++m) c[m] = 0.0;
+    } else {
+        em_synch_cheb(table[((size_t)pl * nfreq + f) * n + j], table[((size_t)(pl + 1) * nfreq + f) * n + j], dp[j],
+                      c);
+    }
+    int *cs = col_shift + ((size_t)pl * nfreq + f) * EM_SY_M;
+    if (pass == 0) {
+#pragma unroll
+        for (int m = 0; m < EM_SY_M; ++m) {
+            const double v = fabs(c[m]);
+            if (v > 0.0 && isfinite(v)) atomicMax(&cs[m], __builtin_amdgcn_frexp_exp(v));
+        }
+        return;
+    }
+#pragma unroll
+    for (int m = 0; m < EM_SY_M; ++m) {
+        const int sh = cs[m] < -2000 ? 0 : cs[m] - EM_SY_OFF;
+        const float v = (float)ldexp(isfinite(c[m]) ? c[m] : 0.0, -sh);
+        const uint16_t h = em_bf16_bits(v);
+        const size_t g = (((size_t)pl * nfreq + f) * EM_SY_M + m) * n + j;
+        kt_hi[g] = h;
+        kt_lo[g] = em_bf16_bits(v - em_bf16_val(h));
+    }
+}
+
+// Per tracer: the end of its sum (first bin whose successor is not positive,
+// as k_emis_synch32), s = 2 t - 1 of its field in its plane interval, the
+// prefactor A(B) and the largest binary exponent of the bins before the end
+// (the contraction's row scale; EM_SY_NONE for an empty row).  One wave per
+// tracer.
+#define EM_SY_NONE (-100000)
+__global__ __launch_bounds__(256) void k_synch_prep(int ntr, int n, int nlogb, double logb_min, double inv_dlogb,
+                                                    const double *__restrict__ cre,
+                                                    const double *__restrict__ b_field, int *__restrict__ jend_out,
+                                                    double *__restrict__ s_out, double *__restrict__ pref_out,
+                                                    int *__restrict__ srow_out)
+{
+    const int lane = threadIdx.x & 63;
+    const int it = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (it >= ntr) return;
+    const double *cl = cre + (size_t)it * n;
+    int jend = n - 1;
+    for (int base = 2; base < n - 1; base += 64) {
+        const int je = base + lane;
+        const bool stop = je < n - 1 && !(cl[je + 1] > 0.0);
+        const unsigned long long m = __ballot(stop);
+        if (m) {
+            jend = base + __ffsll((long long)m) - 1;
+            break;
+        }
+    }
+    int e = EM_SY_NONE;
+    for (int k = lane; k < jend; k += 64) {
+        const double v = fabs(cl[k]);
+        if (v > 0.0 && isfinite(v)) e = max(e, __builtin_amdgcn_frexp_exp(v));
+    }
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) e = max(e, __shfl_xor(e, o, 64));
+    if (lane == 0) {
+        double bval = b_field[it];
+        if (!(bval > 0.0) || !isfinite(bval)) bval = 1.0e-30;
+        const double u = (log10(bval / 1.0e-6) - logb_min) * inv_dlogb;
+        int ib = (int)floor(u);
+        if (ib < 0) ib = 0;
+        if (ib > nlogb - 2) ib = nlogb - 2;
+        double tt = u - (double)ib;
+        if (tt < 0.0) tt = 0.0;
+        if (tt > 1.0) tt = 1.0;
+        srow_out[it] = e;
+        jend_out[it] = jend;
+        s_out[it] = 2.0 * tt - 1.0;
+        pref_out[it] = sqrt(3.0) * pow(4.8e-10, 3.0) * b_field[it] / (2.0 * 9.11e-28 * 2.999e10 * 2.999e10) /
+                       (4.0 * 3.14159265358979323846);
+    }
+}
+
+// The plane-grouped contraction: block (plane interval p, column block cb)
+// keeps the 64 columns cb * 64 .. (8 frequencies x 8 Chebyshev terms) of
+// plane p in LDS and streams the 16-tracer tiles of p's tracers (perm order,
+// p's range [hist[p - 1], hist[p])) as k_emis_stream does, the rows masked
+// past each tracer's end bin.  Blocks are numbered so that the column
+// blocks of one plane share an XCD (blockIdx % 8), whose L2 then serves the
+// plane's spectra to all of them.  Output: eps[i][f] = A_i sum_m T_m(s_i) G.
+template <int KS>
+__global__ __launch_bounds__(256) void k_synch_mfma(int nfreq, int nlogb, int ncb, const double *__restrict__ cre,
+                                                    const int *__restrict__ perm, const int *__restrict__ plane_end,
+                                                    const int *__restrict__ jend, const double *__restrict__ sv,
+                                                    const double *__restrict__ pref,
+                                                    const uint16_t *__restrict__ kt_hi,
+                                                    const uint16_t *__restrict__ kt_lo,
+                                                    const int *__restrict__ col_shift, double *__restrict__ eps)
+{
+    constexpr int K = 32 * KS;
+    __shared__ __attribute__((aligned(16))) uint16_t b_hi[KS][4][EM_TB][8];
+    __shared__ __attribute__((aligned(16))) uint16_t b_lo[KS][4][EM_TB][8];
+    const int xcd = blockIdx.x & 7, loc = blockIdx.x >> 3;
+    const int cb = loc % ncb;
+    const int pl = (loc / ncb) * 8 + xcd;
+    if (pl >= nlogb - 1) return;
+    const int start = pl ? plane_end[pl - 1] : 0, cnt = plane_end[pl] - start;
+    if (cnt <= 0) return;
+    const int ncol = nfreq * EM_SY_M;
+    const int t = threadIdx.x;
+    const int lane = t & 63, w = t >> 6;
+    const size_t pbase = (size_t)pl * ncol;
+    for (int e = t; e < KS * 4 * EM_TB * 8; e += 256) {
+        const int sl = e & 7, c = (e >> 3) % EM_TB, q = (e / (8 * EM_TB)) & 3, ks = e / (32 * EM_TB);
+        const int k = 32 * ks + 8 * (sl >> 1) + 2 * q + (sl & 1);
+        const int col = cb * EM_TB + c;
+        const size_t g = (pbase + (col < ncol ? col : 0)) * K + k;
+        b_hi[ks][q][c][sl] = col < ncol ? kt_hi[g] : (uint16_t)0;
+        b_lo[ks][q][c][sl] = col < ncol ? kt_lo[g] : (uint16_t)0;
+    }
+    __syncthreads();
+    const int r = lane & 15, q = lane >> 4;
+    const int ntiles = (cnt + 15) / 16;
+    constexpr int KC = KS >= 4 ? 4 : (KS >= 2 ? 2 : 1);
+    constexpr int NCH = KS / KC;
+    constexpr int NONE = -100000;
+    for (int tile = w; tile < ntiles; tile += 4) {
+        const bool live = tile * 16 + r < cnt;
+        const int it = live ? perm[start + tile * 16 + r] : 0;
+        const int je = live ? jend[it] : 0;
+        const double *row = cre + (size_t)it * K + 2 * q;
+        floatx4 acc[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[c] = floatx4{0.f, 0.f, 0.f, 0.f};
+        int s_row = NONE;
+        double2 nxt[KC][4];
+#pragma unroll
+        for (int kk = 0; kk < KC; ++kk)
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+                nxt[kk][m] = live ? *reinterpret_cast<const double2 *>(row + 32 * kk + 8 * m) : make_double2(0.0, 0.0);
+#pragma unroll 1
+        for (int ch = 0; ch < NCH; ++ch) {
+            double2 a[KC][4];
+#pragma unroll
+            for (int kk = 0; kk < KC; ++kk)
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    // bins past the tracer's end bin do not enter its sum
+                    const int k = 32 * (KC * ch + kk) + 8 * m + 2 * q;
+                    a[kk][m].x = k < je ? nxt[kk][m].x : 0.0;
+                    a[kk][m].y = k + 1 < je ? nxt[kk][m].y : 0.0;
+                }
+            if (ch + 1 < NCH) {
+#pragma unroll
+                for (int kk = 0; kk < KC; ++kk)
+#pragma unroll
+                    for (int m = 0; m < 4; ++m)
+                        nxt[kk][m] = live ? *reinterpret_cast<const double2 *>(row + 32 * (KC * (ch + 1) + kk) + 8 * m)
+                                          : make_double2(0.0, 0.0);
+            }
+            int e = NONE;
+#pragma unroll
+            for (int kk = 0; kk < KC; ++kk)
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    const double v0 = fabs(a[kk][m].x), v1 = fabs(a[kk][m].y);
+                    if (v0 > 0.0 && isfinite(v0)) e = max(e, __builtin_amdgcn_frexp_exp(v0));
+                    if (v1 > 0.0 && isfinite(v1)) e = max(e, __builtin_amdgcn_frexp_exp(v1));
+                }
+            e = max(e, __shfl_xor(e, 16, 64));
+            e = max(e, __shfl_xor(e, 32, 64));
+            const int s_new = max(s_row, e);
+            if (ch > 0) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int rr = 4 * q + j;
+                    const int so = __shfl(s_row, rr, 64), sn = __shfl(s_new, rr, 64);
+                    if (so != sn && so != NONE) {
+                        const float f = ldexpf(1.0f, max(so - sn, -160));
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) acc[c][j] *= f;
+                    }
+                }
+            }
+            s_row = s_new;
+            const int sh = s_row == NONE ? 0 : s_row - EM_SY_OFF;
+#pragma unroll
+            for (int kk = 0; kk < KC; ++kk) {
+                const int ks = KC * ch + kk;
+                bf16x8 ah, al;
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    const float f0 = (float)ldexp(a[kk][m].x, -sh), f1 = (float)ldexp(a[kk][m].y, -sh);
+                    const uint16_t h0 = em_bf16_bits(f0), h1 = em_bf16_bits(f1);
+                    ah[2 * m] = __builtin_bit_cast(__bf16, h0);
+                    ah[2 * m + 1] = __builtin_bit_cast(__bf16, h1);
+                    al[2 * m] = __builtin_bit_cast(__bf16, em_bf16_bits(f0 - em_bf16_val(h0)));
+                    al[2 * m + 1] = __builtin_bit_cast(__bf16, em_bf16_bits(f1 - em_bf16_val(h1)));
+                }
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const bf16x8 bh = *reinterpret_cast<const bf16x8 *>(&b_hi[ks][q][16 * c + r][0]);
+                    const bf16x8 bl = *reinterpret_cast<const bf16x8 *>(&b_lo[ks][q][16 * c + r][0]);
+                    acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[c], 0, 0, 0);
+                    acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc[c], 0, 0, 0);
+                    acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[c], 0, 0, 0);
+                }
+            }
+        }
+        const int sh = s_row == NONE ? 0 : s_row - EM_SY_OFF;
+        // D layout: lane holds rows 4 (lane / 16) + j, column lane % 16; a
+        // column is (frequency cb * 8 + (16 c + r) / 8, Chebyshev term r % 8)
+        const int mt = r & 7;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int rr = 4 * q + j;
+            const int shr = __shfl(sh, rr, 64);
+            const bool lv = tile * 16 + rr < cnt;
+            const int itr = lv ? perm[start + tile * 16 + rr] : 0;
+            const double sr = lv ? sv[itr] : 0.0;
+            // T_mt(s) by the recurrence
+            double t0 = 1.0, t1 = sr, tm = (mt == 0) ? 1.0 : sr;
+            for (int m = 2; m <= mt; ++m) {
+                tm = 2.0 * sr * t1 - t0;
+                t0 = t1;
+                t1 = tm;
+            }
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int col = cb * EM_TB + 16 * c + r;
+                double v = 0.0;
+                if (col < ncol) v = ldexp((double)acc[c][j], shr + col_shift[pbase + col] - EM_SY_OFF) * tm;
+                v += __shfl_xor(v, 1, 64);
+                v += __shfl_xor(v, 2, 64);
+                v += __shfl_xor(v, 4, 64);
+                const int f = col / EM_SY_M;
+                if (lv && mt == 0 && f < nfreq) eps[(size_t)itr * nfreq + f] = pref[itr] * v;
+            }
+        }
+    }
+}
+
+// Range check of the MFMA contraction.  An operand entry below 2^-126 after
+// scaling (or a product below it) is lost, so each column sum of tracer i
+// misses at most 256 x 2^-67 x 2^(srow_i + cs - 118) of its true value (cs:
+// the column's binary exponent), and eps_f at most
+// |A| 8 2^(srow_i + max_m cs(f, m) - 177).  Where that bound exceeds 2^-17
+// (1e-5) of an emissivity within 25 decades of the tracer's largest one (or
+// the row has content and every emissivity came out zero), the tracer is
+// listed for the FP32 log-domain kernel, which has no range limit.
+__global__ __launch_bounds__(256) void k_synch_check(int ntr, int nfreq, int nlogb, double logb_min,
+                                                     double inv_dlogb, const double *__restrict__ b_field,
+                                                     const int *__restrict__ srow, const double *__restrict__ pref,
+                                                     const int *__restrict__ col_shift,
+                                                     const double *__restrict__ eps, int *__restrict__ fix)
+{
+    const int lane = threadIdx.x & 63;
+    const int it = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (it >= ntr) return;
+    const int sr = srow[it];
+    if (sr == EM_SY_NONE) return;
+    const int pl = em_synch_plane(b_field[it], logb_min, inv_dlogb, nlogb);
+    const double *e = eps + (size_t)it * nfreq;
+    const double ap = fabs(pref[it]);
+    double mx = 0.0;
+    for (int f = lane; f < nfreq; f += 64) mx = fmax(mx, fabs(e[f]));
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+    bool bad = false;
+    for (int f = lane; f < nfreq; f += 64) {
+        const int *cs = col_shift + ((size_t)pl * nfreq + f) * EM_SY_M;
+        int cm = EM_SY_NONE;
+#pragma unroll
+        for (int m = 0; m < EM_SY_M; ++m)
+            if (cs[m] > -2000) cm = max(cm, cs[m]);
+        if (cm == EM_SY_NONE) continue;
+        const double lim = ldexp(ap, sr + cm - 157);
+        if (lim > 0.0 && lim >= 1.0e-25 * mx && fabs(e[f]) < lim) bad = true;
+    }
+    if (__ballot(bad) && lane == 0) fix[1 + atomicAdd(fix, 1)] = it;
+}
+
+extern "C" int ck_emission_synch_mfma(int ntr, int n, int nfreq, int nlogb, double logb_min, double inv_dlogb,
+                                      const double *table, uint16_t *kt_hi, uint16_t *kt_lo, int *col_shift,
+                                      int *work, int *jend, double *sv, double *pref, int *srow, int *fix,
+                                      float *lt, const double *cre, const double *dp, const double *b_field,
+                                      double *eps, void *stream)
+{
+    if (ntr <= 0 || !(n == 64 || n == 128 || n == 256) || nfreq <= 0 || nlogb < 2) return -1;
+    hipStream_t st = (hipStream_t)stream;
+    const size_t tot = (size_t)(nlogb - 1) * nfreq * n;
+    const size_t ncs = (size_t)(nlogb - 1) * nfreq * EM_SY_M;
+    const dim3 g1((ntr + 255) / 256);
+    int *key = work, *hist = work + ntr, *perm = work + ntr + nlogb;
+    if (ck_fill_i32(col_shift, -100000, ncs, stream) || ck_fill_i32(hist, 0, (size_t)nlogb, stream) ||
+        ck_fill_i32(fix, 0, 1, stream))
+        return -1;
+    hipLaunchKernelGGL(k_synch_cheb, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, 0, n, nfreq, nlogb, table,
+                       dp, kt_hi, kt_lo, col_shift);
+    hipLaunchKernelGGL(k_synch_cheb, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, 1, n, nfreq, nlogb, table,
+                       dp, kt_hi, kt_lo, col_shift);
+    hipLaunchKernelGGL(k_synch_plane_count, g1, dim3(256), 0, st, ntr, b_field, logb_min, inv_dlogb, nlogb, key, hist);
+    hipLaunchKernelGGL(k_synch_plane_scan, dim3(1), dim3(256), 0, st, nlogb, hist);
+    hipLaunchKernelGGL(k_synch_plane_place, g1, dim3(256), 0, st, ntr, nlogb, key, hist, perm);
+    hipLaunchKernelGGL(k_synch_prep, dim3((unsigned)((ntr + 3) / 4)), dim3(256), 0, st, ntr, n, nlogb, logb_min,
+                       inv_dlogb, cre, b_field, jend, sv, pref, srow);
+    // after the placement hist[p] is the end of plane interval p in perm
+    const int ncb = (nfreq * EM_SY_M + EM_TB - 1) / EM_TB;
+    const int npg = (nlogb - 1 + 7) / 8;
+    const dim3 gm((unsigned)(npg * ncb * 8));
+#define EM_SY(KS_)                                                                                                     \
+    hipLaunchKernelGGL((k_synch_mfma<KS_>), gm, dim3(256), 0, st, nfreq, nlogb, ncb, cre, perm, hist, jend, sv, pref,  \
+                       kt_hi, kt_lo, col_shift, eps)
+    if (n == 64) EM_SY(2);
+    else if (n == 128) EM_SY(4);
+    else EM_SY(8);
+#undef EM_SY
+    // tracers beyond the contraction's range: the FP32 log-domain kernel
+    const dim3 gw((unsigned)((ntr + 3) / 4));
+    const size_t ttot = (size_t)nlogb * nfreq * n;
+    hipLaunchKernelGGL(k_synch_check, gw, dim3(256), 0, st, ntr, nfreq, nlogb, logb_min, inv_dlogb, b_field, srow,
+                       pref, col_shift, eps, fix);
+    hipLaunchKernelGGL(k_synch_log2t32, dim3((unsigned)((ttot + 255) / 256)), dim3(256), 0, st, nlogb, nfreq, n,
+                       table, lt);
+    hipLaunchKernelGGL(k_emis_synch32, gw, dim3(256), 0, st, ntr, n, nfreq, nlogb, logb_min, inv_dlogb, table,
+                       (const float *)lt, (const int *)(fix + 1), cre, dp, b_field, eps, (const int *)fix);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 extern "C" int ck_emission_synch(int ntr, int n, int nfreq, int nlogb, double logb_min, double inv_dlogb,
                                  const double *table, double *logt, int *work, const double *cre, const double *dp,
                                  const double *b_field, double *eps, int fp64, void *stream)
@@ -865,7 +1276,7 @@ extern "C" int ck_emission_synch(int ntr, int n, int nfreq, int nlogb, double lo
         hipLaunchKernelGGL(k_synch_log2t32, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, nlogb, nfreq, n,
                            table, lt);
         hipLaunchKernelGGL(k_emis_synch32, grid, dim3(256), 0, st, ntr, n, nfreq, nlogb, logb_min, inv_dlogb, table,
-                           (const float *)lt, (const int *)perm, cre, dp, b_field, eps);
+                           (const float *)lt, (const int *)perm, cre, dp, b_field, eps, nullptr);
         return hipGetLastError() == hipSuccess ? 0 : -1;
     }
     hipLaunchKernelGGL(k_emis_synch<double>, grid, dim3(256), 0, st, ntr, n, nfreq, nlogb, logb_min, inv_dlogb,

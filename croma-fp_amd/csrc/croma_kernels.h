@@ -245,6 +245,27 @@ int ck_emission_contract(int ntr, int K, int nbins, int nbins_pad, const double 
 int ck_synch_table(int n, int nfreq, int ntheta, int nlogb, double logb_min, double inv_dlogb, int nx_tab,
                    double xmin, const double *logx_tab, const double *logfx_tab, const double *fx_tab,
                    const double *logy, const double *pitch_weight, double *table, void *stream);
+/* Synchrotron on MFMA: Chebyshev coefficients of the log-B interpolation per
+ * plane interval (kt_hi / kt_lo [(nlogb - 1) * nfreq * 8][n] bf16, col_shift
+ * [(nlogb - 1) * nfreq * 8]), tracers grouped by plane (work: 2 ntr + nlogb
+ * ints), per-tracer end bin / s / prefactor / row exponent (jend, sv, pref,
+ * srow [ntr]), then the split-BF16 x3 contraction; the tracers whose
+ * emissivities fall below its range (fix: count + list, ntr + 1 ints) are
+ * recomputed by the FP32 log-domain kernel from the float log2 table lt
+ * [nlogb * nfreq * n].  n in {64, 128, 256}. */
+int ck_emission_synch_mfma(int ntr, int n, int nfreq, int nlogb, double logb_min, double inv_dlogb,
+                           const double *table, uint16_t *kt_hi, uint16_t *kt_lo, int *col_shift, int *work,
+                           int *jend, double *sv, double *pref, int *srow, int *fix, float *lt, const double *cre,
+                           const double *dp, const double *b_field, double *eps, void *stream);
+/* Grid-constant tables from the shared physics header (croma_tables.hip):
+ * secondary-electron weights fqe_t[jp][je] and np_min[je] from the electron
+ * energies, proton momenta, beta_p and proton bin widths; gamma (kind 0),
+ * neutrino (1) or IC emissivity (2) kernel rows out[nbins][n] over the log10
+ * range lo..hi (mom: proton momenta for kinds 0/1, electron momenta for 2). */
+int ck_secondary_weights(int n, const double *e_energy, const double *pp, const double *beta_p,
+                         const double *pdp, double *fqe_t, int *np_min, void *stream);
+int ck_emission_kernel_table(int kind, int n, int nbins, double lo, double hi, double z,
+                             const double *mom, double *out, void *stream);
 /* Synchrotron: per-tracer log-B interpolated table contraction (VALU, FP64 or FP32).
  * logt: workspace of nlogb * nfreq * n doubles, filled here with the table's
  * logarithms (FP64: natural log, NULL computes them per element; FP32: float

@@ -239,6 +239,94 @@ int croma_hip_emission_synch(const CromaHipContext *ctx, int ntracer, int nfreq,
                                          CROMA_HIP_EMIS_FP64, elapsed_ms);
 }
 
+/* Synchrotron on MFMA (ck_emission_synch_mfma): the Chebyshev expansion of
+ * the log-B interpolation needs table planes no coarser than 0.02 in log10 B
+ * and n <= 256 (the kernel's 64 columns of K = n bins in LDS). */
+static int synch_mfma(const CromaHipContext *ctx, int ntr, int nfreq, int nlogb, double logb_min, double inv_dlogb,
+                      const double *table, const double *cre, const double *b_field, double *eps, double *elapsed_ms)
+{
+    const ChGrid *g = ch_ctx_grid(ctx);
+    const int n = g->n;
+    const size_t ntab = (size_t)nlogb * nfreq * n, nkt = (size_t)(nlogb - 1) * nfreq * 8 * n;
+    const size_t ncs = (size_t)(nlogb - 1) * nfreq * 8;
+    char *d = 0, *p;
+    void *st = 0, *e0 = 0, *e1 = 0;
+    int rc = 0;
+    if (n > 256 || !(inv_dlogb >= 50.0)) {
+        ch_set_error("synch MFMA: needs n <= 256 and planes no coarser than 0.02 in log10 B (n %d, 1/dlogb %g)", n,
+                     inv_dlogb);
+        return CROMA_HIP_EUNSUPPORTED;
+    }
+#define SY_R(b) ((((size_t)(b)) + 255) & ~(size_t)255)
+    d = (char *)ck_malloc(SY_R(ntab * 8) + 2 * SY_R(nkt * 2) + SY_R(ncs * 4) + SY_R((size_t)ntr * n * 8) +
+                          SY_R((size_t)n * 8) + SY_R((size_t)ntr * 8) + SY_R((size_t)ntr * nfreq * 8) +
+                          SY_R((2 * (size_t)ntr + nlogb) * 4) + SY_R((size_t)ntr * 4) + 2 * SY_R((size_t)ntr * 8) +
+                          SY_R((size_t)ntr * 4) + SY_R(((size_t)ntr + 1) * 4) + SY_R(ntab * 4));
+    st = ck_stream_create();
+    e0 = ck_event_create();
+    e1 = ck_event_create();
+    if (!d || !st || !e0 || !e1) {
+        rc = CROMA_HIP_EDEVICE;
+        ch_set_error("synch MFMA: device allocation: %s", ck_last_error());
+        goto out;
+    }
+    {
+        double *d_tab, *d_cre, *d_dp, *d_b, *d_eps, *d_s, *d_pref;
+        uint16_t *d_hi, *d_lo;
+        int *d_cs, *d_work, *d_jend, *d_srow, *d_fix;
+        float *d_lt;
+        p = d;
+        d_tab = (double *)p;
+        p += SY_R(ntab * 8);
+        d_hi = (uint16_t *)p;
+        p += SY_R(nkt * 2);
+        d_lo = (uint16_t *)p;
+        p += SY_R(nkt * 2);
+        d_cs = (int *)p;
+        p += SY_R(ncs * 4);
+        d_cre = (double *)p;
+        p += SY_R((size_t)ntr * n * 8);
+        d_dp = (double *)p;
+        p += SY_R((size_t)n * 8);
+        d_b = (double *)p;
+        p += SY_R((size_t)ntr * 8);
+        d_eps = (double *)p;
+        p += SY_R((size_t)ntr * nfreq * 8);
+        d_work = (int *)p;
+        p += SY_R((2 * (size_t)ntr + nlogb) * 4);
+        d_jend = (int *)p;
+        p += SY_R((size_t)ntr * 4);
+        d_s = (double *)p;
+        p += SY_R((size_t)ntr * 8);
+        d_pref = (double *)p;
+        p += SY_R((size_t)ntr * 8);
+        d_srow = (int *)p;
+        p += SY_R((size_t)ntr * 4);
+        d_fix = (int *)p;
+        p += SY_R(((size_t)ntr + 1) * 4);
+        d_lt = (float *)p;
+#undef SY_R
+        if (ck_h2d(d_tab, table, ntab * 8, st) || ck_h2d(d_cre, cre, (size_t)ntr * n * 8, st) ||
+            ck_h2d(d_dp, g->edp, (size_t)n * 8, st) || ck_h2d(d_b, b_field, (size_t)ntr * 8, st) ||
+            ck_fill_f64(d_eps, 0.0, (size_t)ntr * nfreq, st) || ck_event_record(e0, st) ||
+            ck_emission_synch_mfma(ntr, n, nfreq, nlogb, logb_min, inv_dlogb, d_tab, d_hi, d_lo, d_cs, d_work, d_jend,
+                                   d_s, d_pref, d_srow, d_fix, d_lt, d_cre, d_dp, d_b, d_eps, st) ||
+            ck_event_record(e1, st) || ck_d2h(eps, d_eps, (size_t)ntr * nfreq * 8, st) || ck_stream_sync(st)) {
+            rc = CROMA_HIP_EDEVICE;
+            ch_set_error("synch MFMA: %s", ck_last_error());
+            goto out;
+        }
+        if (elapsed_ms) *elapsed_ms = ck_event_elapsed_ms(e0, e1);
+    }
+out:
+    if (st) ck_stream_sync(st);
+    ck_free(d);
+    ck_event_destroy(e0);
+    ck_event_destroy(e1);
+    if (st) ck_stream_destroy(st);
+    return rc;
+}
+
 int croma_hip_emission_synch_mode(const CromaHipContext *ctx, int ntracer, int nfreq, int nlogb, double logb_min,
                                   double inv_dlogb, const double *table, const double *cre, const double *b_field,
                                   double *eps, int mode, double *elapsed_ms)
@@ -249,10 +337,12 @@ int croma_hip_emission_synch_mode(const CromaHipContext *ctx, int ntracer, int n
     void *st = 0, *e0 = 0, *e1 = 0;
     int n, rc = 0;
     if (!ctx || ntracer <= 0 || nfreq <= 0 || nlogb < 2 || !table || !cre || !b_field || !eps ||
-        (mode != CROMA_HIP_EMIS_FP64 && mode != CROMA_HIP_EMIS_FP32))
+        (mode != CROMA_HIP_EMIS_FP64 && mode != CROMA_HIP_EMIS_FP32 && mode != CROMA_HIP_EMIS_MFMA_XF32))
         return CROMA_HIP_EINVAL;
     g = ch_ctx_grid(ctx);
     n = g->n;
+    if (mode == CROMA_HIP_EMIS_MFMA_XF32) return synch_mfma(ctx, ntracer, nfreq, nlogb, logb_min, inv_dlogb, table,
+                                                            cre, b_field, eps, elapsed_ms);
     d = (char *)ck_malloc(2 * nt_tab * n * 8 + (size_t)ntracer * n * 8 + (size_t)n * 8 + (size_t)ntracer * 8 +
                           (size_t)ntracer * nfreq * 8 + (3 * (size_t)ntracer + (size_t)nlogb) * 4 + 7 * 256);
     st = ck_stream_create();

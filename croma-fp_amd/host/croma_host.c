@@ -173,7 +173,6 @@ struct CromaHipContext {
     /* secondary weights, built on first use */
     int fqe_ready;
     int jp0;
-    double *h_fqe;
     int *h_npmin;
     double *d_fqe;
     void *blas; /* vendor GEMM handle for the split secondary contraction */
@@ -408,7 +407,6 @@ void croma_hip_context_destroy(CromaHipContext *c)
     ck_free(c->d_ic_last);
     ck_free(c->pws_mem);
     ck_blas_destroy(c->blas);
-    free(c->h_fqe);
     free(c->h_npmin);
     if (c->stream_e) ck_stream_destroy(c->stream_e);
     if (c->stream) ck_stream_destroy(c->stream);
@@ -639,15 +637,27 @@ int croma_hip_nsub_estimate_device(const CromaHipContext *c, int ntr, double dt_
 int ch_ctx_fqe(CromaHipContext *c)
 {
     const int n = c->g.n;
+    const size_t nn = (size_t)n * n;
+    double *d_beta = 0;
+    int *d_npmin = 0, jp0, je, rc = 0;
     if (c->fqe_ready) return 0;
-    c->h_fqe = (double *)malloc((size_t)n * n * sizeof(double));
-    c->h_npmin = (int *)malloc((size_t)n * sizeof(int));
-    if (!c->h_fqe || !c->h_npmin) return CROMA_HIP_ENOMEM;
-    ch_secondary_weights(&c->g, c->h_fqe, c->h_npmin, &c->jp0);
-    c->d_fqe = (double *)ck_malloc((size_t)n * n * sizeof(double));
+    if (!c->h_npmin) c->h_npmin = (int *)malloc((size_t)n * sizeof(int));
+    if (!c->h_npmin) return CROMA_HIP_ENOMEM;
+    if (!c->d_fqe) c->d_fqe = (double *)ck_malloc((nn + (size_t)2 * n) * sizeof(double));
     if (!c->d_fqe) return dev_fail("fqe alloc");
-    if (ck_h2d(c->d_fqe, c->h_fqe, (size_t)n * n * sizeof(double), c->stream) || ck_stream_sync(c->stream))
-        return dev_fail("fqe upload");
+    d_beta = c->d_fqe + nn;
+    d_npmin = (int *)(d_beta + n);
+    if (ck_h2d(d_beta, c->g.beta_p, (size_t)n * sizeof(double), c->stream) ||
+        ck_secondary_weights(n, c->dgrid.eE, c->dgrid.pp, d_beta, c->dgrid.pdp, c->d_fqe, d_npmin, c->stream) ||
+        ck_d2h(c->h_npmin, d_npmin, (size_t)n * sizeof(int), c->stream) || ck_stream_sync(c->stream))
+        rc = dev_fail("fqe build");
+    if (rc) return rc;
+    jp0 = n;
+    for (je = 0; je < n; je++) {
+        const int v = c->h_npmin[je] < 0 ? 0 : c->h_npmin[je];
+        if (v < jp0) jp0 = v;
+    }
+    c->jp0 = jp0;
     c->fqe_ready = 1;
     return 0;
 }
@@ -659,9 +669,41 @@ int croma_hip_secondary_weights(const CromaHipContext *cc, double *fqe_t, int *n
     if (!c || !fqe_t || !np_min_qe) return CROMA_HIP_EINVAL;
     rc = ch_ctx_fqe(c);
     if (rc) return rc;
-    memcpy(fqe_t, c->h_fqe, (size_t)c->g.n * c->g.n * sizeof(double));
+    if (ck_d2h(fqe_t, c->d_fqe, (size_t)c->g.n * c->g.n * sizeof(double), c->stream) || ck_stream_sync(c->stream))
+        return dev_fail("fqe download");
     memcpy(np_min_qe, c->h_npmin, (size_t)c->g.n * sizeof(int));
     return 0;
+}
+
+int croma_hip_emission_kernel_table(const CromaHipContext *cc, int kind, int nbins, double lo, double hi, double z,
+                                    double *table, double *elapsed_ms)
+{
+    CromaHipContext *c = (CromaHipContext *)cc;
+    const int n = c ? c->g.n : 0;
+    const double *mom;
+    double *d_out;
+    void *e0 = 0, *e1 = 0;
+    int rc = 0;
+    if (!c || !table || nbins <= 0 || nbins > 65535 || kind < CROMA_HIP_TABLE_GAMMA || kind > CROMA_HIP_TABLE_IC ||
+        !(hi > lo) || !isfinite(lo) || !isfinite(hi) || !isfinite(z))
+        return CROMA_HIP_EINVAL;
+    mom = kind == CROMA_HIP_TABLE_IC ? c->dgrid.ep : c->dgrid.pp;
+    d_out = (double *)ck_malloc((size_t)nbins * n * sizeof(double));
+    if (!d_out) return dev_fail("kernel table alloc");
+    if (elapsed_ms) {
+        e0 = ck_event_create();
+        e1 = ck_event_create();
+    }
+    if ((e0 && ck_event_record(e0, c->stream)) ||
+        ck_emission_kernel_table(kind, n, nbins, lo, hi, z, mom, d_out, c->stream) ||
+        (e1 && ck_event_record(e1, c->stream)) ||
+        ck_d2h(table, d_out, (size_t)nbins * n * sizeof(double), c->stream) || ck_stream_sync(c->stream))
+        rc = dev_fail("kernel table");
+    if (!rc && elapsed_ms) *elapsed_ms = (e0 && e1) ? (double)ck_event_elapsed_ms(e0, e1) : -1.0;
+    if (e0) ck_event_destroy(e0);
+    if (e1) ck_event_destroy(e1);
+    ck_free(d_out);
+    return rc;
 }
 
 /* Accessors for the serving loop (croma_serve.c). */

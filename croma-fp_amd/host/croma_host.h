@@ -40,12 +40,6 @@ void ch_ic_bounds(double z, double *e0min, double *e0max);
 /* IC cooling rate of the electron bins bins[0..count) at z (same arithmetic
  * as ch_ic_cooling_row, which fills every bin). */
 int ch_ic_cooling_bins(const ChGrid *g, double z, int count, const int *bins, double *out);
-int ch_secondary_weights(const ChGrid *g, double *fqe_t, int *np_min_qe, int *jp0_out);
-double ch_gamma_kernel(double eg, double p);
-double ch_nu_kernel(double e, double p);
-double ch_ic_emissivity_kernel(double nu, double gamma, double z);
-/* kind 0 gamma (E in log10 GeV), 1 neutrino, 2 IC (nu in log10 Hz, at z) */
-int ch_emission_table(const ChGrid *g, int kind, int nbins, double lo, double hi, double z, double *out);
 
 /* croma_host.c */
 double ch_now_s(void);

@@ -207,8 +207,21 @@ int croma_hip_ic_cooling_row(const CromaHipContext *ctx, double z, double *row, 
  * ghosts [nz][2] (bins at epm1, epp1; may be NULL). */
 int croma_hip_ic_cooling_rows_device(CromaHipContext *ctx, int nz, const double *z, double *rows, double *ghosts);
 /* Secondary-electron weights, transposed layout (prepare_secondary_kernel_flat,
- * fp_shared_core.c:1453-1506). */
+ * fp_shared_core.c:1453-1506), built on the device once per context
+ * (QeKernel_BB05 quadrature per entry, HADRONIC.c:436-498) and kept there for
+ * the secondary-source contraction; this call copies them out. */
 int croma_hip_secondary_weights(const CromaHipContext *ctx, double *fqe_t, int *np_min_qe);
+/* Emission kernel tables [nbins][n] built on the device over the log10 range
+ * lo..hi (the reference builds them on the CPU: prepare_gamma_kernel_flat,
+ * prepare_neutrino_kernel_flat, prepare_ic_kernel_flat, fp_shared_core.c:
+ * 1724-1804): kind CROMA_HIP_TABLE_GAMMA (E_gamma in GeV, proton grid),
+ * CROMA_HIP_TABLE_NEUTRINO (E_nu in GeV, proton grid) or CROMA_HIP_TABLE_IC
+ * (nu in Hz at redshift z, electron grid).  table: host [nbins][n]. */
+#define CROMA_HIP_TABLE_GAMMA 0
+#define CROMA_HIP_TABLE_NEUTRINO 1
+#define CROMA_HIP_TABLE_IC 2
+int croma_hip_emission_kernel_table(const CromaHipContext *ctx, int kind, int nbins, double lo, double hi,
+                                    double z, double *table, double *elapsed_ms);
 
 /* ---- device-resident batch (cuda_rank_state_upload/download +
  * run_tracer_pipeline, fp_cuda_backend.h:124-150) --------------------- */

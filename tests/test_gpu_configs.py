@@ -140,7 +140,7 @@ def test_c3_emission_sampled(c3, kind, mode):
     assert err < EMIS_TOL[mode]
 
 
-@pytest.mark.parametrize("mode", [ch.EMIS_FP64, ch.EMIS_FP32])
+@pytest.mark.parametrize("mode", [ch.EMIS_FP64, ch.EMIS_FP32, ch.EMIS_MFMA_XF32])
 def test_c3_synch_sampled(c3, mode):
     n, o, s, nfreq = c3["n"], c3["o"], c3["sample"], 64
     nlogb, logb_min, inv_dlogb, ntheta = 512, -3.0, 512 / 6.0, 32

@@ -193,3 +193,58 @@ def test_synch_table_on_device_matches_reference(n):
     err = float(np.max(np.abs(big[m] - big_ref[m]) / np.abs(big_ref[m])))
     print(f"  32x64x{n} table: max rel err {err:.3e} ({ms:.3f} ms)", flush=True)
     assert err < 1e-13
+
+
+@pytest.mark.parametrize("n", [64, 128, 256])
+def test_synch_mfma_matches_oracle(n):
+    """Synchrotron on MFMA (Chebyshev expansion of the log-B interpolation,
+    plane-grouped split-BF16 x3 contraction) against the oracle's FP64 sum on
+    a device-built table of 512 planes over 6 decades in B: 1e-4 over the
+    entries within 25 decades of each tracer's peak; truncated, empty and
+    extreme-scale spectra included."""
+    fx = np.load(os.path.join(GOLD, "fx_table.npz"))
+    o = po.Oracle(n)
+    nfreq, ntheta, nlogb, logb_min, inv = 64, 16, 512, -3.0, 512 / 6.0
+    logy, pw = po.synch_logy(o, nfreq, 7.0, 11.0, ntheta)
+    rng = np.random.default_rng(29)
+    nt = 3000
+    q = rng.uniform(2.0, 3.5, nt)
+    cre = (o.ep[None, :] ** -q[:, None]) * np.exp(-o.ep[None, :] / 10 ** rng.uniform(2, 6, nt)[:, None])
+    cre *= 10 ** rng.uniform(-5, 5, nt)[:, None]
+    cre[5, n // 2:] = 0.0   # truncated: the sum stops at the first empty successor
+    cre[7] = 0.0            # empty
+    cre[9] *= 1e-280
+    cre[11] *= 1e250
+    b = 10 ** rng.uniform(-8.0, -5.0, nt)
+    ctx = ch.Context(n, ch.default_params())
+    try:
+        tab, _ = ctx.synch_kernel_table(nfreq, ntheta, nlogb, logb_min, inv, fx["logx"], fx["logfx"], fx["fx"],
+                                        float(fx["xmin"][0]), logy, pw)
+        eps, ms = ctx.emission_synch(cre, b, tab, nfreq, nlogb, logb_min, inv, mode=ch.EMIS_MFMA_XF32)
+    finally:
+        ctx.close()
+    sample = np.r_[np.arange(16), np.sort(np.random.default_rng(30).choice(np.arange(16, nt), 400, replace=False))]
+    ref = po.emission_synch(n, nfreq, nlogb, logb_min, inv, tab, cre[sample].ravel(), o.edp,
+                            b[sample]).reshape(len(sample), nfreq)
+    assert np.all(eps[7] == 0.0)
+    err = rel(eps[sample], ref)
+    print(f"[emission] synch MFMA n={n} max rel err {err:.3e} ({ms:.3f} ms)", flush=True)
+    assert err < 1e-4
+
+
+def test_synch_mfma_refuses_coarse_planes():
+    """Planes coarser than 0.02 in log10 B leave the Chebyshev truncation
+    unbounded: the MFMA mode refuses them (the FP32 VALU mode serves them)."""
+    fx = np.load(os.path.join(GOLD, "fx_table.npz"))
+    o = po.Oracle(128)
+    logy, pw = po.synch_logy(o, 16, 7.0, 11.0, 8)
+    ctx = ch.Context(128, ch.default_params())
+    try:
+        tab, _ = ctx.synch_kernel_table(16, 8, 32, -3.0, 32 / 6.0, fx["logx"], fx["logfx"], fx["fx"],
+                                        float(fx["xmin"][0]), logy, pw)
+        with pytest.raises(ch.CromaHipError) as ei:
+            ctx.emission_synch(np.ones((4, 128)), np.full(4, 1e-6), tab, 16, 32, -3.0, 32 / 6.0,
+                               mode=ch.EMIS_MFMA_XF32)
+        assert ei.value.rc == ch.EUNSUPPORTED
+    finally:
+        ctx.close()
