@@ -384,18 +384,29 @@ def emission_bench(ch, ntracer=1_000_000, n=256, nbins=64, seed=11):
         res["synch"] = {}
         for mname, mode in (("fp64", ch.EMIS_FP64), ("fp32", ch.EMIS_FP32),
                             ("mfma_split_bf16x3", ch.EMIS_MFMA_XF32)):
-            ms = []
-            for _ in range(2):
-                _, t = ctx.emission_synch(spec, bfield, table, nbins, nlogb, logb_min, inv_dlogb, mode=mode)
-                ms.append(t)
+            plan = ch.SynchPlan(ctx, table, nbins, nlogb, logb_min, inv_dlogb) if mode == ch.EMIS_MFMA_XF32 else None
+            try:
+                ms = []
+                for _ in range(2):
+                    if plan is not None:
+                        _, t = plan.run(spec, bfield)
+                    else:
+                        _, t = ctx.emission_synch(spec, bfield, table, nbins, nlogb, logb_min, inv_dlogb, mode=mode)
+                    ms.append(t)
+            finally:
+                if plan is not None:
+                    plan.close()
             t = ms[-1]
             res["synch"][mname] = {"ms": t, "hbm_gbs": (byts + 8.0 * ntracer) / (t * 1e-3) / 1e9,
                                    "hbm_frac": (byts + 8.0 * ntracer) / (t * 1e-3) / 1e9 / HBM_PEAK_GBS,
                                    "table_build_ms": ms_tab}
-            if mode == ch.EMIS_MFMA_XF32:
-                # the device time includes the per-call Chebyshev coefficient
-                # tables of the 511 plane intervals (a per-table cost)
-                res["synch"][mname]["includes"] = "Chebyshev coefficient tables, plane sort, contraction"
+            if plan is not None:
+                # the Chebyshev coefficient tables of the 511 plane intervals
+                # are built once per table (the plan); each run sorts the
+                # tracers by plane, contracts them and fixes up the tracers
+                # beyond the contraction's range
+                res["synch"][mname]["plan_prepare_ms"] = plan.prepare_ms
+                res["synch"][mname]["includes"] = "plane sort, contraction, range check, FP32 fix-up"
                 res["synch"][mname]["mfma_tflops_issued"] = 3.0 * 2.0 * n * nbins * 8 * ntracer / (t * 1e-3) / 1e12
         out["kernels"] = res
         out["bytes_per_call"] = byts
@@ -405,7 +416,8 @@ def emission_bench(ch, ntracer=1_000_000, n=256, nbins=64, seed=11):
     return out
 
 
-def served_latency(ch, chunks=1000, seconds=240.0, seed=3, compare_chunks=48, plain_chunks=6, plain_max_tracers=2048):
+def served_latency(ch, chunks=1000, seconds=240.0, seed=3, compare_chunks=48, plain_chunks=6, plain_max_tracers=2048,
+                   plain_budget_s=20.0):
     """Poisson stream of mixed chunks through the serving loop, then the
     server's closed-loop capacity on the stream's first chunks against the
     same chunks run one at a time through the batch API; and, on the first
@@ -423,9 +435,13 @@ def served_latency(ch, chunks=1000, seconds=240.0, seed=3, compare_chunks=48, pl
     progress(f"served closed loop vs sequential fused: {out['served_vs_sequential_fused']:.3f}")
     small = [s for s in serve_stream.chunk_specs(20 * plain_chunks, seed)
              if s["ntracer"] <= plain_max_tracers][:plain_chunks]
-    ps = {"chunks": len(small), "max_tracers": plain_max_tracers,
-          "closed_loop": serve_stream.run_closed_loop(small),
-          "sequential_plain": serve_stream.run_sequential(small, path=ch.PATH_PLAIN, verbose=True)}
+    # the plain path launches per bucket and substep (the reference's
+    # structure): its chunks run until plain_budget_s is spent, and the
+    # server's closed loop then runs the same chunks
+    seq = serve_stream.run_sequential(small, path=ch.PATH_PLAIN, verbose=True, budget_s=plain_budget_s)
+    small = small[:seq["chunks"]]
+    ps = {"chunks": len(small), "max_tracers": plain_max_tracers, "budget_s": plain_budget_s,
+          "closed_loop": serve_stream.run_closed_loop(small), "sequential_plain": seq}
     ps["served_vs_plain"] = (ps["closed_loop"]["throughput_substeps_per_s"] /
                              ps["sequential_plain"]["throughput_substeps_per_s"])
     out["plain_set"] = ps

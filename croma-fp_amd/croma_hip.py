@@ -102,6 +102,13 @@ def lib():
             getattr(L, name).restype = None
         L.croma_hip_batch_create.restype = C.c_void_p
         L.croma_hip_batch_create.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+        L.croma_hip_synch_plan_create.restype = C.c_void_p
+        L.croma_hip_synch_plan_create.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double,
+                                                  C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.croma_hip_synch_plan_run.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                               C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.croma_hip_synch_plan_destroy.argtypes = [C.c_void_p]
+        L.croma_hip_synch_plan_destroy.restype = None
         _lib = L
     return _lib
 
@@ -152,6 +159,39 @@ def device_name() -> str:
     buf = C.create_string_buffer(256)
     _chk(lib().croma_hip_get_current_device_id(buf, 256), "device id")
     return buf.value.decode()
+
+
+class SynchPlan:
+    """Synchrotron on MFMA with the table's device form built once
+    (croma_hip_synch_plan_*)."""
+
+    def __init__(self, ctx, table, nfreq, nlogb, logb_min, inv_dlogb):
+        ms = C.c_double()
+        self.n, self.nfreq = ctx.n, int(nfreq)
+        self._table = f64(table)
+        self.h = lib().croma_hip_synch_plan_create(C.c_void_p(ctx.h), self.nfreq, int(nlogb), C.c_double(logb_min),
+                                                   C.c_double(inv_dlogb), _p(self._table), C.byref(ms))
+        if not self.h:
+            raise CromaHipError(-1, "synch_plan_create")
+        self.prepare_ms = ms.value
+
+    def run(self, cre, b_field):
+        """Returns (eps [ntr, nfreq], device ms of the run)."""
+        cre = f64(cre)
+        nt = len(b_field)
+        eps = np.zeros(nt * self.nfreq)
+        ms = C.c_double()
+        _chk(lib().croma_hip_synch_plan_run(C.c_void_p(self.h), nt, _p(cre), _p(f64(b_field)), _p(eps), C.byref(ms)),
+             "synch_plan_run")
+        return eps.reshape(nt, self.nfreq), ms.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().croma_hip_synch_plan_destroy(C.c_void_p(self.h))
+            self.h = None
+
+    def __del__(self):
+        self.close()
 
 
 class Context:

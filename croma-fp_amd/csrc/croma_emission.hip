@@ -1202,30 +1202,41 @@ __global__ __launch_bounds__(256) void k_synch_check(int ntr, int nfreq, int nlo
     if (__ballot(bad) && lane == 0) fix[1 + atomicAdd(fix, 1)] = it;
 }
 
-extern "C" int ck_emission_synch_mfma(int ntr, int n, int nfreq, int nlogb, double logb_min, double inv_dlogb,
-                                      const double *table, uint16_t *kt_hi, uint16_t *kt_lo, int *col_shift,
-                                      int *work, int *jend, double *sv, double *pref, int *srow, int *fix,
-                                      float *lt, const double *cre, const double *dp, const double *b_field,
-                                      double *eps, void *stream)
+extern "C" int ck_synch_mfma_prepare(int n, int nfreq, int nlogb, const double *table, const double *dp,
+                                     uint16_t *kt_hi, uint16_t *kt_lo, int *col_shift, float *lt, void *stream)
 {
-    if (ntr <= 0 || !(n == 64 || n == 128 || n == 256) || nfreq <= 0 || nlogb < 2) return -1;
+    if (!(n == 64 || n == 128 || n == 256) || nfreq <= 0 || nlogb < 2) return -1;
     hipStream_t st = (hipStream_t)stream;
     const size_t tot = (size_t)(nlogb - 1) * nfreq * n;
     const size_t ncs = (size_t)(nlogb - 1) * nfreq * EM_SY_M;
-    const dim3 g1((ntr + 255) / 256);
-    int *key = work, *hist = work + ntr, *perm = work + ntr + nlogb;
-    if (ck_fill_i32(col_shift, -100000, ncs, stream) || ck_fill_i32(hist, 0, (size_t)nlogb, stream) ||
-        ck_fill_i32(fix, 0, 1, stream))
-        return -1;
+    const size_t ttot = (size_t)nlogb * nfreq * n;
+    if (ck_fill_i32(col_shift, -100000, ncs, stream)) return -1;
     hipLaunchKernelGGL(k_synch_cheb, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, 0, n, nfreq, nlogb, table,
                        dp, kt_hi, kt_lo, col_shift);
     hipLaunchKernelGGL(k_synch_cheb, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, 1, n, nfreq, nlogb, table,
                        dp, kt_hi, kt_lo, col_shift);
+    // float log2 table of the FP32 log-domain kernel (the range fix-up)
+    hipLaunchKernelGGL(k_synch_log2t32, dim3((unsigned)((ttot + 255) / 256)), dim3(256), 0, st, nlogb, nfreq, n,
+                       table, lt);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+extern "C" int ck_synch_mfma_run(int ntr, int n, int nfreq, int nlogb, double logb_min, double inv_dlogb,
+                                 const double *table, const uint16_t *kt_hi, const uint16_t *kt_lo,
+                                 const int *col_shift, const float *lt, int *work, int *jend, double *sv,
+                                 double *pref, int *srow, int *fix, const double *cre, const double *dp,
+                                 const double *b_field, double *eps, void *stream)
+{
+    if (ntr <= 0 || !(n == 64 || n == 128 || n == 256) || nfreq <= 0 || nlogb < 2) return -1;
+    hipStream_t st = (hipStream_t)stream;
+    const dim3 g1((ntr + 255) / 256), gw((unsigned)((ntr + 3) / 4));
+    int *key = work, *hist = work + ntr, *perm = work + ntr + nlogb;
+    if (ck_fill_i32(hist, 0, (size_t)nlogb, stream) || ck_fill_i32(fix, 0, 1, stream)) return -1;
     hipLaunchKernelGGL(k_synch_plane_count, g1, dim3(256), 0, st, ntr, b_field, logb_min, inv_dlogb, nlogb, key, hist);
     hipLaunchKernelGGL(k_synch_plane_scan, dim3(1), dim3(256), 0, st, nlogb, hist);
     hipLaunchKernelGGL(k_synch_plane_place, g1, dim3(256), 0, st, ntr, nlogb, key, hist, perm);
-    hipLaunchKernelGGL(k_synch_prep, dim3((unsigned)((ntr + 3) / 4)), dim3(256), 0, st, ntr, n, nlogb, logb_min,
-                       inv_dlogb, cre, b_field, jend, sv, pref, srow);
+    hipLaunchKernelGGL(k_synch_prep, gw, dim3(256), 0, st, ntr, n, nlogb, logb_min, inv_dlogb, cre, b_field, jend,
+                       sv, pref, srow);
     // after the placement hist[p] is the end of plane interval p in perm
     const int ncb = (nfreq * EM_SY_M + EM_TB - 1) / EM_TB;
     const int npg = (nlogb - 1 + 7) / 8;
@@ -1238,14 +1249,10 @@ extern "C" int ck_emission_synch_mfma(int ntr, int n, int nfreq, int nlogb, doub
     else EM_SY(8);
 #undef EM_SY
     // tracers beyond the contraction's range: the FP32 log-domain kernel
-    const dim3 gw((unsigned)((ntr + 3) / 4));
-    const size_t ttot = (size_t)nlogb * nfreq * n;
     hipLaunchKernelGGL(k_synch_check, gw, dim3(256), 0, st, ntr, nfreq, nlogb, logb_min, inv_dlogb, b_field, srow,
                        pref, col_shift, eps, fix);
-    hipLaunchKernelGGL(k_synch_log2t32, dim3((unsigned)((ttot + 255) / 256)), dim3(256), 0, st, nlogb, nfreq, n,
-                       table, lt);
-    hipLaunchKernelGGL(k_emis_synch32, gw, dim3(256), 0, st, ntr, n, nfreq, nlogb, logb_min, inv_dlogb, table,
-                       (const float *)lt, (const int *)(fix + 1), cre, dp, b_field, eps, (const int *)fix);
+    hipLaunchKernelGGL(k_emis_synch32, gw, dim3(256), 0, st, ntr, n, nfreq, nlogb, logb_min, inv_dlogb, table, lt,
+                       (const int *)(fix + 1), cre, dp, b_field, eps, (const int *)fix);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -239,91 +239,166 @@ int croma_hip_emission_synch(const CromaHipContext *ctx, int ntracer, int nfreq,
                                          CROMA_HIP_EMIS_FP64, elapsed_ms);
 }
 
-/* Synchrotron on MFMA (ck_emission_synch_mfma): the Chebyshev expansion of
- * the log-B interpolation needs table planes no coarser than 0.02 in log10 B
- * and n <= 256 (the kernel's 64 columns of K = n bins in LDS). */
+/* Synchrotron on MFMA (ck_synch_mfma_prepare / ck_synch_mfma_run): the
+ * Chebyshev expansion of the log-B interpolation needs table planes no
+ * coarser than 0.02 in log10 B and n <= 256 (the kernel's 64 columns of K = n
+ * bins in LDS).  A plan holds the table's device form; its per-call buffers
+ * grow to the largest call. */
+struct CromaHipSynchPlan {
+    const CromaHipContext *ctx;
+    int n, nfreq, nlogb;
+    double logb_min, inv_dlogb;
+    char *d_const;
+    double *d_tab, *d_dp;
+    uint16_t *d_hi, *d_lo;
+    int *d_cs;
+    float *d_lt;
+    char *d_run;
+    long cap;
+    double *d_cre, *d_b, *d_eps, *d_s, *d_pref;
+    int *d_work, *d_jend, *d_srow, *d_fix;
+    void *stream, *e0, *e1;
+};
+
+#define SY_R(b) ((((size_t)(b)) + 255) & ~(size_t)255)
+
+void croma_hip_synch_plan_destroy(CromaHipSynchPlan *p)
+{
+    if (!p) return;
+    if (p->stream) ck_stream_sync(p->stream);
+    ck_free(p->d_const);
+    ck_free(p->d_run);
+    if (p->e0) ck_event_destroy(p->e0);
+    if (p->e1) ck_event_destroy(p->e1);
+    if (p->stream) ck_stream_destroy(p->stream);
+    free(p);
+}
+
+CromaHipSynchPlan *croma_hip_synch_plan_create(const CromaHipContext *ctx, int nfreq, int nlogb, double logb_min,
+                                               double inv_dlogb, const double *table, double *elapsed_ms)
+{
+    CromaHipSynchPlan *p;
+    const ChGrid *g;
+    size_t ntab, nkt, ncs;
+    char *q;
+    if (!ctx || nfreq <= 0 || nlogb < 2 || !table || !isfinite(logb_min) || !isfinite(inv_dlogb)) {
+        ch_set_error("synch plan: invalid arguments");
+        return 0;
+    }
+    g = ch_ctx_grid(ctx);
+    if (g->n > 256 || !(inv_dlogb >= 50.0)) {
+        ch_set_error("synch MFMA: needs n <= 256 and planes no coarser than 0.02 in log10 B (n %d, 1/dlogb %g)", g->n,
+                     inv_dlogb);
+        return 0;
+    }
+    p = (CromaHipSynchPlan *)calloc(1, sizeof(*p));
+    if (!p) return 0;
+    p->ctx = ctx;
+    p->n = g->n;
+    p->nfreq = nfreq;
+    p->nlogb = nlogb;
+    p->logb_min = logb_min;
+    p->inv_dlogb = inv_dlogb;
+    ntab = (size_t)nlogb * nfreq * p->n;
+    nkt = (size_t)(nlogb - 1) * nfreq * 8 * p->n;
+    ncs = (size_t)(nlogb - 1) * nfreq * 8;
+    p->d_const = (char *)ck_malloc(SY_R(ntab * 8) + SY_R((size_t)p->n * 8) + 2 * SY_R(nkt * 2) + SY_R(ncs * 4) +
+                                   SY_R(ntab * 4));
+    p->stream = ck_stream_create();
+    p->e0 = ck_event_create();
+    p->e1 = ck_event_create();
+    if (!p->d_const || !p->stream || !p->e0 || !p->e1) {
+        ch_set_error("synch plan: device allocation: %s", ck_last_error());
+        croma_hip_synch_plan_destroy(p);
+        return 0;
+    }
+    q = p->d_const;
+    p->d_tab = (double *)q;
+    q += SY_R(ntab * 8);
+    p->d_dp = (double *)q;
+    q += SY_R((size_t)p->n * 8);
+    p->d_hi = (uint16_t *)q;
+    q += SY_R(nkt * 2);
+    p->d_lo = (uint16_t *)q;
+    q += SY_R(nkt * 2);
+    p->d_cs = (int *)q;
+    q += SY_R(ncs * 4);
+    p->d_lt = (float *)q;
+    if (ck_h2d(p->d_tab, table, ntab * 8, p->stream) || ck_h2d(p->d_dp, g->edp, (size_t)p->n * 8, p->stream) ||
+        ck_event_record(p->e0, p->stream) ||
+        ck_synch_mfma_prepare(p->n, nfreq, nlogb, p->d_tab, p->d_dp, p->d_hi, p->d_lo, p->d_cs, p->d_lt, p->stream) ||
+        ck_event_record(p->e1, p->stream) || ck_stream_sync(p->stream)) {
+        ch_set_error("synch plan: %s", ck_last_error());
+        croma_hip_synch_plan_destroy(p);
+        return 0;
+    }
+    if (elapsed_ms) *elapsed_ms = ck_event_elapsed_ms(p->e0, p->e1);
+    return p;
+}
+
+int croma_hip_synch_plan_run(CromaHipSynchPlan *p, int ntr, const double *cre, const double *b_field, double *eps,
+                             double *elapsed_ms)
+{
+    const int n = p ? p->n : 0;
+    if (!p || ntr <= 0 || !cre || !b_field || !eps) return CROMA_HIP_EINVAL;
+    if (ntr > p->cap) {
+        const size_t nt = (size_t)ntr;
+        char *q;
+        ck_stream_sync(p->stream);
+        ck_free(p->d_run);
+        p->cap = 0;
+        p->d_run = (char *)ck_malloc(SY_R(nt * n * 8) + SY_R(nt * 8) + SY_R(nt * p->nfreq * 8) +
+                                     SY_R((2 * nt + p->nlogb) * 4) + SY_R(nt * 4) + 2 * SY_R(nt * 8) + SY_R(nt * 4) +
+                                     SY_R((nt + 1) * 4));
+        if (!p->d_run) {
+            ch_set_error("synch plan: device allocation: %s", ck_last_error());
+            return CROMA_HIP_EDEVICE;
+        }
+        q = p->d_run;
+        p->d_cre = (double *)q;
+        q += SY_R(nt * n * 8);
+        p->d_b = (double *)q;
+        q += SY_R(nt * 8);
+        p->d_eps = (double *)q;
+        q += SY_R(nt * p->nfreq * 8);
+        p->d_work = (int *)q;
+        q += SY_R((2 * nt + p->nlogb) * 4);
+        p->d_jend = (int *)q;
+        q += SY_R(nt * 4);
+        p->d_s = (double *)q;
+        q += SY_R(nt * 8);
+        p->d_pref = (double *)q;
+        q += SY_R(nt * 8);
+        p->d_srow = (int *)q;
+        q += SY_R(nt * 4);
+        p->d_fix = (int *)q;
+        p->cap = ntr;
+    }
+    if (ck_h2d(p->d_cre, cre, (size_t)ntr * n * 8, p->stream) ||
+        ck_h2d(p->d_b, b_field, (size_t)ntr * 8, p->stream) || ck_event_record(p->e0, p->stream) ||
+        ck_synch_mfma_run(ntr, n, p->nfreq, p->nlogb, p->logb_min, p->inv_dlogb, p->d_tab, p->d_hi, p->d_lo, p->d_cs,
+                          p->d_lt, p->d_work, p->d_jend, p->d_s, p->d_pref, p->d_srow, p->d_fix, p->d_cre, p->d_dp,
+                          p->d_b, p->d_eps, p->stream) ||
+        ck_event_record(p->e1, p->stream) || ck_d2h(eps, p->d_eps, (size_t)ntr * p->nfreq * 8, p->stream) ||
+        ck_stream_sync(p->stream)) {
+        ch_set_error("synch MFMA: %s", ck_last_error());
+        return CROMA_HIP_EDEVICE;
+    }
+    if (elapsed_ms) *elapsed_ms = ck_event_elapsed_ms(p->e0, p->e1);
+    return 0;
+}
+#undef SY_R
+
 static int synch_mfma(const CromaHipContext *ctx, int ntr, int nfreq, int nlogb, double logb_min, double inv_dlogb,
                       const double *table, const double *cre, const double *b_field, double *eps, double *elapsed_ms)
 {
-    const ChGrid *g = ch_ctx_grid(ctx);
-    const int n = g->n;
-    const size_t ntab = (size_t)nlogb * nfreq * n, nkt = (size_t)(nlogb - 1) * nfreq * 8 * n;
-    const size_t ncs = (size_t)(nlogb - 1) * nfreq * 8;
-    char *d = 0, *p;
-    void *st = 0, *e0 = 0, *e1 = 0;
-    int rc = 0;
-    if (n > 256 || !(inv_dlogb >= 50.0)) {
-        ch_set_error("synch MFMA: needs n <= 256 and planes no coarser than 0.02 in log10 B (n %d, 1/dlogb %g)", n,
-                     inv_dlogb);
-        return CROMA_HIP_EUNSUPPORTED;
-    }
-#define SY_R(b) ((((size_t)(b)) + 255) & ~(size_t)255)
-    d = (char *)ck_malloc(SY_R(ntab * 8) + 2 * SY_R(nkt * 2) + SY_R(ncs * 4) + SY_R((size_t)ntr * n * 8) +
-                          SY_R((size_t)n * 8) + SY_R((size_t)ntr * 8) + SY_R((size_t)ntr * nfreq * 8) +
-                          SY_R((2 * (size_t)ntr + nlogb) * 4) + SY_R((size_t)ntr * 4) + 2 * SY_R((size_t)ntr * 8) +
-                          SY_R((size_t)ntr * 4) + SY_R(((size_t)ntr + 1) * 4) + SY_R(ntab * 4));
-    st = ck_stream_create();
-    e0 = ck_event_create();
-    e1 = ck_event_create();
-    if (!d || !st || !e0 || !e1) {
-        rc = CROMA_HIP_EDEVICE;
-        ch_set_error("synch MFMA: device allocation: %s", ck_last_error());
-        goto out;
-    }
-    {
-        double *d_tab, *d_cre, *d_dp, *d_b, *d_eps, *d_s, *d_pref;
-        uint16_t *d_hi, *d_lo;
-        int *d_cs, *d_work, *d_jend, *d_srow, *d_fix;
-        float *d_lt;
-        p = d;
-        d_tab = (double *)p;
-        p += SY_R(ntab * 8);
-        d_hi = (uint16_t *)p;
-        p += SY_R(nkt * 2);
-        d_lo = (uint16_t *)p;
-        p += SY_R(nkt * 2);
-        d_cs = (int *)p;
-        p += SY_R(ncs * 4);
-        d_cre = (double *)p;
-        p += SY_R((size_t)ntr * n * 8);
-        d_dp = (double *)p;
-        p += SY_R((size_t)n * 8);
-        d_b = (double *)p;
-        p += SY_R((size_t)ntr * 8);
-        d_eps = (double *)p;
-        p += SY_R((size_t)ntr * nfreq * 8);
-        d_work = (int *)p;
-        p += SY_R((2 * (size_t)ntr + nlogb) * 4);
-        d_jend = (int *)p;
-        p += SY_R((size_t)ntr * 4);
-        d_s = (double *)p;
-        p += SY_R((size_t)ntr * 8);
-        d_pref = (double *)p;
-        p += SY_R((size_t)ntr * 8);
-        d_srow = (int *)p;
-        p += SY_R((size_t)ntr * 4);
-        d_fix = (int *)p;
-        p += SY_R(((size_t)ntr + 1) * 4);
-        d_lt = (float *)p;
-#undef SY_R
-        if (ck_h2d(d_tab, table, ntab * 8, st) || ck_h2d(d_cre, cre, (size_t)ntr * n * 8, st) ||
-            ck_h2d(d_dp, g->edp, (size_t)n * 8, st) || ck_h2d(d_b, b_field, (size_t)ntr * 8, st) ||
-            ck_fill_f64(d_eps, 0.0, (size_t)ntr * nfreq, st) || ck_event_record(e0, st) ||
-            ck_emission_synch_mfma(ntr, n, nfreq, nlogb, logb_min, inv_dlogb, d_tab, d_hi, d_lo, d_cs, d_work, d_jend,
-                                   d_s, d_pref, d_srow, d_fix, d_lt, d_cre, d_dp, d_b, d_eps, st) ||
-            ck_event_record(e1, st) || ck_d2h(eps, d_eps, (size_t)ntr * nfreq * 8, st) || ck_stream_sync(st)) {
-            rc = CROMA_HIP_EDEVICE;
-            ch_set_error("synch MFMA: %s", ck_last_error());
-            goto out;
-        }
-        if (elapsed_ms) *elapsed_ms = ck_event_elapsed_ms(e0, e1);
-    }
-out:
-    if (st) ck_stream_sync(st);
-    ck_free(d);
-    ck_event_destroy(e0);
-    ck_event_destroy(e1);
-    if (st) ck_stream_destroy(st);
+    double t0 = 0.0, t1 = 0.0;
+    int rc;
+    CromaHipSynchPlan *p = croma_hip_synch_plan_create(ctx, nfreq, nlogb, logb_min, inv_dlogb, table, &t0);
+    if (!p) return (ch_ctx_grid(ctx)->n > 256 || !(inv_dlogb >= 50.0)) ? CROMA_HIP_EUNSUPPORTED : CROMA_HIP_EDEVICE;
+    rc = croma_hip_synch_plan_run(p, ntr, cre, b_field, eps, &t1);
+    croma_hip_synch_plan_destroy(p);
+    if (!rc && elapsed_ms) *elapsed_ms = t0 + t1;
     return rc;
 }
 

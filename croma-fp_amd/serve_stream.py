@@ -174,13 +174,17 @@ def run_closed_loop(specs, budget_gb=64.0):
             "wall_s": wall}
 
 
-def run_sequential(specs, path=ch.PATH_FUSED, params=None, verbose=False):
-    """The same chunks one at a time through the batch API (no overlap)."""
+def run_sequential(specs, path=ch.PATH_FUSED, params=None, verbose=False, budget_s=None):
+    """The same chunks one at a time through the batch API (no overlap).
+    budget_s: stop after the chunk that takes the total past it (the result
+    counts the chunks run, a prefix of specs)."""
     params = params or server_params()
     ctxs = {}
-    steps, wall = 0, 0.0
+    steps, wall, done = 0, 0.0, 0
     try:
         for i, spec in enumerate(specs):
+            if budget_s is not None and done and wall >= budget_s:
+                break
             w, kw = make_chunk(spec)
             if w.n not in ctxs:
                 ctxs[w.n] = ch.Context(w.n, params)
@@ -199,11 +203,12 @@ def run_sequential(specs, path=ch.PATH_FUSED, params=None, verbose=False):
             dt = time.perf_counter() - t0
             wall += dt
             steps += t.cell_steps
+            done += 1
             if verbose:
                 print(f"[serve] sequential {'plain' if path == ch.PATH_PLAIN else 'fused'} chunk {i + 1}/{len(specs)}: "
                       f"{w.ntracer} tracers x {w.n} bins, {dt:.2f} s", file=sys.stderr, flush=True)
     finally:
         for c in ctxs.values():
             c.close()
-    return {"chunks": len(specs), "throughput_substeps_per_s": steps / wall, "tracer_substeps": int(steps),
+    return {"chunks": done, "throughput_substeps_per_s": steps / wall, "tracer_substeps": int(steps),
             "wall_s": wall}

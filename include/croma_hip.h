@@ -332,6 +332,24 @@ int croma_hip_emission_synch_mode(const CromaHipContext *ctx, int ntracer, int n
                                   double logb_min, double inv_dlogb, const double *table, const double *cre,
                                   const double *b_field, double *eps, int mode, double *elapsed_ms);
 
+/* Synchrotron on MFMA with the table's device form kept across calls: a plan
+ * builds the Chebyshev coefficients of the log-B interpolation (8 terms per
+ * plane interval, bin and frequency, split into two BF16 parts) and the float
+ * log2 table of the range fix-up once; each run then sorts its tracers by
+ * plane, contracts them on MFMA (split-BF16 x3, FP32 accumulate, 2^59 operand
+ * scales) and recomputes in the FP32 log-domain form every tracer whose
+ * emissivities within 25 decades of its peak fall below the contraction's
+ * range (emissivities within 1e-4 of the FP64 sum).  Needs n <= 256 and
+ * planes no coarser than 0.02 in log10 B.  The same computation as
+ * croma_hip_emission_synch_mode(CROMA_HIP_EMIS_MFMA_XF32), which builds and
+ * drops a plan per call.  A plan belongs to one thread at a time. */
+typedef struct CromaHipSynchPlan CromaHipSynchPlan;
+CromaHipSynchPlan *croma_hip_synch_plan_create(const CromaHipContext *ctx, int nfreq, int nlogb, double logb_min,
+                                               double inv_dlogb, const double *table, double *elapsed_ms);
+int croma_hip_synch_plan_run(CromaHipSynchPlan *plan, int ntracer, const double *cre, const double *b_field,
+                             double *eps, double *elapsed_ms);
+void croma_hip_synch_plan_destroy(CromaHipSynchPlan *plan);
+
 /* ---- output and checkpoint files (tracer_fp_output.c, tracer_fp_restart.c)
  * The reference's byte layouts, float64 native-endian:
  *   CROMA_HIP_OUT_MAP   <dir>/<stem>_core%02d.bin       [tracer][slot][nbin]

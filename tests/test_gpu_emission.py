@@ -232,6 +232,38 @@ def test_synch_mfma_matches_oracle(n):
     assert err < 1e-4
 
 
+def test_synch_plan_reuse_matches_one_shot():
+    """A plan built once serves calls of different sizes (its buffers grow,
+    then shrink to a prefix) with results identical to the one-shot mode."""
+    fx = np.load(os.path.join(GOLD, "fx_table.npz"))
+    n = 128
+    o = po.Oracle(n)
+    nfreq, ntheta, nlogb, logb_min, inv = 32, 12, 256, -3.0, 256 / 4.0
+    logy, pw = po.synch_logy(o, nfreq, 7.0, 11.0, ntheta)
+    rng = np.random.default_rng(41)
+    nt = 2000
+    q = rng.uniform(2.0, 3.5, nt)
+    cre = (o.ep[None, :] ** -q[:, None]) * np.exp(-o.ep[None, :] / 10 ** rng.uniform(2, 6, nt)[:, None])
+    b = 10 ** rng.uniform(-7.5, -5.5, nt)
+    ctx = ch.Context(n, ch.default_params())
+    try:
+        tab, _ = ctx.synch_kernel_table(nfreq, ntheta, nlogb, logb_min, inv, fx["logx"], fx["logfx"], fx["fx"],
+                                        float(fx["xmin"][0]), logy, pw)
+        one, _ = ctx.emission_synch(cre, b, tab, nfreq, nlogb, logb_min, inv, mode=ch.EMIS_MFMA_XF32)
+        plan = ch.SynchPlan(ctx, tab, nfreq, nlogb, logb_min, inv)
+        try:
+            small, _ = plan.run(cre[:300], b[:300])
+            big, ms = plan.run(cre, b)
+            again, _ = plan.run(cre[:300], b[:300])
+        finally:
+            plan.close()
+    finally:
+        ctx.close()
+    print(f"[emission] synch plan: prepare {plan.prepare_ms:.3f} ms, run {ms:.3f} ms", flush=True)
+    assert np.array_equal(big, one)
+    assert np.array_equal(small, one[:300]) and np.array_equal(again, one[:300])
+
+
 def test_synch_mfma_refuses_coarse_planes():
     """Planes coarser than 0.02 in log10 B leave the Chebyshev truncation
     unbounded: the MFMA mode refuses them (the FP32 VALU mode serves them)."""
