@@ -487,6 +487,7 @@ struct Factor {
     double Af[LV + 1];  // forward scan multipliers, zero where the source lane is outside the row / group
     double Qb[LV + 1];  // backward scan multipliers
     double PA, PQ;      // groups wider than a row: in-row inclusive products (prefix of A, suffix of Q)
+    double PQ0, PQ2;    // two-row groups: PQ in rows 0 / 2 only (0 elsewhere)
     double Tf[4], Tb[4];  // and (one group per wave) the row totals of those products
 };
 
@@ -678,7 +679,10 @@ __device__ __forceinline__ void factorize(int gl, int n, const double (&sv)[R], 
         const double inv = d_div_normal(1.0, den);
         const double cp = (base + k == n - 1) ? 0.0 : -c * inv;
         F.iv[k] = inv;
-        F.al[k] = a * inv;
+        // the group's first row has no predecessor: its multiplier is 0, so
+        // the value a substep shifts in from the lane before (another
+        // tracer's lanes, or 0) drops out without a select
+        F.al[k] = (k == 0 && gl == 0) ? 0.0 : a * inv;
         F.vs[k] = inv * sv[k];
         F.g[k] = -cp;
         A *= F.al[k];
@@ -691,6 +695,11 @@ __device__ __forceinline__ void factorize(int gl, int n, const double (&sv)[R], 
     scan_coeffs<G, 0>(gl, rl, A, Bq, F);
     F.PA = A;
     F.PQ = Bq;
+    if constexpr (G == 32) {
+        const int row = (int)((threadIdx.x & 63) >> 4);
+        F.PQ0 = (row == 0) ? Bq : 0.0;
+        F.PQ2 = (row == 2) ? Bq : 0.0;
+    }
     if constexpr (G == 64) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
@@ -751,8 +760,8 @@ __device__ __forceinline__ void apply_solve(int gl, const Factor<R, G> &F, doubl
         const double C = (row == 1) ? C1 : (row == 2) ? C2 : (row == 3) ? C3 : 0.0;
         L = fma(F.PA, C, L);
     }
+    // the first lane of the group multiplies the shifted-in value by al[0] = 0
     double d = wave_up1(L);
-    if (gl == 0) d = 0.0;
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         d = fma(F.al[k], d, t[k]);
@@ -763,11 +772,11 @@ __device__ __forceinline__ void apply_solve(int gl, const Factor<R, G> &F, doubl
     for (int k = R - 2; k >= 0; --k) M = fma(F.g[k], M, t[k]);
     scan_bwd<G, 0, Factor<R, G>::LV>(F.Qb, M);
     if constexpr (G == 32) {
-        // rows 0 and 2 take the value at the first lane of the row after
+        // rows 0 and 2 take the value at the first lane of the row after:
+        // the wave-uniform values at lanes 16 and 48 times multipliers that
+        // are zero outside rows 0 resp. 2
         const double U1 = rdlane64(M, 16), U3 = rdlane64(M, 48);
-        const int row = (int)((threadIdx.x & 63) >> 4);
-        const double D = (row & 1) ? 0.0 : ((row & 2) ? U3 : U1);
-        M = fma(F.PQ, D, M);
+        M = fma(F.PQ0, U1, fma(F.PQ2, U3, M));
     } else if constexpr (G == 64) {
         const double U1 = rdlane64(M, 16), U2 = rdlane64(M, 32), U3 = rdlane64(M, 48);
         const double D2 = U3;
@@ -777,8 +786,9 @@ __device__ __forceinline__ void apply_solve(int gl, const Factor<R, G> &F, doubl
         const double D = (row == 0) ? D0 : (row == 1) ? D1 : (row == 2) ? D2 : 0.0;
         M = fma(F.PQ, D, M);
     }
+    // the group's last row has g = 0 (the system's last row), so the value
+    // shifted in from the lane after drops out without a select
     double xn = wave_dn1(M);
-    if (gl == G - 1) xn = 0.0;
 #pragma unroll
     for (int k = R - 1; k >= 0; --k) {
         xn = fma(F.g[k], xn, t[k]);
