@@ -136,14 +136,13 @@ void *ck_graph_end(void *stream);          /* returns executable graph or NULL *
 int ck_graph_launch(void *graph_exec, void *stream);
 int ck_graph_destroy(void *graph_exec);
 
-/* FP64 GEMM through the vendor library (plain GEMM of the split
- * secondary-source contraction): C = A B, column-major. */
-void *ck_blas_create(void *stream);
-void ck_blas_destroy(void *handle);
-int ck_dgemm_nn(void *handle, int m, int n, int k, const double *A, int lda, const double *B, int ldb, double *C,
-                int ldc, int solution);
-int ck_dgemm_tune(void *handle, int m, int n, int k, const double *A, int lda, const double *B, int ldb, double *C,
-                  int ldc, void *stream, int *best, float *best_ms, float *default_ms);
+/* Secondary-electron contraction of the split two-species launches
+ * (croma_secondary.hip): inje[r][je] = sum_{jp >= np_min[je]} fqe_t[jp][je]
+ * rec[r][jp] for r < rows, rows = *d_rows (device count, capped at rows_cap)
+ * or rows_cap when d_rows is NULL; FP64 MFMA, at most nblocks blocks looping
+ * over row tiles.  n in {64, 128, 256, 512}. */
+int ck_sec_contract(int n, const double *fqe_t, const int *np_min, const double *rec, double *inje, long rows_cap,
+                    const int *d_rows, int nblocks, void *stream);
 
 /* Inverse-Compton cooling rows on the device: for each of nz redshifts,
  * zb[3 iz .. 3 iz + 2] = {z, e0min, e0max} (CMB bounds from the host),

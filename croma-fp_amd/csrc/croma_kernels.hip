@@ -625,112 +625,3 @@ extern "C" int ck_graph_destroy(void *ex)
     return ex ? hip_ok(hipGraphExecDestroy((hipGraphExec_t)ex), "hipGraphExecDestroy") : 0;
 }
 
-// ---------------------------------------------------------------------------
-// Library GEMM for the secondary-source contraction of the split launches.
-// ---------------------------------------------------------------------------
-#define ROCBLAS_BETA_FEATURES_API
-#include <rocblas/rocblas.h>
-#include <stdlib.h>
-
-extern "C" void *ck_blas_create(void *stream)
-{
-    rocblas_handle h = 0;
-    if (rocblas_create_handle(&h) != rocblas_status_success) return 0;
-    if (rocblas_set_stream(h, (hipStream_t)stream) != rocblas_status_success ||
-        rocblas_set_pointer_mode(h, rocblas_pointer_mode_host) != rocblas_status_success ||
-        rocblas_set_atomics_mode(h, rocblas_atomics_not_allowed) != rocblas_status_success) {
-        rocblas_destroy_handle(h);
-        return 0;
-    }
-    return (void *)h;
-}
-
-extern "C" void ck_blas_destroy(void *h)
-{
-    if (h) rocblas_destroy_handle((rocblas_handle)h);
-}
-
-// C (m x n, column-major, ldc) = A (m x k, lda) * B (k x n, ldb), FP64.
-// solution 0: the library's own choice; otherwise a solution index that
-// ck_dgemm_tune returned for this shape.
-extern "C" int ck_dgemm_nn(void *h, int m, int n, int k, const double *A, int lda, const double *B, int ldb,
-                           double *Cm, int ldc, int solution)
-{
-    const double one = 1.0, zero = 0.0;
-    if (!h || m <= 0 || k <= 0 || n < 0) return -1;
-    if (n == 0) return 0;
-    const rocblas_status st =
-        rocblas_gemm_ex((rocblas_handle)h, rocblas_operation_none, rocblas_operation_none, m, n, k, &one, A,
-                        rocblas_datatype_f64_r, lda, B, rocblas_datatype_f64_r, ldb, &zero, Cm, rocblas_datatype_f64_r,
-                        ldc, Cm, rocblas_datatype_f64_r, ldc, rocblas_datatype_f64_r,
-                        solution ? rocblas_gemm_algo_solution_index : rocblas_gemm_algo_standard, solution,
-                        rocblas_gemm_flags_none);
-    if (st != rocblas_status_success) {
-        snprintf(g_errbuf, sizeof(g_errbuf), "rocblas_gemm_ex: status %d", (int)st);
-        return -1;
-    }
-    return 0;
-}
-
-// Times the library's choice and every solution it lists for the shape
-// (the handle disallows atomics, so the listed solutions are deterministic)
-// on the caller's buffers; C is overwritten.  *best = 0 keeps the library's
-// choice unless a listed solution is more than 2% faster.
-extern "C" int ck_dgemm_tune(void *h, int m, int n, int k, const double *A, int lda, const double *B, int ldb,
-                             double *Cm, int ldc, void *stream, int *best, float *best_ms, float *default_ms)
-{
-    const double one = 1.0, zero = 0.0;
-    rocblas_handle hh = (rocblas_handle)h;
-    hipStream_t st = (hipStream_t)stream;
-    rocblas_int cnt = 0;
-    hipEvent_t e0 = 0, e1 = 0;
-    int rc = -1;
-    if (!h || m <= 0 || n <= 0 || k <= 0 || !best) return -1;
-    *best = 0;
-    if (rocblas_gemm_ex_get_solutions(hh, rocblas_operation_none, rocblas_operation_none, m, n, k, &one, A,
-                                      rocblas_datatype_f64_r, lda, B, rocblas_datatype_f64_r, ldb, &zero, Cm,
-                                      rocblas_datatype_f64_r, ldc, Cm, rocblas_datatype_f64_r, ldc,
-                                      rocblas_datatype_f64_r, rocblas_gemm_algo_solution_index, rocblas_gemm_flags_none,
-                                      NULL, &cnt) != rocblas_status_success)
-        cnt = 0;
-    rocblas_int *sol = (rocblas_int *)malloc(sizeof(rocblas_int) * (size_t)(cnt > 0 ? cnt : 1));
-    if (!sol || hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) goto done;
-    if (cnt > 0 &&
-        rocblas_gemm_ex_get_solutions(hh, rocblas_operation_none, rocblas_operation_none, m, n, k, &one, A,
-                                      rocblas_datatype_f64_r, lda, B, rocblas_datatype_f64_r, ldb, &zero, Cm,
-                                      rocblas_datatype_f64_r, ldc, Cm, rocblas_datatype_f64_r, ldc,
-                                      rocblas_datatype_f64_r, rocblas_gemm_algo_solution_index, rocblas_gemm_flags_none,
-                                      sol, &cnt) != rocblas_status_success)
-        cnt = 0;
-    {
-        float dms = -1.0f, bms = 1e30f;
-        for (int i = -1; i < (int)cnt; ++i) {
-            const int idx = i < 0 ? 0 : (int)sol[i];
-            float ms = 0.0f;
-            int bad = 0;
-            for (int r = 0; r < 4 && !bad; ++r) {
-                if (r == 1 && hipEventRecord(e0, st) != hipSuccess) bad = 1;
-                if (ck_dgemm_nn(h, m, n, k, A, lda, B, ldb, Cm, ldc, idx)) bad = 1;
-            }
-            if (bad || hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
-                if (i < 0) goto done;
-                continue;
-            }
-            ms /= 3.0f;
-            if (i < 0) dms = ms;
-            else if (ms < bms) {
-                bms = ms;
-                if (ms < 0.98f * dms) *best = idx;
-            }
-        }
-        if (default_ms) *default_ms = dms;
-        if (best_ms) *best_ms = *best ? bms : dms;
-    }
-    rc = 0;
-done:
-    if (e0) (void)hipEventDestroy(e0);
-    if (e1) (void)hipEventDestroy(e1);
-    free(sol);
-    return rc;
-}

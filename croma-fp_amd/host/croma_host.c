@@ -175,9 +175,6 @@ struct CromaHipContext {
     int jp0;
     int *h_npmin;
     double *d_fqe;
-    void *blas; /* vendor GEMM handle for the split secondary contraction */
-    int gemm_sol;       /* its solution for gemm_sol_rows rows (0: the library's choice) */
-    long gemm_sol_rows;
     /* the last IC row a window built (device, n doubles) and its (z, lo, hi):
      * the next window's first snapshot is usually that z */
     double *d_ic_last;
@@ -406,7 +403,6 @@ void croma_hip_context_destroy(CromaHipContext *c)
     ck_free(c->d_fqe);
     ck_free(c->d_ic_last);
     ck_free(c->pws_mem);
-    ck_blas_destroy(c->blas);
     free(c->h_npmin);
     if (c->stream_e) ck_stream_destroy(c->stream_e);
     if (c->stream) ck_stream_destroy(c->stream);
@@ -713,6 +709,8 @@ const CkParams *ch_ctx_prm(const CromaHipContext *c) { return &c->prm; }
 int ch_ctx_cu_count(const CromaHipContext *c) { return c->cu_count; }
 CromaHipTile ch_ctx_tile(const CromaHipContext *c, int protons) { return c->tile[protons ? 1 : 0]; }
 const double *ch_ctx_d_fqe(const CromaHipContext *c) { return c->d_fqe; }
+/* np_min of the secondary weights on the device (after the weights and beta_p) */
+const int *ch_ctx_d_npmin(const CromaHipContext *c) { return c->d_fqe ? (const int *)(c->d_fqe + (size_t)c->g.n * c->g.n + c->g.n) : 0; }
 int ch_ctx_jp0(const CromaHipContext *c) { return c->jp0; }
 
 /* ------------------------------------------------------------------ */
@@ -836,7 +834,7 @@ typedef struct {
     /* pipelined split windows: the GEMM and the electron pass have graphs
      * of their own (they are issued after waits on the other stream) */
     ChGraph graph_g, graph_e;
-    void *ev_g, *ev_e;       /* GEMM done (library stream), electron pass done (electron stream) */
+    void *ev_g, *ev_e;       /* contraction done, electron pass done (electron stream) */
     int dev_nsub;            /* nsub, n_on, order and segment rows are built on the device */
     int *d_key, *d_segloc, *d_blk, *d_hist; /* their work buffers */
 } WinBuf;
@@ -1237,8 +1235,8 @@ static int prepare_window(CromaHipBatch *b, WinBuf *w, int s_first, int count, i
         ps->cell_steps += cs;
         ps->species_steps += cs * (b->protons ? 2 : 1);
     } else {
-        /* the device assigns the segment rows: the GEMM covers every row the
-         * batch can hold (rows past this window's count are stale and unread) */
+        /* the device assigns the segment rows: the contraction's launch
+         * covers the batch's capacity and stops at the device-side count */
         w->segq = split ? b->segrows_cap : 0;
     }
     w->count = count;
@@ -1335,22 +1333,21 @@ static int launch_tasks(CromaHipContext *c, int protons, const CkSlot *d_slot, c
 #define CH_CTR_E 32
 
 /* Split two-species window: proton pass recording the segment-start rows,
- * one GEMM of all rows against the secondary weights (rows jp0..n-1), then
- * the electron pass. */
+ * the contraction of every row against the secondary weights
+ * (croma_secondary.hip), then the electron pass.  With the schedule built on
+ * the device the contraction reads the row count from the window's segment
+ * total (d_blk[nblk], written by the ordering kernels) and w->segq is the
+ * batch's capacity; with the host schedule w->segq is the count rounded up
+ * to the row quantum (rows past the count hold stale finite rows whose
+ * products are never read).  Either way the launch depends only on w->segq,
+ * which keys the window's graph. */
 static int launch_gemm(CromaHipBatch *b, WinBuf *w)
 {
     CromaHipContext *c = b->ctx;
-    const int n = c->g.n, jp0 = c->jp0;
-    /* rows segrows..segq-1 (the rounding to the row quantum, which keeps the
-     * GEMM shape and so the launch graph fixed while segrows drifts) hold
-     * stale rows whose products are never read */
-    if (!c->blas) return dev_fail("GEMM handle");
-    if (w->segq > 0 && jp0 < n &&
-        ck_dgemm_nn(c->blas, n, (int)w->segq, n - jp0, c->d_fqe + (size_t)jp0 * n, n, b->d_segrec + jp0, n,
-                    b->d_seginje, n, w->segq == c->gemm_sol_rows ? c->gemm_sol : 0))
-        return dev_fail("secondary GEMM");
-    if (w->segq > 0 && jp0 >= n && ck_fill_f64(b->d_seginje, 0.0, (size_t)w->segq * n, c->stream))
-        return dev_fail("secondary clear");
+    const int nblk = (b->ntr + 255) / 256;
+    if (w->segq > 0 && ck_sec_contract(c->g.n, c->d_fqe, ch_ctx_d_npmin(c), b->d_segrec, b->d_seginje, w->segq,
+                                       w->dev_nsub ? w->d_blk + nblk : 0, 8 * c->cu_count, c->stream))
+        return dev_fail("secondary contraction");
     return 0;
 }
 
@@ -1414,7 +1411,7 @@ static int launch_window(CromaHipBatch *b, WinBuf *w, WinBuf *prev, int last, Gr
     key[4] = (long)(size_t)b->d_segrec;
     key[5] = b->protons | ((long)c->coeff_fp32 << 1);
     key[6] = c->jp0 | ((long)w->dev_nsub << 20) | ((long)w->count << 21);
-    key[7] = b->ntr | ((long)(w->segq == c->gemm_sol_rows ? c->gemm_sol : 0) << 32);
+    key[7] = b->ntr;
     wi.b = b;
     wi.w = w;
     wi.pipe = pipe;
@@ -1713,7 +1710,6 @@ int croma_hip_batch_run(CromaHipBatch *b, int s0, int s1, int nsnap_total, const
                 return dev_fail("segment rows");
         }
         b->seg_quantum = q;
-        if (!c->blas && !(c->blas = ck_blas_create(c->stream))) return dev_fail("GEMM handle");
     }
     nwin = (s1 - s0 + wsz - 1) / wsz;
     for (k = 0; k < 2; k++)
@@ -2177,20 +2173,6 @@ int croma_hip_batch_autotune(CromaHipBatch *b, int s, int nsnap_total, const dou
             }
         }
         bt.chain = (cms[1] < 0.99 * cms[0]) ? 2 : 1;
-    }
-    /* the split GEMM's solution for the batch's row capacity (every window
-     * runs the GEMM over all of it) */
-    if (b->protons && bt.secondary_gemm && b->segrows_cap > 0 && c->blas && c->jp0 < c->g.n &&
-        b->segrows_cap <= 0x7fffffffL) {
-        const int n = c->g.n, jp0 = c->jp0;
-        int sol = 0;
-        if (ck_dgemm_tune(c->blas, n, (int)b->segrows_cap, n - jp0, c->d_fqe + (size_t)jp0 * n, n, b->d_segrec + jp0,
-                          n, b->d_seginje, n, c->stream, &sol, 0, 0)) {
-            rc = dev_fail("GEMM solution tuning");
-            goto out;
-        }
-        c->gemm_sol = sol;
-        c->gemm_sol_rows = b->segrows_cap;
     }
 out:
     if (d_save) {

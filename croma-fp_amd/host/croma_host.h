@@ -56,6 +56,7 @@ CromaHipTile ch_ctx_tile(const CromaHipContext *c, int protons);
 int ch_ctx_coeff_fp32(const CromaHipContext *c); /* FP32 coefficient evaluation selected */
 int ch_ctx_fqe(CromaHipContext *c);              /* builds the secondary weights once */
 const double *ch_ctx_d_fqe(const CromaHipContext *c);
+const int *ch_ctx_d_npmin(const CromaHipContext *c);
 int ch_ctx_jp0(const CromaHipContext *c);
 int ch_ctx_plain_ws(CromaHipContext *c, int count, CkPlainWs *out); /* plain-path workspace of >= count tracers */
 void *ch_ctx_stream(const CromaHipContext *c);

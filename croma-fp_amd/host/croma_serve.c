@@ -77,7 +77,6 @@ typedef struct {
     long segoff_cap;
     double *d_segrec, *d_seginje;
     long segrows_cap;
-    void *blas;
 } SvClass;
 
 struct CromaHipServer {
@@ -489,7 +488,6 @@ static int sv_launch_class(CromaHipServer *s, int ci)
             k->segrows_cap = (k->d_segrec && k->d_seginje) ? cap : 0;
             if (!k->segrows_cap) return CROMA_HIP_ENOMEM;
         }
-        if (!k->blas && !(k->blas = ck_blas_create(k->stream))) return CROMA_HIP_EDEVICE;
     }
     if (ntasks > s->tmp_cap) {
         free(s->tmp_tasks);
@@ -580,7 +578,7 @@ static int sv_launch_class(CromaHipServer *s, int ci)
         const int nblocks = ch_ctx_cu_count(c) * tile.waves_per_simd;
         for (i = 0; i < nl; i++)
             if (ck_stream_wait_event(k->stream, k->launched[i]->ev_ready)) return CROMA_HIP_EDEVICE;
-        const int n = SV_NS[ci >> 1], jp0 = ch_ctx_jp0(c);
+        const int n = SV_NS[ci >> 1];
         int rc = ck_h2d(k->d_slots, k->h_slots, (size_t)nslots * sizeof(CkSlot), k->stream) ||
                  ck_h2d(k->d_tasks, k->h_tasks, (size_t)ntasks * sizeof(CkTask), k->stream) ||
                  (split && ck_h2d(k->d_segoff, k->h_segoff, (size_t)nsegoff * sizeof(int), k->stream)) ||
@@ -591,10 +589,9 @@ static int sv_launch_class(CromaHipServer *s, int ci)
         if (!rc && split) {
             rc = ck_launch_fused(n, tile.rows_per_lane, tile.waves_per_simd, nblocks, 2, *dg, k->d_slots, k->d_tasks,
                                  ntasks, k->d_ctrl, k->d_ctrl + 1, k->stream);
-            if (!rc && segrows > 0 && jp0 < n)
-                rc = ck_dgemm_nn(k->blas, n, (int)segrows, n - jp0, ch_ctx_d_fqe(c) + (size_t)jp0 * n, n,
-                                 k->d_segrec + jp0, n, k->d_seginje, n, 0);
-            if (!rc && segrows > 0 && jp0 >= n) rc = ck_fill_f64(k->d_seginje, 0.0, (size_t)segrows * n, k->stream);
+            if (!rc && segrows > 0)
+                rc = ck_sec_contract(n, ch_ctx_d_fqe(c), ch_ctx_d_npmin(c), k->d_segrec, k->d_seginje, segrows, 0,
+                                     8 * ch_ctx_cu_count(c), k->stream);
             if (!rc) rc = ck_fill_i32(k->d_ctrl, 0, 1, k->stream);
             if (!rc)
                 rc = ck_launch_fused(n, tile.rows_per_lane, tile.waves_per_simd, nblocks, 3, *dg, k->d_slots,
@@ -829,7 +826,6 @@ void croma_hip_serve_destroy(CromaHipServer *s)
         ck_free(k->d_segoff);
         ck_free(k->d_segrec);
         ck_free(k->d_seginje);
-        ck_blas_destroy(k->blas);
         free(k->launched);
         if (k->stream) ck_stream_destroy(k->stream);
     }

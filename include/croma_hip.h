@@ -91,9 +91,10 @@ typedef struct {
     int rows_per_lane;    /* 2, 4, 8 or 16 spectrum rows per lane: n / rows_per_lane lanes (4..64) per tracer */
     int waves_per_simd;   /* 1 or 2: register budget of the kernel and resident persistent waves per SIMD */
     int chain;            /* batch runs: snapshots advanced per queue task (1 = one launch per snapshot) */
-    int secondary_gemm;   /* two species: 1 = proton pass, one GEMM of all segment-start proton rows
-                           * against the secondary weights, electron pass; 0 = contraction inside
-                           * the fused kernel (always used when the eta cap couples the species) */
+    int secondary_gemm;   /* two species: 1 = proton pass, one FP64 MFMA contraction of all
+                           * segment-start proton rows against the secondary weights, electron
+                           * pass; 0 = contraction inside the fused kernel (always used when the
+                           * eta cap couples the species) */
 } CromaHipTile;
 
 /* Stage timers, the subset of TracerFpGpuTimes (include/tracer_fp_types.h:10-57)
@@ -127,10 +128,10 @@ int croma_hip_context_set_tile(CromaHipContext *ctx, int protons, const CromaHip
 int croma_hip_context_get_tile(const CromaHipContext *ctx, int protons, CromaHipTile *tile);
 /* 1 (default): a snapshot window whose launch shape repeats the previous
  * window of its buffer set replays a captured hipGraph of its launches
- * (fill, fused kernel, GEMM, fused kernel); 0: plain stream launches. No
+ * (fill, fused kernel, contraction, fused kernel); 0: plain stream launches. No
  * reference counterpart (the reference launches per bucket and segment). */
 int croma_hip_context_set_graphs(CromaHipContext *ctx, int enable);
-/* 1: in batch runs with the split secondary GEMM, the electron pass of each
+/* 1: in batch runs with the split secondary contraction, the electron pass of each
  * snapshot window runs on a second stream beside the proton pass of the next
  * window, each on half the device; 0 (default): the passes of a window run
  * in turn on the full device.  The spectra are the same either way; which is
@@ -260,11 +261,8 @@ int croma_hip_batch_sync(CromaHipBatch *b);
  * after each trial and at the end.  Then, where snapshot s + 1 is resident
  * with its successor, the winner advances snapshots s and s + 1 with one
  * and with two snapshots per queue task, and keeps two if that is more than
- * 1% faster.  For a split tile it then times the vendor library's
- * deterministic FP64 GEMM solutions at the batch's segment-row capacity and
- * keeps the fastest for this context.  Stores the fastest tile in the
- * context and returns it.  No reference counterpart (the reference has no
- * tiles). */
+ * 1% faster.  Stores the fastest tile in the context and returns it.  No
+ * reference counterpart (the reference has no tiles). */
 int croma_hip_batch_autotune(CromaHipBatch *b, int isnap, int nsnap_total, const double *z,
                              const double *dt_snap, double elapsed_gyr, CromaHipTile *best);
 
