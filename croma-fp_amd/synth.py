@@ -19,9 +19,11 @@ snapshot in n, T, B; fresh div v draws) so that the coefficient
 interpolation between snapshots is exercised.  Redshift runs linearly from
 z0 to 0; snapshot durations are flat LCDM age differences (H0 = 70,
 Omega_m = 0.3).  The initial spectrum is N(p) = A p^-2.2 exp(-p / p_cut).
-DSA-like sources (configs with dsa_frac > 0) add Q(p) = A p^-q exp(-p/p_max)
-on a random fraction of (tracer, snapshot) pairs, q = 2(M^2+1)/(M^2-1) with
-Mach ~ logU(2, 10).
+Configs with dsa_frac > 0 add shock-injected sources on a random fraction
+of (tracer, snapshot) pairs: Mach ~ logU(2, 10), the pre-shock temperature
+and density the tracer's background at that snapshot, through the library's
+DSA builder (eta(M)-normalised proton and electron spectra per unit volume,
+times the tracer's downstream volume, per Gyr; tracer_fp_dsa.c:184-505).
 """
 from __future__ import annotations
 
@@ -34,6 +36,7 @@ KB_EV = 8.62e-5
 EV_ERG = 1.6e-12
 C_LIGHT = 2.999e10
 MU_MOL = 0.59
+M_PROTON_G = 1.6726e-24
 GYR_S = 3.1536e16
 MPC_CM = 3.0857e24
 FIELDS = ("n_gas", "kbt", "b_field", "divv", "l_turb", "dv", "cs", "beta_pl")
@@ -138,17 +141,25 @@ def make_workload(n=128, ntracer=1024, nsnap=100, species=1, dpp_model=3, seed=1
         crp0 = np.zeros(nt * n)
     w = Workload(n, nt, nsnap, species, dpp_model, z, dt, bg, crp0, cre0, np.full(nt, 1.0e7))
     if dsa_frac > 0.0:
+        # shock injection through the library's DSA builder (TRACER_SOURCE
+        # mode, tracer_fp_dsa.c:456-505): the hit tracers' pre-shock state is
+        # the snapshot's background, rho = mu m_p n_gas
+        import croma_hip as ch
+        dprm = ch.dsa_default_params()
         for s in range(nsnap):
             hit = rng.random(nt) < dsa_frac
             if not hit.any():
                 continue
-            mach = 10.0 ** rng.uniform(math.log10(2.0), 1.0, nt)
-            q = 2.0 * (mach ** 2 + 1.0) / (mach ** 2 - 1.0)
-            qe = np.where(hit[:, None], pe[None, :] ** -q[:, None] * np.exp(-pe[None, :] / 1.0e5), 0.0)
-            w.qepri[s] = np.ascontiguousarray(qe.ravel())
+            mach = np.where(hit, 10.0 ** rng.uniform(math.log10(2.0), 1.0, nt), np.nan)
+            temp1 = bg[s, 1] / (KB_EV * 1.0e-9)
+            rho1 = bg[s, 0] * MU_MOL * M_PROTON_G
+            qe = np.zeros(nt * n)
+            qp = np.zeros(nt * n)
+            ch.dsa_inject(n, dprm, ch.DSA_SOURCE, mach, rho1, w.mass, temp1=temp1, dt_gyr=float(dt[s]), qpi=qp,
+                          qepri=qe, pmin=-1.0, pmax=8.0, pemin=0.0, pemax=6.0)
+            w.qepri[s] = qe
             if species in (2, 3):
-                qp = np.where(hit[:, None], 100.0 * pp[None, :] ** -q[:, None] * np.exp(-pp[None, :] / 1.0e6), 0.0)
-                w.qpi[s] = np.ascontiguousarray(qp.ravel())
+                w.qpi[s] = qp
     return w
 
 
