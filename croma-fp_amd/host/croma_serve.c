@@ -6,7 +6,7 @@
  * scheduler thread admits them against a fixed HBM pool (first fit, in
  * arrival order), uploads their spectra, backgrounds of every snapshot and
  * sources once (they stay resident until the chunk finishes), builds their
- * IC cooling rows on the device and their nsub schedule on the host, and
+ * IC cooling rows and their nsub schedule on the device, and
  * then advances every admitted chunk snapshot by snapshot.  Chunks of one
  * launch class (grid size x species set) share one persistent fused launch
  * per step: the task list of a launch is the cost-ordered union of the
@@ -231,6 +231,16 @@ static CromaHipContext *sv_ctx(CromaHipServer *s, int nidx)
     return s->ctx[nidx];
 }
 
+/* Tracer-substeps of a request, from its nsub schedule once that is on the host. */
+static void sv_count_steps(SvReq *r)
+{
+    const size_t cnt = (size_t)r->nsnap * r->ntr;
+    long long t = 0;
+    size_t i;
+    for (i = 0; i < cnt; i++) t += r->h_sched[i];
+    r->cell_steps = t;
+}
+
 /* Host preparation and uploads of an admitted request (scheduler thread). */
 static int sv_prepare(CromaHipServer *s, SvReq *r)
 {
@@ -264,18 +274,9 @@ static int sv_prepare(CromaHipServer *s, SvReq *r)
         ch_set_error("serve: chunk layout exceeds its budget charge");
         return CROMA_HIP_EINVAL;
     }
-    /* nsub schedule of every snapshot (host, exact) and CMB bounds */
+    /* CMB bounds of every redshift (the nsub schedule is built on the
+     * device below; cell_steps is counted once it is on the host) */
     r->cell_steps = 0;
-    for (q = 0; q < r->nsnap; q++) {
-        const double *hb = r->bg + (size_t)q * CK_NBG * r->ntr;
-        int *ns = r->h_sched + (size_t)q * r->ntr, *non = r->h_sched + ((size_t)r->nsnap + q) * r->ntr, i;
-        double ends[2];
-        ch_ctx_ic_ends(c, r->z[q], ends);
-        ch_nsub_estimate(g, &r->prm, r->ntr, r->dt[q], ends[0], ends[1], hb, hb + (size_t)r->ntr,
-                         hb + 2 * (size_t)r->ntr, hb + 3 * (size_t)r->ntr, hb + 4 * (size_t)r->ntr,
-                         hb + 5 * (size_t)r->ntr, ns, non);
-        for (i = 0; i < r->ntr; i++) r->cell_steps += ns[i];
-    }
     for (q = 0; q <= r->nsnap; q++) {
         r->h_zb[3 * q] = r->z[q];
         ch_ic_bounds(r->z[q], &r->h_zb[3 * q + 1], &r->h_zb[3 * q + 2]);
@@ -294,8 +295,21 @@ static int sv_prepare(CromaHipServer *s, SvReq *r)
         if (r->has_qepri) rc |= ck_h2d(r->d_qepri, r->h_qepri, m * sizeof(double), cs);
         if (r->has_mass) rc |= ck_h2d(r->d_mass, r->h_mass, (size_t)r->ntr * sizeof(double), cs);
         rc |= ck_h2d(r->d_bg, r->bg, (size_t)(r->nsnap + 1) * CK_NBG * r->ntr * sizeof(double), cs);
-        rc |= ck_h2d(r->d_nsub, r->h_sched, (size_t)r->nsnap * r->ntr * sizeof(int), cs);
-        rc |= ck_h2d(r->d_non, r->h_sched + (size_t)r->nsnap * r->ntr, (size_t)r->nsnap * r->ntr * sizeof(int), cs);
+        /* nsub schedule of every snapshot on the device (d_nsub_one: the
+         * host estimate's integers, tracer_fp_nsub.c:94-282, with the IC
+         * rate at the end bins from the host's exact rows), then to the
+         * pinned host copy the task ordering reads */
+        for (q = 0; q < r->nsnap && !rc; q++) {
+            const double *db = r->d_bg + (size_t)q * CK_NBG * r->ntr;
+            double ends[2];
+            ch_ctx_ic_ends(c, r->z[q], ends);
+            rc |= ck_nsub_arrays(*ch_ctx_dgrid(c), r->prm, r->ntr, r->dt[q], ends[0], ends[1], db,
+                                 db + (size_t)r->ntr, db + 2 * (size_t)r->ntr, db + 3 * (size_t)r->ntr,
+                                 db + 4 * (size_t)r->ntr, db + 5 * (size_t)r->ntr, r->d_nsub + (size_t)q * r->ntr,
+                                 r->d_non + (size_t)q * r->ntr, cs);
+        }
+        rc |= ck_d2h(r->h_sched, r->d_nsub, (size_t)r->nsnap * r->ntr * sizeof(int), cs);
+        rc |= ck_d2h(r->h_sched + (size_t)r->nsnap * r->ntr, r->d_non, (size_t)r->nsnap * r->ntr * sizeof(int), cs);
         rc |= ck_h2d(r->d_zb, r->h_zb, (size_t)(r->nsnap + 1) * 3 * sizeof(double), cs);
         rc |= ck_ic_rows(r->n, ch_ctx_dgrid(c)->ep, g->epm1, g->epp1, r->nsnap + 1, r->d_zb, r->d_ric, 0,
                          s->d_admit_err, cs);
@@ -448,6 +462,7 @@ static int sv_launch_class(CromaHipServer *s, int ci)
             if (e == 0) continue;
             if (e < 0) return CROMA_HIP_EDEVICE;
             r->ready = 1;
+            sv_count_steps(r);
         }
         r->inflight = r->nsnap - r->snap_next < tile.chain ? r->nsnap - r->snap_next : tile.chain;
         nslots += r->inflight;

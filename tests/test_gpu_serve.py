@@ -39,7 +39,12 @@ def test_alone_equals_mix_and_oracle(server):
         print(f"[serve] alone {spec}", flush=True)
         _, kw = serve_stream.make_chunk(spec)
         rid = server.submit(**kw)
-        alone.append(server.wait(rid)[:2])
+        crp, cre, st = server.wait(rid)
+        alone.append((crp, cre))
+        # the served chunk's nsub schedule is built on the device: its
+        # tracer-substeps equal the batch path's for the same chunk
+        seq = serve_stream.run_sequential([spec])
+        assert st["cell_steps"] == seq["tracer_substeps"] > 0, (st["cell_steps"], seq["tracer_substeps"])
     print("[serve] mix", flush=True)
     ids = [server.submit(**serve_stream.make_chunk(spec)[1]) for spec in MIX]
     mixed = [server.wait(i)[:2] for i in ids]
