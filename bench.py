@@ -578,7 +578,7 @@ def main():
     split = bool(w.protons and tile[3])
     roof = {"bound": "fp64_valu", "achieved": tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": tflops / FP64_PEAK_TFLOPS, "traffic": traffic,
-            "kernel": ("k_fused proton pass + secondary GEMM + k_fused electron pass" if split else "k_fused")
+            "kernel": ("k_fused proton pass + k_sec_contract + k_fused electron pass" if split else "k_fused")
                       + f" (one launch window of {tile[2]} snapshot(s))",
             "avg_launch_ms": kern_ms, "alg_flops_per_launch": flops,
             "hbm": {"alg_bytes_per_launch": alg_bytes, "measured_bytes_per_launch": traffic,

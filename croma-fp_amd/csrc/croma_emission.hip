@@ -967,20 +967,26 @@ __global__ __launch_bounds__(256) void k_synch_prep(int ntr, int n, int nlogb, d
     const int it = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (it >= ntr) return;
     const double *cl = cre + (size_t)it * n;
+    // one pass over the row: the end bin and the exponents of the bins
+    // before it (bins 0 and 1 always come before it)
+    int e = EM_SY_NONE;
+    if (lane < 2) {
+        const double v = fabs(cl[lane]);
+        if (v > 0.0 && isfinite(v)) e = __builtin_amdgcn_frexp_exp(v);
+    }
     int jend = n - 1;
     for (int base = 2; base < n - 1; base += 64) {
         const int je = base + lane;
-        const bool stop = je < n - 1 && !(cl[je + 1] > 0.0);
+        const bool in = je < n - 1;
+        const double v = in ? fabs(cl[je]) : 0.0;
+        const bool stop = in && !(cl[je + 1] > 0.0);
         const unsigned long long m = __ballot(stop);
+        const int lim = m ? base + __ffsll((long long)m) - 1 : n - 1;
+        if (je < lim && v > 0.0 && isfinite(v)) e = max(e, __builtin_amdgcn_frexp_exp(v));
         if (m) {
-            jend = base + __ffsll((long long)m) - 1;
+            jend = lim;
             break;
         }
-    }
-    int e = EM_SY_NONE;
-    for (int k = lane; k < jend; k += 64) {
-        const double v = fabs(cl[k]);
-        if (v > 0.0 && isfinite(v)) e = max(e, __builtin_amdgcn_frexp_exp(v));
     }
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) e = max(e, __shfl_xor(e, o, 64));

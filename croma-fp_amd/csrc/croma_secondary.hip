@@ -6,16 +6,19 @@
 // (the product of prepare_secondary_sources_batch, fp_shared_core.c:1508-1589,
 // over the weights of prepare_secondary_kernel_flat, :1453-1506).
 //
-// FP64 MFMA (v_mfma_f64_16x16x4_f64): a block stages 32 rows (16 at n = 512)
-// in LDS and its four waves each own a quarter of the n output columns as
-// 16-column tiles, every weight fragment serving both 16-row tiles;
-// the weights (n x n doubles, L2-resident) stream in as the B operand.  A
-// column tile starts its k loop at the first proton bin with a nonzero
-// weight in any of its 16 columns (np_min), so the zero triangle of the
-// weights costs nothing.  The row count can come from device memory (the
-// device-side segment count of the window), so the launch shape depends only
-// on the capacity and a captured graph replays it unchanged; blocks loop over
-// row tiles and stop at the count.
+// FP64 MFMA (v_mfma_f64_16x16x4_f64), no LDS: a block takes 32 rows and its
+// four waves each own a quarter of the n output columns as 16-column tiles,
+// every weight fragment serving both 16-row tiles and every row fragment all
+// of the wave's column tiles.  The rows come straight from memory into the A
+// operand (the four waves of a block read the same rows, the later ones from
+// the cache); the weights (n x n doubles, L2-resident) are the B operand.
+// The wave's k loop starts at the first proton bin with a nonzero weight in
+// any of its columns (np_min), so the zero triangle of the weights costs
+// nothing; the weights below a column's own start are zero and add exact
+// zeros, so no branch separates the products.  The row count can come from
+// device memory (the device-side segment count of the window), so the
+// launch shape depends only on the capacity and a captured graph replays it
+// unchanged; blocks loop over row tiles and stop at the count.
 #include <hip/hip_runtime.h>
 
 #include "croma_kernels.h"
@@ -29,80 +32,61 @@ __global__ __launch_bounds__(256) void k_sec_contract(const double *__restrict__
                                                       const double *__restrict__ rec, double *__restrict__ inje,
                                                       long rows_cap, const int *__restrict__ d_rows)
 {
-    constexpr int RT = N >= 512 ? 16 : 32;  // rows per block tile (RT / 16 MFMA row tiles per wave)
-    constexpr int MT = RT / 16;
-    constexpr int LD = N + 4;               // padded row stride: 16 rows x 4 k of a fragment hit distinct banks
-    constexpr int NT = N / 16;              // 16-column tiles
-    constexpr int TW = NT / 4;              // column tiles per wave
-    constexpr int PER = RT * (N / 2) / 256; // row pairs each thread stages per tile
-    constexpr int KG = 4;                   // k steps per group of weight loads
-    static_assert(NT % 4 == 0 && (RT * (N / 2)) % 256 == 0 && N % (4 * KG) == 0, "tile shape");
-    __shared__ double sA[RT * LD];
+    constexpr int MT = 2;        // 16-row MFMA tiles per block (32 rows)
+    constexpr int NT = N / 16;   // 16-column tiles
+    constexpr int TW = NT / 4;   // column tiles per wave
+    static_assert(NT % 4 == 0 && N % 16 == 0, "tile shape");
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int i16 = lane & 15, q = lane >> 4;
     const long rows = d_rows ? min((long)*d_rows, rows_cap) : rows_cap;
     // the wave's k loop starts at the first nonzero weight of any of its
-    // columns, rounded down to a multiple of 4 KG
+    // columns, rounded down to a multiple of 16
     int kmin = N;
 #pragma unroll
     for (int i = 0; i < TW; ++i) {
         int v = np_min[(w * TW + i) * 16 + i16];
 #pragma unroll
         for (int o = 1; o < 16; o <<= 1) v = min(v, __shfl_xor(v, o, 64));
-        // wave-uniform (every 16-lane block reduced the same columns): a
-        // scalar loop bound
-        kmin = min(kmin, __builtin_amdgcn_readfirstlane(max(v, 0) & ~(4 * KG - 1)));
+        // wave-uniform (every 16-lane block reduced the same columns)
+        kmin = min(kmin, __builtin_amdgcn_readfirstlane(max(v, 0) & ~15));
     }
-    const double *bcol = fqe_t + (size_t)q * N + w * TW * 16 + i16;
-    for (long r0 = (long)blockIdx.x * RT; r0 < rows; r0 += (long)gridDim.x * RT) {
-        // stage the tile's rows: every load in flight before the first store
-        // (the other block on the CU runs its products meanwhile)
-        double2 st[PER];
+    // Within a group of 16 k the k order is permuted alike for both
+    // operands: product u of the group takes k = k0 + 4 q + u in lane block
+    // q, so a lane's four A values are one contiguous 32-byte load and its B
+    // values four rows of the weights.
+    const double *bcol = fqe_t + (size_t)(4 * q) * N + w * TW * 16 + i16;
+    for (long r0 = (long)blockIdx.x * (16 * MT); r0 < rows; r0 += (long)gridDim.x * (16 * MT)) {
+        const double *arow[MT];
 #pragma unroll
-        for (int u = 0; u < PER; ++u) {
-            const int e = threadIdx.x + 256 * u;
-            const int rr = e / (N / 2), c = 2 * (e % (N / 2));
-            st[u] = (r0 + rr < rows) ? *reinterpret_cast<const double2 *>(rec + (size_t)(r0 + rr) * N + c)
-                                     : make_double2(0.0, 0.0);
+        for (int m = 0; m < MT; ++m) {
+            // rows past the capacity read the last row (in bounds; their
+            // products are not stored)
+            const long r = min(r0 + 16 * m + i16, rows_cap - 1);
+            arow[m] = rec + (size_t)r * N + 4 * q;
         }
-        __syncthreads();  // the previous tile's reads of sA are done
-#pragma unroll
-        for (int u = 0; u < PER; ++u) {
-            const int e = threadIdx.x + 256 * u;
-            const int rr = e / (N / 2), c = 2 * (e % (N / 2));
-            sA[rr * LD + c] = st[u].x;
-            sA[rr * LD + c + 1] = st[u].y;
-        }
-        __syncthreads();
-        // The wave's TW column tiles advance together from the smallest of
-        // their start steps (a tile's weights below its own start are zero:
-        // those products add exact zeros, so no branch separates the
-        // products and the accumulators stay fixed registers).  A fragment:
-        // lane holds row i16 of a 16-row tile, k = k0 + q; B fragment: k =
-        // k0 + q, column i16 of the column tile.  Each group of KG k steps
-        // issues its TW x KG weight loads together.
         sec_d4 acc[MT][TW];
 #pragma unroll
         for (int m = 0; m < MT; ++m)
 #pragma unroll
             for (int i = 0; i < TW; ++i) acc[m][i] = sec_d4{0.0, 0.0, 0.0, 0.0};
-        for (int k0 = kmin; k0 < N; k0 += 4 * KG) {
-            double bb[KG][TW];
+        for (int k0 = kmin; k0 < N; k0 += 16) {
+            double4 a[MT];
 #pragma unroll
-            for (int u = 0; u < KG; ++u)
+            for (int m = 0; m < MT; ++m) a[m] = *reinterpret_cast<const double4 *>(arow[m] + k0);
+            double bb[4][TW];
 #pragma unroll
-                for (int i = 0; i < TW; ++i) bb[u][i] = bcol[(size_t)(k0 + 4 * u) * N + 16 * i];
+            for (int u = 0; u < 4; ++u)
 #pragma unroll
-            for (int u = 0; u < KG; ++u) {
-                double a[MT];
+                for (int i = 0; i < TW; ++i) bb[u][i] = bcol[(size_t)(k0 + u) * N + 16 * i];
 #pragma unroll
-                for (int m = 0; m < MT; ++m) a[m] = sA[(16 * m + i16) * LD + k0 + 4 * u + q];
+            for (int u = 0; u < 4; ++u)
 #pragma unroll
                 for (int i = 0; i < TW; ++i)
 #pragma unroll
-                    for (int m = 0; m < MT; ++m)
-                        acc[m][i] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[m], bb[u][i], acc[m][i], 0, 0, 0);
-            }
+                    for (int m = 0; m < MT; ++m) {
+                        const double av = u == 0 ? a[m].x : u == 1 ? a[m].y : u == 2 ? a[m].z : a[m].w;
+                        acc[m][i] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bb[u][i], acc[m][i], 0, 0, 0);
+                    }
         }
         // D layout of the FP64 16x16 MFMA: register v of a lane holds row
         // 4 v + q of the tile, column i16 (one row per 16-lane block per
