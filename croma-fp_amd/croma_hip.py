@@ -558,6 +558,8 @@ class Server:
 OUT_MAP, OUT_TILE = 0, 1
 
 DSA_STATE, DSA_SOURCE = 1, 2
+DSA_REACC_OFF, DSA_REACC_POSITIVE_DELTA, DSA_REACC_DRURY83 = 0, 1, 2
+DSA_REACC_KIND_POSITIVE, DSA_REACC_KIND_DRURY, DSA_REACC_KIND_ADIABATIC, DSA_REACC_KIND_ENERGY = 0, 1, 2, 3
 DSA_KR07, DSA_KR07R, DSA_KR13, DSA_KR13R, DSA_CS14, DSA_CS14R, DSA_RYU19, DSA_RYU19R = range(8)
 
 
@@ -565,7 +567,8 @@ class DsaParams(C.Structure):
     _fields_ = [("eta_model", C.c_int), ("inject_species", C.c_int), ("chi_p", C.c_double), ("chi_e", C.c_double),
                 ("k_ep", C.c_double), ("pmax_p", C.c_double), ("pmax_e", C.c_double), ("gamma_gas", C.c_double),
                 ("min_mach", C.c_double), ("mu_mol", C.c_double), ("pmin_dsa_p_log10", C.c_double),
-                ("pmin_dsa_e_log10", C.c_double)]
+                ("pmin_dsa_e_log10", C.c_double), ("reacc_mode", C.c_int), ("eta_model_reacc", C.c_int),
+                ("reacc_eta_cap", C.c_double), ("xcr_pmin_pmc", C.c_double)]
 
 
 def dsa_default_params(**kw):
@@ -595,20 +598,42 @@ def dsa_shock_sources(n, params, mach, temp1, rho1, u1=0.0, pmin=-1.0, pmax=8.0,
 
 
 def dsa_inject(n, params, mode, mach, rho1, mass, temp1=None, u1=None, flux=None, side=None, dt_gyr=0.0,
-               crp=None, cre=None, qpi=None, qepri=None, pmin=-1.0, pmax=8.0, pemin=0.0, pemax=6.0):
-    """Adds one snapshot's DSA injection in place (state or source arrays); returns the injected count."""
+               crp=None, cre=None, qpi=None, qepri=None, pmin=-1.0, pmax=8.0, pemin=0.0, pemax=6.0, dis=None):
+    """Adds one snapshot's DSA injection (and, with params.reacc_mode, the
+    reacceleration of crp / cre) in place; returns the injected count.  dis:
+    uint8 [ntracer] receiving the Drury83 tracers' disable_adiabatic flags."""
     gs = GridSpec(n, pmin, pmax, pemin, pemax)
     nt = len(mach)
     ninj = C.c_int()
     opt = lambda a: _p(f64(a)) if a is not None else None
     for a in (crp, cre, qpi, qepri):
         assert a is None or (a.dtype == np.float64 and a.flags.c_contiguous and a.size == nt * n)
-    _chk(lib().croma_hip_dsa_inject(C.byref(gs), C.byref(params), int(mode), nt, C.c_double(dt_gyr), _p(f64(mach)),
-                                    opt(temp1), _p(f64(rho1)), opt(u1), opt(flux), opt(side), _p(f64(mass)),
-                                    _p(crp) if crp is not None else None, _p(cre) if cre is not None else None,
-                                    _p(qpi) if qpi is not None else None, _p(qepri) if qepri is not None else None,
-                                    C.byref(ninj)), "dsa_inject")
+    assert dis is None or (dis.dtype == np.uint8 and dis.size == nt)
+    _chk(lib().croma_hip_dsa_inject_reacc(C.byref(gs), C.byref(params), int(mode), nt, C.c_double(dt_gyr),
+                                          _p(f64(mach)), opt(temp1), _p(f64(rho1)), opt(u1), opt(flux), opt(side),
+                                          _p(f64(mass)), _p(crp) if crp is not None else None,
+                                          _p(cre) if cre is not None else None, _p(qpi) if qpi is not None else None,
+                                          _p(qepri) if qepri is not None else None, _p(dis) if dis is not None else None,
+                                          C.byref(ninj)), "dsa_inject")
     return ninj.value
+
+
+def dsa_reacc_delta(n, kind, species, state, v1, v2, q, r, pmin_mc, pmin=-1.0, pmax=8.0, pemin=0.0, pemax=6.0):
+    """One reacceleration row of a spectrum (croma_hip_dsa_reacc_delta): (delta, energy density)."""
+    gs = GridSpec(n, pmin, pmax, pemin, pemax)
+    delta = np.zeros(n)
+    ed = C.c_double()
+    _chk(lib().croma_hip_dsa_reacc_delta(C.byref(gs), int(kind), int(species), _p(f64(state)), C.c_double(v1),
+                                         C.c_double(v2), C.c_double(q), C.c_double(r), C.c_double(pmin_mc),
+                                         _p(delta), C.byref(ed)), "dsa_reacc_delta")
+    return delta, ed.value
+
+
+def dsa_eta_interpolated(mach, initial_model, reacc_model, x_cr):
+    L = lib()
+    L.croma_hip_dsa_eta_interpolated.restype = C.c_double
+    L.croma_hip_dsa_eta_interpolated.argtypes = [C.c_double, C.c_int, C.c_int, C.c_double]
+    return L.croma_hip_dsa_eta_interpolated(float(mach), int(initial_model), int(reacc_model), float(x_cr))
 
 
 class CheckpointMeta(C.Structure):

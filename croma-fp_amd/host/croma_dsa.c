@@ -1,9 +1,11 @@
 /*
  * croma_dsa.c -- DSA source builder: shock-accelerated CR injection per
  * tracer and snapshot (tracer_fp_apply_dsa_tracer_injection_snapshot,
- * tracer_fp_dsa.c:184-505, reacceleration off), with the injection models of
- * DSA_MODELS.c.  Host C; the rows it produces are the qpi / qepri (source
- * mode) or state increments (state mode) the solve consumes.
+ * tracer_fp_dsa.c:184-505), with the injection models of DSA_MODELS.c and
+ * the reacceleration of the pre-shock spectrum (positive delta, Drury 1983
+ * convolution; tracer_fp_dsa_reacc.c).  Host C; the rows it produces are the
+ * qpi / qepri (source mode) or state increments (state mode) the solve
+ * consumes.
  *
  * The arithmetic follows the reference expression by expression (FP64, no
  * contraction), so the rows equal the reference's bit for bit.
@@ -23,13 +25,22 @@
 #define DS_4PI 12.566370614359172
 #define DS_MSUN 1.99e+33
 #define DS_NSIMPSON 4000
+/* the reacceleration's kinetic energies use the solver's c (CONSTANTS.c:3)
+ * and the driver's own masses and k_B (tracer_fp_dsa.c:22-24) */
+#define RA_C 2.999e+10
+#define RA_MP 1.67262192369e-24
+#define RA_ME 9.1093837015e-28
+#define RA_KB 1.380649e-16
 
 enum { FAM_KR07, FAM_KR13, FAM_RYU19 };
 
-/* efficiency fits eta(M) = sum_k b_k (M - 1)^k / M^4 with their branches */
+/* efficiency fits eta(M) = sum_k b_k (M - 1)^k / M^4 with their branches;
+ * x_cr0: the CR pressure fraction at which a reacceleration model takes
+ * over from the initial one */
 typedef struct {
     int family;
     int reacc;
+    double x_cr0;
     double b[5];
     double mach_crit;
     double scale;
@@ -37,14 +48,14 @@ typedef struct {
 
 /* the reference's model table (DSA_MODELS.c:287-354), DSAModelID order */
 static const DsModel ds_models[8] = {
-    {FAM_KR07, 0, {5.46, -9.78, 4.17, -0.33, 0.57}, 1.0, 1.0},
-    {FAM_KR07, 1, {0.24, -1.56, 2.8, 0.512, 0.557}, 1.0, 1.0},
-    {FAM_KR13, 0, {-2.87, 9.67, -8.88, 1.94, 0.18}, 2.0, 1.0},
-    {FAM_KR13, 1, {-0.72, 2.73, -3.29, 1.34, 0.19}, 2.0, 1.0},
-    {FAM_KR13, 0, {-2.87, 9.67, -8.88, 1.94, 0.18}, 2.0, 0.5},
-    {FAM_KR13, 1, {-0.72, 2.73, -3.29, 1.34, 0.19}, 2.0, 0.5},
-    {FAM_RYU19, 0, {-1.5255, 2.4026, -1.2534, 0.2215, 0.0336}, 2.25, 1.0},
-    {FAM_RYU19, 1, {0.3965, -0.21898, -0.2074, 0.1319, 0.0351}, 2.25, 1.0},
+    {FAM_KR07, 0, 0.0, {5.46, -9.78, 4.17, -0.33, 0.57}, 1.0, 1.0},
+    {FAM_KR07, 1, 0.3, {0.24, -1.56, 2.8, 0.512, 0.557}, 1.0, 1.0},
+    {FAM_KR13, 0, 0.0, {-2.87, 9.67, -8.88, 1.94, 0.18}, 2.0, 1.0},
+    {FAM_KR13, 1, 0.05, {-0.72, 2.73, -3.29, 1.34, 0.19}, 2.0, 1.0},
+    {FAM_KR13, 0, 0.0, {-2.87, 9.67, -8.88, 1.94, 0.18}, 2.0, 0.5},
+    {FAM_KR13, 1, 0.05, {-0.72, 2.73, -3.29, 1.34, 0.19}, 2.0, 0.5},
+    {FAM_RYU19, 0, 0.0, {-1.5255, 2.4026, -1.2534, 0.2215, 0.0336}, 2.25, 1.0},
+    {FAM_RYU19, 1, 0.05, {0.3965, -0.21898, -0.2074, 0.1319, 0.0351}, 2.25, 1.0},
 };
 
 void croma_hip_dsa_default_params(CromaHipDsaParams *p)
@@ -63,6 +74,10 @@ void croma_hip_dsa_default_params(CromaHipDsaParams *p)
     p->mu_mol = 0.59;
     p->pmin_dsa_p_log10 = -3.0;
     p->pmin_dsa_e_log10 = -3.0;
+    p->reacc_mode = CROMA_HIP_DSA_REACC_OFF;
+    p->eta_model_reacc = CROMA_HIP_DSA_RYU19R;
+    p->reacc_eta_cap = 0.0;
+    p->xcr_pmin_pmc = 1.0;
 }
 
 static double ds_poly(double mach, const double b[5])
@@ -106,6 +121,157 @@ double croma_hip_dsa_eta(double mach, int id)
     }
     eta *= m->scale;
     return eta < 0.0 ? 0.0 : eta;
+}
+
+/* ---- reacceleration of the pre-shock spectrum (tracer_fp_dsa_reacc.c,
+ * driven as tracer_fp_dsa.c:322-418 does) ---------------------------- */
+
+/* kinetic energy [erg] of momentum p (m c) */
+static double ra_ekin(double p, int electron)
+{
+    const double m = electron ? RA_ME : RA_MP;
+    return m * RA_C * RA_C * (sqrt(1.0 + p * p) - 1.0);
+}
+
+/* eta weighted between the initial model and the reacceleration model by
+ * x_cr / x_cr0 of the latter, clipped to [0, 1] (DSA_MODELS.c:474-522) */
+double croma_hip_dsa_eta_interpolated(double mach, int initial_id, int reacc_id, double x_cr)
+{
+    double e0, e1, x0, w;
+    if (initial_id < 0 || initial_id >= 8 || reacc_id < 0 || reacc_id >= 8) return 0.0;
+    e0 = croma_hip_dsa_eta(mach, initial_id);
+    e1 = croma_hip_dsa_eta(mach, reacc_id);
+    x0 = ds_models[reacc_id].x_cr0;
+    if (x0 <= 0.0) return e0;
+    w = x_cr / x0;
+    if (w < 0.0) w = 0.0;
+    if (w > 1.0) w = 1.0;
+    return (1.0 - w) * e0 + w * e1;
+}
+
+/* kinetic energy density of a state row above pmin in a volume
+ * (tracer_fp_dsa.c:120-139) */
+static double ra_state_edens(const double *p, const double *dp, const double *state, int n, double vol, double pmin,
+                             int electron)
+{
+    double e = 0.0;
+    int j;
+    if (!(vol > 0.0)) return 0.0;
+    for (j = 0; j < n; j++) {
+        const double ek = ra_ekin(p[j], electron);
+        if (p[j] < pmin) continue;
+        e += ek * state[j] * dp[j];
+    }
+    return e / vol;
+}
+
+/* the energy density of a delta row over bins 2..n-3 above pmin
+ * (tracer_dsa_reacc_delta_energy_density_1d) */
+static double ra_delta_edens(const double *p, const double *dp, const double *delta, int n, double pmin, int electron)
+{
+    double e = 0.0;
+    int j;
+    if (n <= 4) return 0.0;
+    for (j = 2; j <= n - 3; j++) {
+        if (!(p[j] > 0.0) || p[j] < pmin || !isfinite(delta[j])) continue;
+        e += ra_ekin(p[j], electron) * delta[j] * dp[j];
+    }
+    return e;
+}
+
+/* positive delta: the pre-shock spectrum f seeds the test-particle solution
+ * 4 pi p^2 q p^-q int p'^(q-1) f dp' from the second bin on; where that
+ * exceeds the current density the excess is the delta (tracer_dsa_reacc_
+ * positive_delta_1d).  Returns the delta's energy density. */
+static double ra_positive_delta(const double *p, const double *dp, const double *state, int n, double vd, double q,
+                                double pmin, double *delta, int electron)
+{
+    double acc = 0.0, e = 0.0;
+    int j;
+    for (j = 0; j < n; j++) delta[j] = 0.0;
+    if (n <= 0 || !(vd > 0.0) || !(q > 0.0)) return 0.0;
+    for (j = (n > 1); j < n; j++) {
+        const double pj = p[j];
+        double cur, f, re;
+        if (!(pj > 0.0) || pj < pmin) continue;
+        cur = state[j] / vd;
+        f = cur / (4.0 * M_PI * pj * pj);
+        acc += pow(pj, q - 1.0) * f * dp[j];
+        re = 4.0 * M_PI * pj * pj * q * pow(pj, -q) * acc;
+        if (re > cur) {
+            delta[j] = re - cur;
+            e += ra_ekin(pj, electron) * delta[j] * dp[j];
+        }
+    }
+    return e;
+}
+
+/* Drury (1983): the upstream spectrum convolved to (a + 2) p^-a int p'^(a-1)
+ * N_up dp', a = q - 2, minus the current downstream spectrum, the delta not
+ * below minus the current value (tracer_dsa_reacc_convolution_delta_1d).
+ * Returns the delta's energy density over every bin above pmin. */
+static double ra_drury(const double *p, const double *dp, const double *state, int n, double vu, double vd, double q,
+                       double pmin, double *delta, int electron)
+{
+    const double a = q - 2.0;
+    double acc = 0.0, e = 0.0;
+    int j;
+    for (j = 0; j < n; j++) delta[j] = 0.0;
+    if (n <= 0 || !(vu > 0.0) || !(vd > 0.0) || !(q > 0.0)) return 0.0;
+    for (j = (n > 1); j < n; j++) {
+        const double pj = p[j];
+        double seed, cur, d;
+        if (!(pj > 0.0)) continue;
+        seed = state[j] / vu;
+        cur = state[j] / vd;
+        if (pj < pmin) continue;
+        acc += pow(pj, a - 1.0) * seed * dp[j];
+        d = (a + 2.0) * pow(pj, -a) * acc - cur;
+        if (!isfinite(d)) continue;
+        if (d < -cur) d = -cur;
+        delta[j] = d;
+        e += ra_ekin(pj, electron) * d * dp[j];
+    }
+    return e;
+}
+
+/* value at p_mc of a row, linear between the bins that bracket it inside
+ * bins 1..n-2, 0 outside (tracer_fp_dsa_reacc.c:24-50) */
+static double ra_interp(const double *p, const double *row, int n, double x)
+{
+    int lo = 1, hi = n - 2;
+    if (n <= 2 || !(x > 0.0) || x < p[lo] || x > p[hi]) return 0.0;
+    while (hi - lo > 1) {
+        const int mid = lo + (hi - lo) / 2;
+        if (p[mid] <= x) lo = mid;
+        else hi = mid;
+    }
+    if (p[hi] <= p[lo]) return row[lo];
+    return row[lo] + (row[hi] - row[lo]) * (x - p[lo]) / (p[hi] - p[lo]);
+}
+
+/* pure adiabatic compression by r: p -> r^(1/3) p, density x r^(2/3), minus
+ * the current downstream spectrum (tracer_dsa_reacc_adiabatic_delta_1d);
+ * sd: n doubles of work.  Returns the delta's energy density. */
+static double ra_adiabatic(const double *p, const double *dp, const double *state, int n, double vu, double vd,
+                           double r, double pmin, double *delta, double *sd, int electron)
+{
+    double ps, norm;
+    int j;
+    for (j = 0; j < n; j++) delta[j] = 0.0;
+    if (n <= 0 || !(vu > 0.0) || !(vd > 0.0) || !(r > 0.0) || !isfinite(r)) return 0.0;
+    ps = cbrt(r);
+    norm = ps * ps;
+    for (j = 0; j < n; j++) sd[j] = state[j] / vu;
+    for (j = 0; j < n; j++) {
+        double cur, ad;
+        if (!(p[j] > 0.0) || p[j] < pmin) continue;
+        cur = state[j] / vd;
+        ad = norm * ra_interp(p, sd, n, p[j] / ps);
+        if (!isfinite(cur) || !isfinite(ad)) continue;
+        delta[j] = ad - cur;
+    }
+    return ra_delta_edens(p, dp, delta, n, pmin, electron);
 }
 
 static double ds_ratio(double mach, double g)
@@ -305,6 +471,7 @@ typedef struct {
     double inv_dt;
     const double *mach, *temp1, *rho1, *u1, *flux, *side, *mass;
     double *crp, *cre, *qpi, *qepri;
+    unsigned char *dis;
     int ninj[64];
 } DsJob;
 
@@ -341,6 +508,106 @@ static int ds_resolve(const DsJob *J, long i, double *mach, double *temp, double
     return 1;
 }
 
+/* The reacceleration rows of one spectrum on a grid (the functions of
+ * include/tracer_fp_dsa_reacc.h): kind CROMA_HIP_DSA_REACC_KIND_*;
+ * species 0 protons, 1 electrons; returns the delta's energy density. */
+int croma_hip_dsa_reacc_delta(const CromaHipGridSpec *grid, int kind, int species, const double *state,
+                              double volume_upstream, double volume_downstream, double q, double compression_ratio,
+                              double pmin_mc, double *delta, double *edens)
+{
+    ChGrid *g = 0;
+    const double *p, *dp;
+    double *sd = 0;
+    int rc;
+    if (!state || !delta || !edens || species < 0 || species > 1 || kind < 0 || kind > 3) return CROMA_HIP_EINVAL;
+    if ((rc = ds_grid(grid, &g)) != 0) return rc;
+    p = species ? g->ep : g->pp;
+    dp = species ? g->edp : g->pdp;
+    if (kind == CROMA_HIP_DSA_REACC_KIND_POSITIVE) {
+        *edens = ra_positive_delta(p, dp, state, g->n, volume_downstream, q, pmin_mc, delta, species);
+    } else if (kind == CROMA_HIP_DSA_REACC_KIND_DRURY) {
+        *edens = ra_drury(p, dp, state, g->n, volume_upstream, volume_downstream, q, pmin_mc, delta, species);
+    } else if (kind == CROMA_HIP_DSA_REACC_KIND_ADIABATIC) {
+        sd = (double *)malloc((size_t)g->n * sizeof(double));
+        if (!sd) {
+            free(g);
+            return CROMA_HIP_ENOMEM;
+        }
+        *edens = ra_adiabatic(p, dp, state, g->n, volume_upstream, volume_downstream, compression_ratio, pmin_mc,
+                              delta, sd, species);
+    } else {
+        /* energy density of a given delta row (delta is the input here) */
+        *edens = ra_delta_edens(p, dp, state, g->n, pmin_mc, species);
+    }
+    free(sd);
+    free(g);
+    return 0;
+}
+
+/* Reacceleration delta rows of one tracer from its pre-shock state
+ * (tracer_fp_dsa.c:322-418): dpr / der [n] and the scale the driver applies
+ * to them; *energy is the reacceleration energy [erg]. */
+static double ds_reacc(const DsJob *J, long i, double mach, double temp, double rho1, double u1, double ratio,
+                       double q, double ppm, double pem, double mass_cgs, double vu, double vd, double *dpr,
+                       double *der, double *adp, double *ade, double *sd, double *energy)
+{
+    const ChGrid *g = J->g;
+    const CromaHipDsaParams *P = J->P;
+    const int n = g->n;
+    const double *xp = J->crp + (size_t)i * n, *xe = J->cre + (size_t)i * n;
+    double scale = 0.0;
+    int j;
+    *energy = 0.0;
+    for (j = 0; j < n; j++) dpr[j] = der[j] = 0.0;
+    if (P->reacc_mode == CROMA_HIP_DSA_REACC_POSITIVE_DELTA) {
+        /* extra efficiency from the CR pressure already present */
+        double x_cr = 0.0, extra, budget;
+        {
+            const double pcr = ra_state_edens(g->pp, g->pdp, xp, n, vd, P->xcr_pmin_pmc, 0) / 3.0;
+            const double pth = rho1 * RA_KB * temp / (P->mu_mol * RA_MP);
+            if (pth > 0.0 && isfinite(pcr) && isfinite(pth) && pcr > 0.0) x_cr = pcr / pth;
+        }
+        extra = croma_hip_dsa_eta_interpolated(mach, P->eta_model, P->eta_model_reacc, x_cr) -
+                croma_hip_dsa_eta(mach, P->eta_model);
+        if (extra < 0.0 || !isfinite(extra)) extra = 0.0;
+        budget = extra * 0.5 * mass_cgs * u1 * u1;
+        if (budget > 0.0) {
+            const double ep = ra_positive_delta(g->pp, g->pdp, xp, n, vd, q, ppm, dpr, 0);
+            const double ee = ra_positive_delta(g->ep, g->edp, xe, n, vd, q, pem, der, 1);
+            *energy = (ep + ee) * vd;
+            if (*energy > 0.0 && isfinite(*energy)) {
+                scale = (*energy > budget) ? budget / *energy : 1.0;
+            } else {
+                for (j = 0; j < n; j++) dpr[j] = der[j] = 0.0;
+            }
+        }
+    } else if (P->reacc_mode == CROMA_HIP_DSA_REACC_DRURY83) {
+        double cp, ce;
+        ra_drury(g->pp, g->pdp, xp, n, vu, vd, q, ppm, dpr, 0);
+        ra_drury(g->ep, g->edp, xe, n, vu, vd, q, pem, der, 1);
+        cp = ra_delta_edens(g->pp, g->pdp, dpr, n, ppm, 0);
+        ce = ra_delta_edens(g->ep, g->edp, der, n, pem, 1);
+        *energy = (cp + ce) * vd;
+        scale = 1.0;
+        if (P->reacc_eta_cap > 0.0 && isfinite(P->reacc_eta_cap)) {
+            /* the part beyond adiabatic compression is capped at
+             * reacc_eta_cap of the upstream kinetic energy */
+            const double budget = P->reacc_eta_cap * 0.5 * mass_cgs * u1 * u1;
+            const double ap = ra_adiabatic(g->pp, g->pdp, xp, n, vu, vd, ratio, ppm, adp, sd, 0);
+            const double ae = ra_adiabatic(g->ep, g->edp, xe, n, vu, vd, ratio, pem, ade, sd, 1);
+            const double nonad = (cp + ce - ap - ae) * vd;
+            if (nonad > 0.0 && isfinite(nonad) && budget > 0.0 && isfinite(budget)) {
+                if (nonad > budget) scale = budget / nonad;
+                for (j = 0; j < n; j++) dpr[j] = adp[j] + scale * (dpr[j] - adp[j]);
+                for (j = 0; j < n; j++) der[j] = ade[j] + scale * (der[j] - ade[j]);
+                *energy = (ap + ae + scale * (cp + ce - ap - ae)) * vd;
+            }
+        }
+        if (J->dis) J->dis[i] = 1;
+    }
+    return scale;
+}
+
 static void ds_range(void *arg, long i0, long i1)
 {
     DsJob *J = (DsJob *)arg;
@@ -349,28 +616,36 @@ static void ds_range(void *arg, long i0, long i1)
     const int n = g->n;
     const int inj_p = P->inject_species != CROMA_HIP_DSA_SPECIES_ELECTRON;
     const int inj_e = P->inject_species != CROMA_HIP_DSA_SPECIES_PROTON;
-    double *sp = (double *)malloc(2 * (size_t)n * sizeof(double)), *se;
+    const int reacc = P->reacc_mode == CROMA_HIP_DSA_REACC_POSITIVE_DELTA || P->reacc_mode == CROMA_HIP_DSA_REACC_DRURY83;
+    double *sp = (double *)malloc(7 * (size_t)n * sizeof(double)), *se, *dpr, *der, *adp, *ade, *sd;
     long i, cnt = 0;
     if (!sp) return;
     se = sp + n;
+    dpr = se + n;
+    der = dpr + n;
+    adp = der + n;
+    ade = adp + n;
+    sd = ade + n;
     for (i = i0; i < i1; i++) {
-        double mach, temp, rho1, u1, ratio, mass_cgs, vd, vu;
+        double mach, temp, rho1, u1, ratio, mass_cgs, vd, vu, q, ppm, pem, rscale = 0.0, renergy = 0.0;
         int fl, j, applied = 0;
         if (!ds_resolve(J, i, &mach, &temp, &rho1, &u1)) continue;
         ratio = ds_ratio(mach, P->gamma_gas);
         if (!(ratio > 0.0) || !isfinite(ratio)) continue;
-        {
-            const double q = ds_slope(mach, P->gamma_gas);
-            const double ppm = ds_pinj(temp, P->chi_p, DS_MP);
-            const double pem = ds_pinj(temp, P->chi_e > 0.0 ? P->chi_e : P->chi_p, DS_ME);
-            if (!(q > 0.0) || !isfinite(q) || !(ppm > 0.0) || !isfinite(ppm) || !(pem > 0.0) || !isfinite(pem))
-                continue;
-        }
+        q = ds_slope(mach, P->gamma_gas);
+        ppm = ds_pinj(temp, P->chi_p, DS_MP);
+        pem = ds_pinj(temp, P->chi_e > 0.0 ? P->chi_e : P->chi_p, DS_ME);
+        if (!(q > 0.0) || !isfinite(q) || !(ppm > 0.0) || !isfinite(ppm) || !(pem > 0.0) || !isfinite(pem))
+            continue;
         fl = ds_shock(g, P, mach, temp, rho1, u1, sp, se);
         mass_cgs = J->mass[i] * DS_MSUN;
         vu = mass_cgs / rho1;
         vd = mass_cgs / (ratio * rho1);
         if (!(vu > 0.0) || !isfinite(vu) || !(vd > 0.0) || !isfinite(vd)) continue;
+        /* the reacceleration reads the state before this snapshot's
+         * injection is added */
+        if (reacc) rscale = ds_reacc(J, i, mach, temp, rho1, u1, ratio, q, ppm, pem, mass_cgs, vu, vd, dpr, der, adp,
+                                     ade, sd, &renergy);
         if (J->mode == CROMA_HIP_DSA_STATE) {
             if (inj_p && (fl & 1)) {
                 for (j = 0; j < n; j++) J->crp[i * n + j] += sp[j] * vd;
@@ -390,24 +665,42 @@ static void ds_range(void *arg, long i0, long i1)
                 applied = 1;
             }
         }
+        if (reacc) {
+            /* the Drury83 deltas carry their cap already */
+            const double as = (P->reacc_mode == CROMA_HIP_DSA_REACC_DRURY83) ? 1.0 : rscale;
+            if (J->mode == CROMA_HIP_DSA_STATE) {
+                for (j = 0; j < n; j++) J->crp[i * n + j] += dpr[j] * vd * as;
+                for (j = 0; j < n; j++) J->cre[i * n + j] += der[j] * vd * as;
+            } else {
+                for (j = 0; j < n; j++) J->qpi[i * n + j] += dpr[j] * vd * as * J->inv_dt;
+                for (j = 0; j < n; j++) J->qepri[i * n + j] += der[j] * vd * as * J->inv_dt;
+            }
+            if (rscale > 0.0 && isfinite(renergy) && fabs(renergy) > 0.0) applied = 1;
+        }
         cnt += applied;
     }
     free(sp);
     __atomic_fetch_add(&J->ninj[0], (int)cnt, __ATOMIC_RELAXED);
 }
 
-int croma_hip_dsa_inject(const CromaHipGridSpec *grid, const CromaHipDsaParams *p, int mode, int ntracer,
-                         double dt_gyr, const double *mach, const double *temp1_K, const double *rho1_cgs,
-                         const double *u1_cgs, const double *kinetic_flux, const double *side_code,
-                         const double *tracer_mass, double *crp, double *cre, double *qpi, double *qepri,
-                         int *ninjected)
+int croma_hip_dsa_inject_reacc(const CromaHipGridSpec *grid, const CromaHipDsaParams *p, int mode, int ntracer,
+                               double dt_gyr, const double *mach, const double *temp1_K, const double *rho1_cgs,
+                               const double *u1_cgs, const double *kinetic_flux, const double *side_code,
+                               const double *tracer_mass, double *crp, double *cre, double *qpi, double *qepri,
+                               unsigned char *disable_adiabatic, int *ninjected)
 {
     ChGrid *g = 0;
     DsJob J;
     int rc;
     if (ninjected) *ninjected = 0;
     if (!p || ntracer < 0 || !mach || !rho1_cgs || !tracer_mass || (!kinetic_flux && !temp1_K)) return CROMA_HIP_EINVAL;
-    if (mode == CROMA_HIP_DSA_STATE && (!crp || !cre)) return CROMA_HIP_EINVAL;
+    if (p->reacc_mode < CROMA_HIP_DSA_REACC_OFF || p->reacc_mode > CROMA_HIP_DSA_REACC_DRURY83 ||
+        p->eta_model < 0 || p->eta_model >= 8 || p->eta_model_reacc < 0 || p->eta_model_reacc >= 8)
+        return CROMA_HIP_EINVAL;
+    /* state mode and any reacceleration read or write the state rows
+     * (tracer_fp_dsa.c:214-221) */
+    if ((mode == CROMA_HIP_DSA_STATE || p->reacc_mode != CROMA_HIP_DSA_REACC_OFF) && (!crp || !cre))
+        return CROMA_HIP_EINVAL;
     if (mode == CROMA_HIP_DSA_SOURCE && (!qpi || !qepri || !(dt_gyr > 0.0) || !isfinite(dt_gyr)))
         return CROMA_HIP_EINVAL;
     if (mode != CROMA_HIP_DSA_STATE && mode != CROMA_HIP_DSA_SOURCE) return CROMA_HIP_EINVAL;
@@ -429,8 +722,19 @@ int croma_hip_dsa_inject(const CromaHipGridSpec *grid, const CromaHipDsaParams *
     J.cre = cre;
     J.qpi = qpi;
     J.qepri = qepri;
+    J.dis = disable_adiabatic;
     ch_parallel_for(ntracer, 16, ds_range, &J);
     if (ninjected) *ninjected = J.ninj[0];
     free(g);
     return 0;
+}
+
+int croma_hip_dsa_inject(const CromaHipGridSpec *grid, const CromaHipDsaParams *p, int mode, int ntracer,
+                         double dt_gyr, const double *mach, const double *temp1_K, const double *rho1_cgs,
+                         const double *u1_cgs, const double *kinetic_flux, const double *side_code,
+                         const double *tracer_mass, double *crp, double *cre, double *qpi, double *qepri,
+                         int *ninjected)
+{
+    return croma_hip_dsa_inject_reacc(grid, p, mode, ntracer, dt_gyr, mach, temp1_K, rho1_cgs, u1_cgs, kinetic_flux,
+                                      side_code, tracer_mass, crp, cre, qpi, qepri, 0, ninjected);
 }

@@ -405,8 +405,8 @@ int croma_hip_batch_checkpoint_read(CromaHipBatch *b, const char *dir, const Cro
 
 /* ---- DSA source builder (tracer_fp_apply_dsa_tracer_injection_snapshot,
  * tracer_fp_dsa.c:184-505; DSA_MODELS.c:394-1558) -----------------------
- * Shock-accelerated CR injection per tracer, without reacceleration: the
- * proton spectrum A_p (p/p_inj)^-q exp(-(p/p_max)^2) normalised to
+ * Shock-accelerated CR injection per tracer, with or without
+ * reacceleration of the pre-shock spectrum: the proton spectrum A_p (p/p_inj)^-q exp(-(p/p_max)^2) normalised to
  * eta(M) of the upstream kinetic energy flux, the electrons at K_ep of the
  * protons at p_inj,p (suprathermal injection), on the FP grid as 4 pi p^2 f. */
 #define CROMA_HIP_DSA_KR07 0   /* eta models, DSAModelID order (DSA_MODELS.h:37-47) */
@@ -432,10 +432,23 @@ typedef struct {
     double min_mach;        /* DSAMinMach: 1.5 */
     double mu_mol;          /* mu_mol: 0.59 */
     double pmin_dsa_p_log10, pmin_dsa_e_log10; /* DSA grid start (tracer_fp.c:841): -3, -3 */
+    /* reacceleration of the tracer's pre-shock spectrum (TracerDsaReaccMode,
+     * include/tracer_fp_types.h:321-325; tracer_fp_dsa.c:322-418) */
+    int reacc_mode;         /* CROMA_HIP_DSA_REACC_*: off */
+    int eta_model_reacc;    /* DSAEtaModelReacc (params.c:98): RYU19R */
+    double reacc_eta_cap;   /* DSAReaccEtaCap (params.c:110): 0 (<= 0: no cap) */
+    double xcr_pmin_pmc;    /* DSAXcrPminPmc (params.c:109): 1 */
 } CromaHipDsaParams;
+#define CROMA_HIP_DSA_REACC_OFF 0
+#define CROMA_HIP_DSA_REACC_POSITIVE_DELTA 1 /* eta interpolated in X_cr, extra energy as a positive delta */
+#define CROMA_HIP_DSA_REACC_DRURY83 2        /* Drury 1983 convolution of the pre-shock spectrum */
 void croma_hip_dsa_default_params(CromaHipDsaParams *p);
 /* eta(M) of a model (dsa_eta_from_id). */
 double croma_hip_dsa_eta(double mach, int eta_model);
+/* eta between an initial and a reacceleration model, weighted by
+ * x_cr / x_cr0 of the latter clipped to [0, 1] (dsa_eta_interpolated,
+ * DSA_MODELS.c:502-522). */
+double croma_hip_dsa_eta_interpolated(double mach, int initial_model, int reacc_model, double x_cr);
 /* Source rows of one shock on a grid: source_p [n], source_e [n] (per unit
  * volume, 4 pi p^2 f), flags[0] / [1] = proton / electron source built.
  * u1 <= 0 derives it from mach and temp1. */
@@ -455,6 +468,35 @@ int croma_hip_dsa_inject(const CromaHipGridSpec *grid, const CromaHipDsaParams *
                          const double *u1_cgs, const double *kinetic_flux, const double *side_code,
                          const double *tracer_mass, double *crp, double *cre, double *qpi, double *qepri,
                          int *ninjected);
+/* One reacceleration row of a spectrum on a grid (tracer_fp_dsa_reacc.h:
+ * tracer_dsa_reacc_positive_delta_1d, _convolution_delta_1d,
+ * _adiabatic_delta_1d, _delta_energy_density_1d).  species 0 protons (the
+ * proton grid), 1 electrons.  kind ENERGY takes the delta row as state and
+ * only returns its energy density.  *edens: the function's return value. */
+#define CROMA_HIP_DSA_REACC_KIND_POSITIVE 0
+#define CROMA_HIP_DSA_REACC_KIND_DRURY 1
+#define CROMA_HIP_DSA_REACC_KIND_ADIABATIC 2
+#define CROMA_HIP_DSA_REACC_KIND_ENERGY 3
+int croma_hip_dsa_reacc_delta(const CromaHipGridSpec *grid, int kind, int species, const double *state,
+                              double volume_upstream, double volume_downstream, double q, double compression_ratio,
+                              double pmin_mc, double *delta, double *edens);
+/* croma_hip_dsa_inject with reacceleration (p->reacc_mode, tracer_fp_dsa.c:312-505 with
+ * tracer_fp_dsa_reacc.c): the tracer's current spectra crp / cre (required
+ * for any reacceleration mode, also in source mode; read before this
+ * snapshot's injection is added) give the reacceleration delta, added like
+ * the fresh source (state: V_down delta x scale; source: / dt).  Positive
+ * delta: eta interpolated between the initial and the reacceleration model
+ * in X_cr = P_cr / P_th (protons above xcr_pmin_pmc), the extra energy
+ * (eta_interp - eta) M u1^2 / 2 caps the delta's energy.  Drury83: the
+ * convolution delta, its non-adiabatic energy capped at
+ * reacc_eta_cap M u1^2 / 2 when reacc_eta_cap > 0; each such tracer gets
+ * disable_adiabatic[i] = 1 (NULL: not written), the flag the solve's
+ * per-tracer dis input takes (include/tracer_fp_solve.inc:80-94). */
+int croma_hip_dsa_inject_reacc(const CromaHipGridSpec *grid, const CromaHipDsaParams *p, int mode, int ntracer,
+                               double dt_gyr, const double *mach, const double *temp1_K, const double *rho1_cgs,
+                               const double *u1_cgs, const double *kinetic_flux, const double *side_code,
+                               const double *tracer_mass, double *crp, double *cre, double *qpi, double *qepri,
+                               unsigned char *disable_adiabatic, int *ninjected);
 
 /* ---- serving: continuous batching of tracer chunks ------------------ */
 typedef struct CromaHipServer CromaHipServer;

@@ -3,6 +3,7 @@
 # where they lie under /root/reference; nothing is copied.
 #
 # Compiled: fp_shared_core.c FP_Coef.c CONSTANTS.c params.c tracer_fp_nsub.c DSA_MODELS.c
+#   tracer_fp_dsa_reacc.c
 # Not compilable here (GSL / MPI headers absent): Chang_Cooper.c HADRONIC.c
 #   Synchrotron.c EMISSION.c tracer_fp_batch.c tracer_fp_setup.c tracer_fp_dsa.c
 # The wrappers in ref_harness.c reach only the compiled functions; the link
@@ -13,7 +14,8 @@ HERE := $(dir $(abspath $(lastword $(MAKEFILE_LIST))))
 OUT := $(HERE)_ref
 SIZES := 64 128 256 512
 SRCS := $(REF)/src/fp_shared_core.c $(REF)/src/FP_Coef.c $(REF)/src/CONSTANTS.c \
-        $(REF)/src/params.c $(REF)/src/tracer_fp_nsub.c $(REF)/src/DSA_MODELS.c
+        $(REF)/src/params.c $(REF)/src/tracer_fp_nsub.c $(REF)/src/DSA_MODELS.c \
+        $(REF)/src/tracer_fp_dsa_reacc.c
 CFLAGS_REF := -O2 -std=gnu11 -fPIC -fopenmp -ffp-contract=off -w \
               -ffunction-sections -fdata-sections -I$(REF)/include
 

@@ -4,7 +4,8 @@
  * restatement can be checked against them.  TEST INFRASTRUCTURE ONLY.
  *
  * Only reference files that compile with this image's toolchain are used
- * (fp_shared_core.c, FP_Coef.c, CONSTANTS.c, params.c, tracer_fp_nsub.c).
+ * (fp_shared_core.c, FP_Coef.c, CONSTANTS.c, params.c, tracer_fp_nsub.c,
+ * DSA_MODELS.c, tracer_fp_dsa_reacc.c).
  * Chang_Cooper.c, HADRONIC.c, Synchrotron.c, EMISSION.c and the batch driver
  * need GSL / MPI headers that are absent, so nothing here reaches them (the
  * synchrotron emission driver is left out: its fallback branch calls
@@ -297,3 +298,33 @@ REF_API int ref_dsa_sources(int eta_model, double chi_p, double chi_e, double k_
 }
 
 REF_API double ref_dsa_eta(double mach, int eta_model) { return dsa_eta_from_id(mach, (DSAModelID)eta_model); }
+
+/* Reacceleration pieces (tracer_fp_dsa_reacc.c, DSA_MODELS.c:474-522) on the
+ * harness grid: kind 0 positive delta (volume_downstream = v2), 1 Drury83
+ * convolution (v1 upstream, v2 downstream), 2 adiabatic compression delta
+ * (compression ratio r); species 0 protons (proton grid), 1 electrons
+ * (electron grid).  Returns the function's energy density; delta [n]. */
+#include "tracer_fp_dsa_reacc.h"
+REF_API double ref_dsa_reacc(int kind, int species, const double *state, double v1, double v2, double q, double r,
+                             double pmin_mc, double *delta)
+{
+    const CRspectrum *g = species ? &g_cre : &g_crp;
+    const int n = species ? npe : np;
+    const DSASpecies sp = species ? DSA_SPECIES_ELECTRON : DSA_SPECIES_PROTON;
+    if (kind == 0) return tracer_dsa_reacc_positive_delta_1d(g, state, n, v2, q, pmin_mc, delta, sp);
+    if (kind == 1) return tracer_dsa_reacc_convolution_delta_1d(g, state, n, v1, v2, q, pmin_mc, delta, sp);
+    return tracer_dsa_reacc_adiabatic_delta_1d(g, state, n, v1, v2, r, pmin_mc, delta, sp);
+}
+
+/* the energy density of a delta row (tracer_dsa_reacc_delta_energy_density_1d) */
+REF_API double ref_dsa_reacc_edens(int species, const double *delta, double pmin_mc)
+{
+    const CRspectrum *g = species ? &g_cre : &g_crp;
+    return tracer_dsa_reacc_delta_energy_density_1d(g, delta, species ? npe : np, pmin_mc,
+                                                    species ? DSA_SPECIES_ELECTRON : DSA_SPECIES_PROTON);
+}
+
+REF_API double ref_dsa_eta_interpolated(double mach, int initial_id, int reacc_id, double x_cr)
+{
+    return dsa_eta_interpolated(mach, (DSAModelID)initial_id, (DSAModelID)reacc_id, x_cr);
+}

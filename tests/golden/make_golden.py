@@ -17,7 +17,8 @@ F(x) itself is checked only against its asymptotes (parity unpinned vs GSL).
 It also records the byte layout of the reference's checkpoint header
 (ckpt_header_layout.json) from the compiled struct.
 
-and the DSA source rows of seeded shocks (dsa_n{64,256}.npz).
+and the DSA source rows of seeded shocks (dsa_n{64,256}.npz) and the
+reacceleration rows of seeded pre-shock spectra (dsa_reacc_n{64,256}.npz).
 
 Usage:  python tests/golden/make_golden.py [--layout-only | --dsa-only]
 """
@@ -270,6 +271,59 @@ def dsa_fixture(n, rng):
                 pmax_e=pmax_e, source_p=sp, source_e=se, flags=flags, spec=spec, eta_mach=mgrid, eta=eta)
 
 
+def reacc_fixture(n, rng):
+    """Reacceleration rows (tracer_fp_dsa_reacc.c through ref_dsa_reacc /
+    ref_dsa_reacc_edens) of seeded pre-shock spectra on the harness grid, and
+    eta interpolated between initial and reacceleration models
+    (DSA_MODELS.c:474-522 through ref_dsa_eta_interpolated)."""
+    lib = ref_lib(n)
+    pp, pdp, pE = np.zeros(n), np.zeros(n), np.zeros(n)
+    ep, edp, eE = np.zeros(n), np.zeros(n), np.zeros(n)
+    pg, eg, bp = np.zeros(2), np.zeros(2), np.zeros(n)
+    d = C.c_double
+    lib.ref_grid(d(-1.0), d(8.0), d(0.0), d(6.0), P(pp), P(pdp), P(pE), P(pg), P(ep), P(edp), P(eE), P(eg), P(bp))
+    lib.ref_dsa_reacc.restype = C.c_double
+    lib.ref_dsa_reacc_edens.restype = C.c_double
+    lib.ref_dsa_eta_interpolated.restype = C.c_double
+    ncase = 24
+    mach = 10 ** rng.uniform(np.log10(1.5), np.log10(30.0), ncase)
+    g = 5.0 / 3.0
+    r = (g + 1.0) * mach ** 2 / ((g - 1.0) * mach ** 2 + 2.0)
+    q = 3.0 * r / (r - 1.0)
+    v1 = 10 ** rng.uniform(60.0, 70.0, ncase)
+    v2 = v1 / r
+    pmin = 10 ** rng.uniform(-3.0, 0.5, ncase)
+    state = np.zeros((ncase, 2, n))
+    for i in range(ncase):
+        for sp, axis in ((0, pp), (1, ep)):
+            s_ = rng.uniform(2.0, 3.5)
+            cut = 10 ** rng.uniform(1.0, 6.0)
+            state[i, sp] = 10 ** rng.uniform(50, 60) * axis ** -s_ * np.exp(-axis / cut)
+        if i % 6 == 5:
+            state[i, :, n // 3: n // 2] = 0.0  # a gap
+        if i % 8 == 7:
+            state[i] = 0.0  # empty
+    delta = np.zeros((ncase, 3, 2, n))
+    edens = np.zeros((ncase, 3, 2))
+    edens_of = np.zeros((ncase, 3, 2))
+    for i in range(ncase):
+        for kind in range(3):
+            for sp in range(2):
+                row = np.ascontiguousarray(state[i, sp])
+                out = np.zeros(n)
+                edens[i, kind, sp] = lib.ref_dsa_reacc(kind, sp, P(row), d(v1[i]), d(v2[i]), d(q[i]), d(r[i]),
+                                                       d(pmin[i]), P(out))
+                delta[i, kind, sp] = out
+                edens_of[i, kind, sp] = lib.ref_dsa_reacc_edens(sp, P(out), d(pmin[i]))
+    ms = np.concatenate([np.linspace(1.0, 3.0, 41), np.linspace(3.0, 40.0, 75)])
+    xs = np.array([-0.1, 0.0, 0.01, 0.03, 0.05, 0.1, 0.3, 1.0])
+    pairs = np.array([(0, 1), (2, 3), (4, 5), (6, 7), (6, 1), (0, 0)], np.int32)
+    eta_i = np.array([[[lib.ref_dsa_eta_interpolated(d(m), int(a), int(b), d(x)) for x in xs] for m in ms]
+                      for a, b in pairs])
+    return dict(mach=mach, r=r, q=q, v1=v1, v2=v2, pmin=pmin, state=state, delta=delta, edens=edens,
+                edens_of=edens_of, eta_mach=ms, eta_x=xs, eta_pairs=pairs, eta_interp=eta_i)
+
+
 def main():
     if not os.path.isdir("/root/reference"):
         print("reference sources absent; nothing to do")
@@ -281,6 +335,9 @@ def main():
     drng = np.random.default_rng(8080)
     for n in (64, 256):
         np.savez_compressed(os.path.join(HERE, f"dsa_n{n}.npz"), **dsa_fixture(n, drng))
+    rrng = np.random.default_rng(8181)
+    for n in (64, 256):
+        np.savez_compressed(os.path.join(HERE, f"dsa_reacc_n{n}.npz"), **reacc_fixture(n, rrng))
     if "--dsa-only" in sys.argv:
         return 0
     logx, logfx, fx, xmin = fx_table()
