@@ -38,6 +38,27 @@ __device__ __forceinline__ uint16_t em_bf16_bits(float f)
 
 __device__ __forceinline__ float em_bf16_val(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
+// Lane permutations inside 16-lane rows by DPP (every lane has a source):
+// EM_XOR1 / EM_XOR2 swap lanes l ^ 1 / l ^ 2 (quad_perm), EM_HMIRROR maps
+// lane i of an 8-lane half row to 7 - i (pairs its two quads), EM_MIRROR
+// lane i of the row to 15 - i (pairs its two halves).
+#define EM_XOR1 0xB1
+#define EM_XOR2 0x4E
+#define EM_HMIRROR 0x141
+#define EM_MIRROR 0x140
+template <int CTRL>
+__device__ __forceinline__ double em_dpp64(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+template <int CTRL>
+__device__ __forceinline__ int em_dpp32(int v)
+{
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+
 // Largest binary exponent of row i of spectra (frexp convention), 0 for an
 // all-zero row.  4 threads per row.
 __device__ __forceinline__ int em_row_shift(const double *__restrict__ row, int K, int sub)
@@ -743,21 +764,14 @@ __global__ __launch_bounds__(256) void k_emis_synch(int ntr, int n, int nfreq, i
 // no term leaves float's range however far the spectrum and the kernel span:
 // log2|w| per bin in LDS (w = N dp), log2 K interpolated on the float log2
 // table, the hardware exp2, and the FP64 result A acc 2^E.
-__global__ __launch_bounds__(256) void k_emis_synch32(int ntr, int n, int nfreq, int nlogb, double logb_min,
-                                                      double inv_dlogb, const double *__restrict__ table,
-                                                      const float *__restrict__ lt, const int *__restrict__ perm,
-                                                      const double *__restrict__ cre, const double *__restrict__ dp,
-                                                      const double *__restrict__ b_field, double *__restrict__ eps,
-                                                      const int *__restrict__ count)
+// One tracer of the FP32 log-domain sum (wave w of the block, LDS rows s_lw /
+// s_sg [w]).
+__device__ __forceinline__ void em_synch32_one(int it, int lane, int w, int n, int nfreq, int nlogb, double logb_min,
+                                               double inv_dlogb, const double *__restrict__ table,
+                                               const float *__restrict__ lt, const double *__restrict__ cre,
+                                               const double *__restrict__ dp, const double *__restrict__ b_field,
+                                               double *__restrict__ eps, float (*s_lw)[512], float (*s_sg)[512])
 {
-    __shared__ float s_lw[4][512];
-    __shared__ float s_sg[4][512];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int slot = blockIdx.x * 4 + w;
-    // count (device): the number of tracers listed in perm, when the list
-    // was made on the device
-    if (slot >= (count ? *count : ntr)) return;
-    const int it = perm ? perm[slot] : slot;
     const double *cl = cre + (size_t)it * n;
     int jend = n - 1;
     for (int base = 2; base < n - 1; base += 64) {
@@ -841,6 +855,38 @@ __global__ __launch_bounds__(256) void k_emis_synch32(int ntr, int n, int nfreq,
     }
 }
 
+template <bool LOOP>
+__global__ __launch_bounds__(256) void k_emis_synch32(int ntr, int n, int nfreq, int nlogb, double logb_min,
+                                                      double inv_dlogb, const double *__restrict__ table,
+                                                      const float *__restrict__ lt, const int *__restrict__ perm,
+                                                      const double *__restrict__ cre, const double *__restrict__ dp,
+                                                      const double *__restrict__ b_field, double *__restrict__ eps,
+                                                      const int *__restrict__ count)
+{
+    __shared__ float s_lw[4][512];
+    __shared__ float s_sg[4][512];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    // count (device): the number of tracers listed in perm, when the list
+    // was made on the device; with LOOP the waves of a smaller grid loop
+    // over the slots, else one slot per wave
+    const int nslot = count ? *count : ntr;
+    if constexpr (!LOOP) {
+        const int slot = blockIdx.x * 4 + w;
+        if (slot >= nslot) return;
+        em_synch32_one(perm ? perm[slot] : slot, lane, w, n, nfreq, nlogb, logb_min, inv_dlogb, table, lt, cre, dp,
+                       b_field, eps, s_lw, s_sg);
+    } else {
+        for (int slot = blockIdx.x * 4 + w; slot < nslot; slot += (int)gridDim.x * 4) {
+            em_synch32_one(perm ? perm[slot] : slot, lane, w, n, nfreq, nlogb, logb_min, inv_dlogb, table, lt, cre,
+                           dp, b_field, eps, s_lw, s_sg);
+            // the next slot rewrites this wave's LDS rows
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Synchrotron on MFMA.  Between table planes p and p + 1 the reference
 // interpolates the kernel log-linearly in log B where both entries are
@@ -908,10 +954,19 @@ __device__ __forceinline__ void em_synch_cheb(double K0, double K1, double w, do
     c[1] = 0.5 * (K1 - K0) * w;
 }
 
-// Pass 0: the largest binary exponent of every column (p, f * 8 + m) over its
-// bins (atomic max); pass 1: the entries scaled by their column's power of
-// two, split into bf16 hi / lo.  One thread per (p, f, j); bins 0 and 1 are
-// zero (the sum starts at bin 2).  kt [p][f * 8 + m][j], col_shift [p][f * 8 + m].
+// Pass 0: the largest binary exponent of every frequency's 8 columns
+// (p, f, m = 0..7) over their bins (atomic max); pass 1: the entries scaled
+// by that power of two (one scale per frequency, so the Chebyshev sum of the
+// columns needs no per-column rescaling), split into bf16 hi / lo.  One
+// thread per (p, f, j); bins 0 and 1 are zero (the sum starts at bin 2).
+// col_shift [p][f].  kt holds the columns in blocks of 64 in the order the
+// contraction keeps them in LDS: frequencies in groups of 16, each group as
+// two blocks (terms 0..3, then 4..7), block column 16 c + r = frequency
+// 16 g + r, term 4 h + c; inside a block kt [ks][q][column][sl] for bin
+// j = 32 ks + 8 (sl / 2) + 2 q + sl % 2 (the operand k order of
+// v_mfma_f32_16x16x32_bf16 permuted alike for both operands), so a block
+// copies one column block with 16-byte loads.  Frequencies past nfreq stay
+// zero.
 __global__ __launch_bounds__(256) void k_synch_cheb(int pass, int n, int nfreq, int nlogb,
                                                     const double *__restrict__ table, const double *__restrict__ dp,
                                                     uint16_t *__restrict__ kt_hi, uint16_t *__restrict__ kt_lo,
@@ -931,281 +986,397 @@ __global__ __launch_bounds__(256) void k_synch_cheb(int pass, int n, int nfreq, 
         em_synch_cheb(table[((size_t)pl * nfreq + f) * n + j], table[((size_t)(pl + 1) * nfreq + f) * n + j], dp[j],
                       c);
     }
-    int *cs = col_shift + ((size_t)pl * nfreq + f) * EM_SY_M;
+    int *cs = col_shift + (size_t)pl * nfreq + f;
     if (pass == 0) {
+        int e = -100000;
 #pragma unroll
         for (int m = 0; m < EM_SY_M; ++m) {
             const double v = fabs(c[m]);
-            if (v > 0.0 && isfinite(v)) atomicMax(&cs[m], __builtin_amdgcn_frexp_exp(v));
+            if (v > 0.0 && isfinite(v)) e = max(e, __builtin_amdgcn_frexp_exp(v));
         }
+        if (e > -100000) atomicMax(cs, e);
         return;
     }
+    const int sh = *cs < -2000 ? 0 : *cs - EM_SY_OFF;
+    const int nblk = 2 * ((nfreq + 15) / 16);
+    const int kk = j & 31;
+    const size_t kofs = ((size_t)(j >> 5) * 4 + ((kk & 7) >> 1)) * (EM_TB * 8) + 2 * (kk >> 3) + (kk & 1);
 #pragma unroll
     for (int m = 0; m < EM_SY_M; ++m) {
-        const int sh = cs[m] < -2000 ? 0 : cs[m] - EM_SY_OFF;
         const float v = (float)ldexp(isfinite(c[m]) ? c[m] : 0.0, -sh);
         const uint16_t h = em_bf16_bits(v);
-        const size_t g = (((size_t)pl * nfreq + f) * EM_SY_M + m) * n + j;
+        const int blk = 2 * (f / 16) + m / 4, col = 16 * (m % 4) + f % 16;
+        const size_t g = ((size_t)pl * nblk + blk) * (size_t)(n / 32) * 4 * EM_TB * 8 + kofs + (size_t)col * 8;
         kt_hi[g] = h;
         kt_lo[g] = em_bf16_bits(v - em_bf16_val(h));
     }
 }
 
-// Per tracer: the end of its sum (first bin whose successor is not positive,
-// as k_emis_synch32), s = 2 t - 1 of its field in its plane interval, the
-// prefactor A(B) and the largest binary exponent of the bins before the end
-// (the contraction's row scale; EM_SY_NONE for an empty row).  One wave per
-// tracer.
 #define EM_SY_NONE (-100000)
-__global__ __launch_bounds__(256) void k_synch_prep(int ntr, int n, int nlogb, double logb_min, double inv_dlogb,
-                                                    const double *__restrict__ cre,
-                                                    const double *__restrict__ b_field, int *__restrict__ jend_out,
-                                                    double *__restrict__ s_out, double *__restrict__ pref_out,
-                                                    int *__restrict__ srow_out)
+
+// Work items of the fused contraction: plane interval p belongs to XCD p % 8
+// (blockIdx % 8 of the persistent grid) and holds ceil(count_p / tpb) items;
+// ioff[p] = the items of the XCD's earlier planes, itot[x] = the XCD's
+// total.  One wave per XCD.
+__global__ __launch_bounds__(512) void k_synch_items(int nlogb, int tpb, const int *__restrict__ plane_end,
+                                                     int *__restrict__ ioff, int *__restrict__ itot)
 {
-    const int lane = threadIdx.x & 63;
-    const int it = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (it >= ntr) return;
-    const double *cl = cre + (size_t)it * n;
-    // one pass over the row: the end bin and the exponents of the bins
-    // before it (bins 0 and 1 always come before it)
-    int e = EM_SY_NONE;
-    if (lane < 2) {
-        const double v = fabs(cl[lane]);
-        if (v > 0.0 && isfinite(v)) e = __builtin_amdgcn_frexp_exp(v);
-    }
-    int jend = n - 1;
-    for (int base = 2; base < n - 1; base += 64) {
-        const int je = base + lane;
-        const bool in = je < n - 1;
-        const double v = in ? fabs(cl[je]) : 0.0;
-        const bool stop = in && !(cl[je + 1] > 0.0);
-        const unsigned long long m = __ballot(stop);
-        const int lim = m ? base + __ffsll((long long)m) - 1 : n - 1;
-        if (je < lim && v > 0.0 && isfinite(v)) e = max(e, __builtin_amdgcn_frexp_exp(v));
-        if (m) {
-            jend = lim;
-            break;
-        }
-    }
+    const int lane = threadIdx.x & 63, x = threadIdx.x >> 6;
+    const int np = nlogb - 1;
+    const int npx = np > x ? (np - x + 7) / 8 : 0;
+    int carry = 0;
+    for (int base = 0; base < npx; base += 64) {
+        const int i = base + lane;
+        const int p = x + 8 * i;
+        const int c = i < npx ? plane_end[p] - (p ? plane_end[p - 1] : 0) : 0;
+        const int items = (c + tpb - 1) / tpb;
+        int incl = items;
 #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) e = max(e, __shfl_xor(e, o, 64));
-    if (lane == 0) {
-        double bval = b_field[it];
-        if (!(bval > 0.0) || !isfinite(bval)) bval = 1.0e-30;
-        const double u = (log10(bval / 1.0e-6) - logb_min) * inv_dlogb;
-        int ib = (int)floor(u);
-        if (ib < 0) ib = 0;
-        if (ib > nlogb - 2) ib = nlogb - 2;
-        double tt = u - (double)ib;
-        if (tt < 0.0) tt = 0.0;
-        if (tt > 1.0) tt = 1.0;
-        srow_out[it] = e;
-        jend_out[it] = jend;
-        s_out[it] = 2.0 * tt - 1.0;
-        pref_out[it] = sqrt(3.0) * pow(4.8e-10, 3.0) * b_field[it] / (2.0 * 9.11e-28 * 2.999e10 * 2.999e10) /
-                       (4.0 * 3.14159265358979323846);
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += y;
+        }
+        if (i < npx) ioff[p] = carry + incl - items;
+        carry += __shfl(incl, 63, 64);
     }
+    if (lane == 0) itot[x] = carry;
 }
 
-// The plane-grouped contraction: block (plane interval p, column block cb)
-// keeps the 64 columns cb * 64 .. (8 frequencies x 8 Chebyshev terms) of
-// plane p in LDS and streams the 16-tracer tiles of p's tracers (perm order,
-// p's range [hist[p - 1], hist[p])) as k_emis_stream does, the rows masked
-// past each tracer's end bin.  Blocks are numbered so that the column
-// blocks of one plane share an XCD (blockIdx % 8), whose L2 then serves the
-// plane's spectra to all of them.  Output: eps[i][f] = A_i sum_m T_m(s_i) G.
-template <int KS>
-__global__ __launch_bounds__(256) void k_synch_mfma(int nfreq, int nlogb, int ncb, const double *__restrict__ cre,
-                                                    const int *__restrict__ perm, const int *__restrict__ plane_end,
-                                                    const int *__restrict__ jend, const double *__restrict__ sv,
-                                                    const double *__restrict__ pref,
-                                                    const uint16_t *__restrict__ kt_hi,
-                                                    const uint16_t *__restrict__ kt_lo,
-                                                    const int *__restrict__ col_shift, double *__restrict__ eps)
+// The plane-grouped contraction, one pass over the spectra.  A persistent
+// block (XCD x = blockIdx % 8) of NW waves takes work items of x's planes:
+// 16 TPW NW tracers of one plane interval p, TPW 16-tracer tiles per wave.
+// Per item:
+//  1. each wave reads its tiles' rows, finds each row's end bin (the first
+//     bin from 3 on that is not positive ends the sum at the bin before: the
+//     sum runs over bins 2 .. jend - 1, as k_emis_synch32) and its largest
+//     binary exponent over the bins before the end, and keeps the rows in
+//     registers as split BF16 operands (scaled to a largest entry of 2^59,
+//     masked past the end);
+//  2. the block copies plane p's column blocks one after the other into
+//     LDS; every wave runs its tiles against each with
+//     v_mfma_f32_16x16x32_bf16 (split x3).  Two blocks make a group of 16
+//     frequencies: lane (r, q) then holds all 8 Chebyshev terms of frequency
+//     16 g + r for rows 4 q + j, and writes
+//     eps[i][f] = A_i 2^(scales) sum_m T_m(s_i) G[i][f, m] with no exchange
+//     between lanes;
+//  3. the range check, on the tracer's own emissivities: an operand entry
+//     below 2^-126 after scaling (or a product below it) is lost, so each
+//     column sum of tracer i misses at most 256 x 2^-67 x 2^(srow_i + cs - 118)
+//     of its true value (cs: the frequency's binary exponent) and eps_f at
+//     most lim_f = |A| 8 2^(srow_i + cs_f - 177); where lim_f exceeds 2^-17
+//     (1e-5) of an emissivity within 25 decades of the tracer's largest one
+//     (or the row has content and every emissivity came out zero), the tracer
+//     is listed for the FP32 log-domain kernel, which has no range limit.
+// Planes are spread over XCDs by p % 8, so a plane's column blocks (n x 512
+// x 4 bytes) are read from HBM once and then served by that XCD's L2.
+template <int KS, int TPW, int NW>
+__global__ __launch_bounds__(64 * NW, 2) void k_synch_fused(int nfreq, int nlogb, double logb_min, double inv_dlogb,
+                                                            const double *__restrict__ cre,
+                                                            const double *__restrict__ b_field,
+                                                            const int *__restrict__ perm,
+                                                            const int *__restrict__ plane_end,
+                                                            const int *__restrict__ ioff,
+                                                            const int *__restrict__ itot,
+                                                            const uint16_t *__restrict__ kt_hi,
+                                                            const uint16_t *__restrict__ kt_lo,
+                                                            const int *__restrict__ col_shift,
+                                                            double *__restrict__ eps, int *__restrict__ fix)
 {
     constexpr int K = 32 * KS;
-    __shared__ __attribute__((aligned(16))) uint16_t b_hi[KS][4][EM_TB][8];
-    __shared__ __attribute__((aligned(16))) uint16_t b_lo[KS][4][EM_TB][8];
-    const int xcd = blockIdx.x & 7, loc = blockIdx.x >> 3;
-    const int cb = loc % ncb;
-    const int pl = (loc / ncb) * 8 + xcd;
-    if (pl >= nlogb - 1) return;
-    const int start = pl ? plane_end[pl - 1] : 0, cnt = plane_end[pl] - start;
-    if (cnt <= 0) return;
-    const int ncol = nfreq * EM_SY_M;
-    const int t = threadIdx.x;
-    const int lane = t & 63, w = t >> 6;
-    const size_t pbase = (size_t)pl * ncol;
-    for (int e = t; e < KS * 4 * EM_TB * 8; e += 256) {
-        const int sl = e & 7, c = (e >> 3) % EM_TB, q = (e / (8 * EM_TB)) & 3, ks = e / (32 * EM_TB);
-        const int k = 32 * ks + 8 * (sl >> 1) + 2 * q + (sl & 1);
-        const int col = cb * EM_TB + c;
-        const size_t g = (pbase + (col < ncol ? col : 0)) * K + k;
-        b_hi[ks][q][c][sl] = col < ncol ? kt_hi[g] : (uint16_t)0;
-        b_lo[ks][q][c][sl] = col < ncol ? kt_lo[g] : (uint16_t)0;
-    }
-    __syncthreads();
+    constexpr int CH = KS * 4 * EM_TB * 8; // bf16 per half of one column block
+    constexpr int NONE = EM_SY_NONE;
+    constexpr int NT = 64 * NW;            // threads
+    constexpr int TPB = 16 * TPW * NW;     // tracers per item
+    constexpr int HC = CH / 2; // bf16 per half of one half block (32 columns)
+    // two half-block buffers: one is filled while the other multiplies
+    __shared__ __attribute__((aligned(16))) uint16_t b_hi[2][HC];
+    __shared__ __attribute__((aligned(16))) uint16_t b_lo[2][HC];
+    __shared__ __attribute__((aligned(16))) double s_t[TPB][EM_SY_M]; // T_m(s) of each tracer
+    __shared__ double s_a[TPB];
+    __shared__ int s_it[TPB], s_sr[TPB];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int r = lane & 15, q = lane >> 4;
-    const int ntiles = (cnt + 15) / 16;
-    constexpr int KC = KS >= 4 ? 4 : (KS >= 2 ? 2 : 1);
-    constexpr int NCH = KS / KC;
-    constexpr int NONE = -100000;
-    for (int tile = w; tile < ntiles; tile += 4) {
-        const bool live = tile * 16 + r < cnt;
-        const int it = live ? perm[start + tile * 16 + r] : 0;
-        const int je = live ? jend[it] : 0;
-        const double *row = cre + (size_t)it * K + 2 * q;
-        floatx4 acc[4];
+    const int x = blockIdx.x & 7, nbx = gridDim.x >> 3;
+    const int np = nlogb - 1;
+    const int npx = np > x ? (np - x + 7) / 8 : 0;
+    const int nitems = itot[x];
+    const int ngrp = (nfreq + 15) / 16;
+    for (int item = blockIdx.x >> 3; item < nitems; item += nbx) {
+        // the item's plane: the last of x's planes whose first item is <= item
+        int lo = 0, hi = npx - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (ioff[x + 8 * mid] <= item) lo = mid;
+            else hi = mid - 1;
+        }
+        const int p = x + 8 * lo;
+        const int pend = plane_end[p];
+        const int t0 = (p ? plane_end[p - 1] : 0) + (item - ioff[p]) * TPB;
+        // 1. rows -> split BF16 operands
+        bf16x8 ah[TPW][KS], al[TPW][KS];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) acc[c] = floatx4{0.f, 0.f, 0.f, 0.f};
-        int s_row = NONE;
-        double2 nxt[KC][4];
+        for (int tt = 0; tt < TPW; ++tt) {
+            const int loc = 16 * (TPW * w + tt) + r;
+            // past the plane's end a lane reads the plane's last tracer (in
+            // bounds) and masks the whole row
+            const bool live = t0 + loc < pend;
+            const int it = perm[min(t0 + loc, pend - 1)];
+            // lane (r, q) holds bins 32 ks + 8 m + 2 q (+1) of row r.  Three
+            // reads of the row (the later ones from the cache), so the raw
+            // row never has to sit in registers whole:
+            // (a) the end of the sum and each 32-bin block's largest exponent
+            const double *row = cre + (size_t)it * K + 2 * q;
+            // (loads in groups of SG blocks: the scheduling barriers keep the
+            // compiler from hoisting every load of the row at once)
+            constexpr int SG = KS < 4 ? KS : 4;
+            // bins as compile-time offsets o from the lane's first bin 2 q
+            // (k = o + 2 q), so no per-bin index occupies a register
+            const int q2 = 2 * q;
+            int kfo = K, eblk[KS];
 #pragma unroll
-        for (int kk = 0; kk < KC; ++kk)
+            for (int g0 = 0; g0 < KS; g0 += SG) {
+                double2 v[SG][4];
 #pragma unroll
-            for (int m = 0; m < 4; ++m)
-                nxt[kk][m] = live ? *reinterpret_cast<const double2 *>(row + 32 * kk + 8 * m) : make_double2(0.0, 0.0);
-#pragma unroll 1
-        for (int ch = 0; ch < NCH; ++ch) {
-            double2 a[KC][4];
-#pragma unroll
-            for (int kk = 0; kk < KC; ++kk)
-#pragma unroll
-                for (int m = 0; m < 4; ++m) {
-                    // bins past the tracer's end bin do not enter its sum
-                    const int k = 32 * (KC * ch + kk) + 8 * m + 2 * q;
-                    a[kk][m].x = k < je ? nxt[kk][m].x : 0.0;
-                    a[kk][m].y = k + 1 < je ? nxt[kk][m].y : 0.0;
-                }
-            if (ch + 1 < NCH) {
-#pragma unroll
-                for (int kk = 0; kk < KC; ++kk)
+                for (int g = 0; g < SG; ++g)
 #pragma unroll
                     for (int m = 0; m < 4; ++m)
-                        nxt[kk][m] = live ? *reinterpret_cast<const double2 *>(row + 32 * (KC * (ch + 1) + kk) + 8 * m)
-                                          : make_double2(0.0, 0.0);
+                        v[g][m] = *reinterpret_cast<const double2 *>(row + 32 * (g0 + g) + 8 * m);
+#pragma unroll
+                for (int g = 0; g < SG; ++g) {
+                    int e = NONE;
+#pragma unroll
+                    for (int m = 0; m < 4; ++m) {
+                        const int o = 32 * (g0 + g) + 8 * m;
+                        // the first non-positive bin from 3 on (offsets 0, 1
+                        // are bins 2 q, 2 q + 1, from 3 on only for q >= 1)
+                        if ((o >= 3 || o + q2 >= 3) && !(v[g][m].x > 0.0)) kfo = min(kfo, o);
+                        if ((o + 1 >= 3 || o + 1 + q2 >= 3) && !(v[g][m].y > 0.0)) kfo = min(kfo, o + 1);
+                        const double v0 = fabs(v[g][m].x), v1 = fabs(v[g][m].y);
+                        if (v0 > 0.0 && isfinite(v0)) e = max(e, __builtin_amdgcn_frexp_exp(v0));
+                        if (v1 > 0.0 && isfinite(v1)) e = max(e, __builtin_amdgcn_frexp_exp(v1));
+                    }
+                    eblk[g0 + g] = e;
+                }
+                __builtin_amdgcn_sched_barrier(0);
             }
+            int kf = kfo + q2;
+            if (!live) kf = 0;
+            kf = min(kf, __shfl_xor(kf, 16, 64));
+            kf = min(kf, __shfl_xor(kf, 32, 64));
+            const int jend = kf - 1;
+            // a bin of the lane enters the sum if its offset is below jr
+            const int jr = jend - q2;
+            // (b) the largest exponent of the bins before the end: whole
+            // blocks before block jend / 32, that block bin by bin
+            const int kb = max(jend, 0) >> 5;
             int e = NONE;
 #pragma unroll
-            for (int kk = 0; kk < KC; ++kk)
+            for (int ks = 0; ks < KS; ++ks)
+                if (ks < kb) e = max(e, eblk[ks]);
 #pragma unroll
-                for (int m = 0; m < 4; ++m) {
-                    const double v0 = fabs(a[kk][m].x), v1 = fabs(a[kk][m].y);
-                    if (v0 > 0.0 && isfinite(v0)) e = max(e, __builtin_amdgcn_frexp_exp(v0));
-                    if (v1 > 0.0 && isfinite(v1)) e = max(e, __builtin_amdgcn_frexp_exp(v1));
-                }
+            for (int m = 0; m < 4; ++m) {
+                const int ob = 32 * kb + 8 * m;
+                const double2 v = *reinterpret_cast<const double2 *>(row + ob);
+                const double v0 = fabs(v.x), v1 = fabs(v.y);
+                if (ob < jr && v0 > 0.0 && isfinite(v0)) e = max(e, __builtin_amdgcn_frexp_exp(v0));
+                if (ob + 1 < jr && v1 > 0.0 && isfinite(v1)) e = max(e, __builtin_amdgcn_frexp_exp(v1));
+            }
             e = max(e, __shfl_xor(e, 16, 64));
             e = max(e, __shfl_xor(e, 32, 64));
-            const int s_new = max(s_row, e);
-            if (ch > 0) {
+            // (c) the split BF16 operands, masked past the end
+            const int sh = e == NONE ? 0 : e - EM_SY_OFF;
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const int rr = 4 * q + j;
-                    const int so = __shfl(s_row, rr, 64), sn = __shfl(s_new, rr, 64);
-                    if (so != sn && so != NONE) {
-                        const float f = ldexpf(1.0f, max(so - sn, -160));
-#pragma unroll
-                        for (int c = 0; c < 4; ++c) acc[c][j] *= f;
-                    }
-                }
-            }
-            s_row = s_new;
-            const int sh = s_row == NONE ? 0 : s_row - EM_SY_OFF;
-#pragma unroll
-            for (int kk = 0; kk < KC; ++kk) {
-                const int ks = KC * ch + kk;
-                bf16x8 ah, al;
+            for (int ks = 0; ks < KS; ++ks) {
+                if (ks % SG == 0 && ks > 0) __builtin_amdgcn_sched_barrier(0);
+                bf16x8 h8, l8;
 #pragma unroll
                 for (int m = 0; m < 4; ++m) {
-                    const float f0 = (float)ldexp(a[kk][m].x, -sh), f1 = (float)ldexp(a[kk][m].y, -sh);
+                    const int o = 32 * ks + 8 * m;
+                    const double2 v = *reinterpret_cast<const double2 *>(row + o);
+                    const float f0 = o < jr ? (float)ldexp(v.x, -sh) : 0.0f;
+                    const float f1 = o + 1 < jr ? (float)ldexp(v.y, -sh) : 0.0f;
                     const uint16_t h0 = em_bf16_bits(f0), h1 = em_bf16_bits(f1);
-                    ah[2 * m] = __builtin_bit_cast(__bf16, h0);
-                    ah[2 * m + 1] = __builtin_bit_cast(__bf16, h1);
-                    al[2 * m] = __builtin_bit_cast(__bf16, em_bf16_bits(f0 - em_bf16_val(h0)));
-                    al[2 * m + 1] = __builtin_bit_cast(__bf16, em_bf16_bits(f1 - em_bf16_val(h1)));
+                    h8[2 * m] = __builtin_bit_cast(__bf16, h0);
+                    h8[2 * m + 1] = __builtin_bit_cast(__bf16, h1);
+                    l8[2 * m] = __builtin_bit_cast(__bf16, em_bf16_bits(f0 - em_bf16_val(h0)));
+                    l8[2 * m + 1] = __builtin_bit_cast(__bf16, em_bf16_bits(f1 - em_bf16_val(h1)));
+                }
+                ah[tt][ks] = h8;
+                al[tt][ks] = l8;
+            }
+            if (q == 0) {
+                // per tracer: s = 2 t - 1 of its field in the plane
+                // interval, T_0 .. T_7 of s, the prefactor A(B)
+                double sv = 0.0, A = 0.0;
+                if (live) {
+                    double bval = b_field[it];
+                    if (!(bval > 0.0) || !isfinite(bval)) bval = 1.0e-30;
+                    const double u = (log10(bval / 1.0e-6) - logb_min) * inv_dlogb;
+                    int ib = (int)floor(u);
+                    if (ib < 0) ib = 0;
+                    if (ib > nlogb - 2) ib = nlogb - 2;
+                    double tt2 = u - (double)ib;
+                    if (tt2 < 0.0) tt2 = 0.0;
+                    if (tt2 > 1.0) tt2 = 1.0;
+                    sv = 2.0 * tt2 - 1.0;
+                    A = sqrt(3.0) * pow(4.8e-10, 3.0) * b_field[it] / (2.0 * 9.11e-28 * 2.999e10 * 2.999e10) /
+                        (4.0 * 3.14159265358979323846);
+                }
+                double c0 = 1.0, c1 = sv;
+                s_t[loc][0] = 1.0;
+                s_t[loc][1] = sv;
+#pragma unroll
+                for (int m = 2; m < EM_SY_M; ++m) {
+                    const double cm = 2.0 * sv * c1 - c0;
+                    c0 = c1;
+                    c1 = cm;
+                    s_t[loc][m] = cm;
+                }
+                s_a[loc] = A;
+                s_it[loc] = live ? it : -1;
+                s_sr[loc] = e;
+            }
+            // keep the second tile's loads after the first tile's conversion
+            // (the raw rows of both tiles at once would not fit the
+            // registers)
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        // the range check's running largest emissivity / largest violated
+        // bound of the lane's output rows (rows 4 q + j of each tile, the
+        // lane's frequencies)
+        double rmx[TPW][4], rlm[TPW][4];
+#pragma unroll
+        for (int tt = 0; tt < TPW; ++tt)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) rmx[tt][j] = rlm[tt][j] = 0.0;
+        // 2. the plane's column blocks, two per group of 16 frequencies, as
+        // half blocks of 32 columns (hh = 4 grp + 2 hf + hc: group, block of
+        // the group, half of the block) through the two LDS buffers.  A half
+        // block goes straight from global memory into LDS (global_load_lds,
+        // 16 bytes per lane: a wave's lanes fill 1 KB from the wave's LDS
+        // base): unit U = (ks * 4 + q) * 32 + col of the half (its 32
+        // columns of one (ks, q) are 512 contiguous bytes of the block).
+        const uint4 *gh = reinterpret_cast<const uint4 *>(kt_hi + (size_t)p * 2 * ngrp * CH);
+        const uint4 *gl = reinterpret_cast<const uint4 *>(kt_lo + (size_t)p * 2 * ngrp * CH);
+        const int nhh = 4 * ngrp;
+        auto issue = [&](int hh) {
+            const size_t base = (size_t)(hh >> 1) * (CH / 8) + 32 * (hh & 1);
+#pragma unroll
+            for (int u = 0; u < HC / 8 / NT; ++u) {
+                const int U = t + NT * u;
+                const size_t gi = base + (size_t)(U >> 5) * 64 + (U & 31);
+                __builtin_amdgcn_global_load_lds(
+                    gh + gi, (__attribute__((address_space(3))) void *)(reinterpret_cast<uint4 *>(b_hi[hh & 1]) + NT * u + 64 * w),
+                    16, 0, 0);
+                __builtin_amdgcn_global_load_lds(
+                    gl + gi, (__attribute__((address_space(3))) void *)(reinterpret_cast<uint4 *>(b_lo[hh & 1]) + NT * u + 64 * w),
+                    16, 0, 0);
+            }
+        };
+        __syncthreads(); // every wave is done with the previous item's buffers
+        issue(0);
+        for (int grp = 0; grp < ngrp; ++grp) {
+            floatx4 acc[TPW][8]; // [tile][term]
+#pragma unroll
+            for (int tt = 0; tt < TPW; ++tt)
+#pragma unroll
+                for (int c = 0; c < 8; ++c) acc[tt][c] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int s4 = 0; s4 < 4; ++s4) {
+                const int hh = 4 * grp + s4;
+                // half block hh has landed and every wave is done with hh - 1,
+                // whose buffer the next half block then fills
+                __builtin_amdgcn_s_waitcnt(0);
+                __syncthreads();
+                if (hh + 1 < nhh) issue(hh + 1);
+                const uint16_t *bhh = b_hi[hh & 1], *bll = b_lo[hh & 1];
+                // the two column tiles of the half: the B fragments of step
+                // ks + 1 are read while step ks multiplies, the two tiles'
+                // products interleaved (two independent accumulator chains);
+                // the scheduling barriers keep the compiler from hoisting
+                // further reads, which would not fit the registers beside
+                // the operands
+                bf16x8 bh[2][2], bl[2][2]; // [buffer][tile]
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+                    bh[0][c] = *reinterpret_cast<const bf16x8 *>(&bhh[(q * 32 + 16 * c + r) * 8]);
+                    bl[0][c] = *reinterpret_cast<const bf16x8 *>(&bll[(q * 32 + 16 * c + r) * 8]);
                 }
 #pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    const bf16x8 bh = *reinterpret_cast<const bf16x8 *>(&b_hi[ks][q][16 * c + r][0]);
-                    const bf16x8 bl = *reinterpret_cast<const bf16x8 *>(&b_lo[ks][q][16 * c + r][0]);
-                    acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[c], 0, 0, 0);
-                    acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc[c], 0, 0, 0);
-                    acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[c], 0, 0, 0);
+                for (int ks = 0; ks < KS; ++ks) {
+                    if (ks + 1 < KS) {
+#pragma unroll
+                        for (int c = 0; c < 2; ++c) {
+                            const int o = (((ks + 1) * 4 + q) * 32 + 16 * c + r) * 8;
+                            bh[(ks + 1) & 1][c] = *reinterpret_cast<const bf16x8 *>(&bhh[o]);
+                            bl[(ks + 1) & 1][c] = *reinterpret_cast<const bf16x8 *>(&bll[o]);
+                        }
+                    }
+#pragma unroll
+                    for (int tt = 0; tt < TPW; ++tt) {
+                        floatx4 &a0 = acc[tt][2 * s4], &a1 = acc[tt][2 * s4 + 1];
+                        a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[tt][ks], bh[ks & 1][0], a0, 0, 0, 0);
+                        a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[tt][ks], bh[ks & 1][1], a1, 0, 0, 0);
+                        a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[tt][ks], bl[ks & 1][0], a0, 0, 0, 0);
+                        a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[tt][ks], bl[ks & 1][1], a1, 0, 0, 0);
+                        a0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[tt][ks], bh[ks & 1][0], a0, 0, 0, 0);
+                        a1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[tt][ks], bh[ks & 1][1], a1, 0, 0, 0);
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
                 }
             }
+            // D layout: lane (r, q) holds rows 4 q + j, column 16 c + r of
+            // each half block: frequency 16 grp + r, term 2 s4 + c
+            const int f = 16 * grp + r;
+            const int cs = f < nfreq ? col_shift[(size_t)p * nfreq + f] : NONE;
+#pragma unroll
+            for (int tt = 0; tt < TPW; ++tt)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    // per-tracer values of output row 4 q + j (written in
+                    // phase 1 by this wave; the barriers since order them)
+                    const int loc = 16 * (TPW * w + tt) + 4 * q + j;
+                    const double ra = s_a[loc];
+                    const int rit = s_it[loc], rsr = s_sr[loc];
+                    const double4 ta = *reinterpret_cast<const double4 *>(&s_t[loc][0]);
+                    const double4 tb = *reinterpret_cast<const double4 *>(&s_t[loc][4]);
+                    double v = ta.x * (double)acc[tt][0][j];
+                    v = fma(ta.y, (double)acc[tt][1][j], v);
+                    v = fma(ta.z, (double)acc[tt][2][j], v);
+                    v = fma(ta.w, (double)acc[tt][3][j], v);
+                    v = fma(tb.x, (double)acc[tt][4][j], v);
+                    v = fma(tb.y, (double)acc[tt][5][j], v);
+                    v = fma(tb.z, (double)acc[tt][6][j], v);
+                    v = fma(tb.w, (double)acc[tt][7][j], v);
+                    const int shr = rsr == NONE ? 0 : rsr - EM_SY_OFF;
+                    const bool ok = f < nfreq && cs > -2000;
+                    const double ev = ok ? ra * ldexp(v, shr + cs - EM_SY_OFF) : 0.0;
+                    if (f < nfreq && rit >= 0) eps[(size_t)rit * nfreq + f] = ev;
+                    const double lim = (ok && rsr != NONE) ? ldexp(fabs(ra), rsr + cs - 157) : 0.0;
+                    rmx[tt][j] = fmax(rmx[tt][j], fabs(ev));
+                    if (lim > 0.0 && fabs(ev) < lim) rlm[tt][j] = fmax(rlm[tt][j], lim);
+                }
         }
-        const int sh = s_row == NONE ? 0 : s_row - EM_SY_OFF;
-        // D layout: lane holds rows 4 (lane / 16) + j, column lane % 16; a
-        // column is (frequency cb * 8 + (16 c + r) / 8, Chebyshev term r % 8)
-        const int mt = r & 7;
+        // 3. range check of the wave's rows: the largest emissivity and the
+        // largest violated bound over the row's 16 lanes
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int rr = 4 * q + j;
-            const int shr = __shfl(sh, rr, 64);
-            const bool lv = tile * 16 + rr < cnt;
-            const int itr = lv ? perm[start + tile * 16 + rr] : 0;
-            const double sr = lv ? sv[itr] : 0.0;
-            // T_mt(s) by the recurrence
-            double t0 = 1.0, t1 = sr, tm = (mt == 0) ? 1.0 : sr;
-            for (int m = 2; m <= mt; ++m) {
-                tm = 2.0 * sr * t1 - t0;
-                t0 = t1;
-                t1 = tm;
+        for (int tt = 0; tt < TPW; ++tt)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                double mx = rmx[tt][j], lm = rlm[tt][j];
+                mx = fmax(mx, em_dpp64<EM_XOR1>(mx));
+                mx = fmax(mx, em_dpp64<EM_XOR2>(mx));
+                mx = fmax(mx, em_dpp64<EM_HMIRROR>(mx));
+                mx = fmax(mx, em_dpp64<EM_MIRROR>(mx));
+                lm = fmax(lm, em_dpp64<EM_XOR1>(lm));
+                lm = fmax(lm, em_dpp64<EM_XOR2>(lm));
+                lm = fmax(lm, em_dpp64<EM_HMIRROR>(lm));
+                lm = fmax(lm, em_dpp64<EM_MIRROR>(lm));
+                const int rit = s_it[16 * (TPW * w + tt) + 4 * q + j];
+                if (r == 0 && rit >= 0 && lm > 0.0 && lm >= 1.0e-25 * mx) fix[1 + atomicAdd(fix, 1)] = rit;
             }
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                const int col = cb * EM_TB + 16 * c + r;
-                double v = 0.0;
-                if (col < ncol) v = ldexp((double)acc[c][j], shr + col_shift[pbase + col] - EM_SY_OFF) * tm;
-                v += __shfl_xor(v, 1, 64);
-                v += __shfl_xor(v, 2, 64);
-                v += __shfl_xor(v, 4, 64);
-                const int f = col / EM_SY_M;
-                if (lv && mt == 0 && f < nfreq) eps[(size_t)itr * nfreq + f] = pref[itr] * v;
-            }
-        }
     }
-}
-
-// Range check of the MFMA contraction.  An operand entry below 2^-126 after
-// scaling (or a product below it) is lost, so each column sum of tracer i
-// misses at most 256 x 2^-67 x 2^(srow_i + cs - 118) of its true value (cs:
-// the column's binary exponent), and eps_f at most
-// |A| 8 2^(srow_i + max_m cs(f, m) - 177).  Where that bound exceeds 2^-17
-// (1e-5) of an emissivity within 25 decades of the tracer's largest one (or
-// the row has content and every emissivity came out zero), the tracer is
-// listed for the FP32 log-domain kernel, which has no range limit.
-__global__ __launch_bounds__(256) void k_synch_check(int ntr, int nfreq, int nlogb, double logb_min,
-                                                     double inv_dlogb, const double *__restrict__ b_field,
-                                                     const int *__restrict__ srow, const double *__restrict__ pref,
-                                                     const int *__restrict__ col_shift,
-                                                     const double *__restrict__ eps, int *__restrict__ fix)
-{
-    const int lane = threadIdx.x & 63;
-    const int it = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (it >= ntr) return;
-    const int sr = srow[it];
-    if (sr == EM_SY_NONE) return;
-    const int pl = em_synch_plane(b_field[it], logb_min, inv_dlogb, nlogb);
-    const double *e = eps + (size_t)it * nfreq;
-    const double ap = fabs(pref[it]);
-    double mx = 0.0;
-    for (int f = lane; f < nfreq; f += 64) mx = fmax(mx, fabs(e[f]));
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
-    bool bad = false;
-    for (int f = lane; f < nfreq; f += 64) {
-        const int *cs = col_shift + ((size_t)pl * nfreq + f) * EM_SY_M;
-        int cm = EM_SY_NONE;
-#pragma unroll
-        for (int m = 0; m < EM_SY_M; ++m)
-            if (cs[m] > -2000) cm = max(cm, cs[m]);
-        if (cm == EM_SY_NONE) continue;
-        const double lim = ldexp(ap, sr + cm - 157);
-        if (lim > 0.0 && lim >= 1.0e-25 * mx && fabs(e[f]) < lim) bad = true;
-    }
-    if (__ballot(bad) && lane == 0) fix[1 + atomicAdd(fix, 1)] = it;
 }
 
 extern "C" int ck_synch_mfma_prepare(int n, int nfreq, int nlogb, const double *table, const double *dp,
@@ -1214,9 +1385,14 @@ extern "C" int ck_synch_mfma_prepare(int n, int nfreq, int nlogb, const double *
     if (!(n == 64 || n == 128 || n == 256) || nfreq <= 0 || nlogb < 2) return -1;
     hipStream_t st = (hipStream_t)stream;
     const size_t tot = (size_t)(nlogb - 1) * nfreq * n;
-    const size_t ncs = (size_t)(nlogb - 1) * nfreq * EM_SY_M;
+    const size_t ncs = (size_t)(nlogb - 1) * nfreq;
     const size_t ttot = (size_t)nlogb * nfreq * n;
-    if (ck_fill_i32(col_shift, -100000, ncs, stream)) return -1;
+    // kt: (nlogb - 1) x 2 ceil(nfreq / 16) column blocks of 64 x n bf16; the
+    // frequencies past nfreq stay zero
+    const size_t nkt = (size_t)(nlogb - 1) * 2 * ((nfreq + 15) / 16) * EM_TB * n;
+    if (ck_fill_i32(col_shift, -100000, ncs, stream) || ck_fill_i32((int *)kt_hi, 0, nkt / 2, stream) ||
+        ck_fill_i32((int *)kt_lo, 0, nkt / 2, stream))
+        return -1;
     hipLaunchKernelGGL(k_synch_cheb, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, 0, n, nfreq, nlogb, table,
                        dp, kt_hi, kt_lo, col_shift);
     hipLaunchKernelGGL(k_synch_cheb, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, 1, n, nfreq, nlogb, table,
@@ -1229,36 +1405,45 @@ extern "C" int ck_synch_mfma_prepare(int n, int nfreq, int nlogb, const double *
 
 extern "C" int ck_synch_mfma_run(int ntr, int n, int nfreq, int nlogb, double logb_min, double inv_dlogb,
                                  const double *table, const uint16_t *kt_hi, const uint16_t *kt_lo,
-                                 const int *col_shift, const float *lt, int *work, int *jend, double *sv,
-                                 double *pref, int *srow, int *fix, const double *cre, const double *dp,
-                                 const double *b_field, double *eps, void *stream)
+                                 const int *col_shift, const float *lt, int *work, int *fix, const double *cre,
+                                 const double *dp, const double *b_field, double *eps, void *stream)
 {
     if (ntr <= 0 || !(n == 64 || n == 128 || n == 256) || nfreq <= 0 || nlogb < 2) return -1;
     hipStream_t st = (hipStream_t)stream;
-    const dim3 g1((ntr + 255) / 256), gw((unsigned)((ntr + 3) / 4));
+    static int ncu = 0;
+    if (!ncu) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0)
+            ncu = 256;
+    }
+    const dim3 g1((ntr + 255) / 256);
     int *key = work, *hist = work + ntr, *perm = work + ntr + nlogb;
+    int *ioff = perm + ntr, *itot = ioff + nlogb;
     if (ck_fill_i32(hist, 0, (size_t)nlogb, stream) || ck_fill_i32(fix, 0, 1, stream)) return -1;
     hipLaunchKernelGGL(k_synch_plane_count, g1, dim3(256), 0, st, ntr, b_field, logb_min, inv_dlogb, nlogb, key, hist);
     hipLaunchKernelGGL(k_synch_plane_scan, dim3(1), dim3(256), 0, st, nlogb, hist);
     hipLaunchKernelGGL(k_synch_plane_place, g1, dim3(256), 0, st, ntr, nlogb, key, hist, perm);
-    hipLaunchKernelGGL(k_synch_prep, gw, dim3(256), 0, st, ntr, n, nlogb, logb_min, inv_dlogb, cre, b_field, jend,
-                       sv, pref, srow);
     // after the placement hist[p] is the end of plane interval p in perm
-    const int ncb = (nfreq * EM_SY_M + EM_TB - 1) / EM_TB;
-    const int npg = (nlogb - 1 + 7) / 8;
-    const dim3 gm((unsigned)(npg * ncb * 8));
+    const int nbx = ncu / 8 > 0 ? ncu / 8 : 1; // CUs per XCD
+    // persistent: two blocks of four waves per CU (the same number per XCD),
+    // one 16-tracer tile per wave (the accumulators of all 8 terms of 16
+    // frequencies for two tiles would not fit the registers beside their
+    // operands)
+    hipLaunchKernelGGL(k_synch_items, dim3(1), dim3(512), 0, st, nlogb, 64, hist, ioff, itot);
 #define EM_SY(KS_)                                                                                                     \
-    hipLaunchKernelGGL((k_synch_mfma<KS_>), gm, dim3(256), 0, st, nfreq, nlogb, ncb, cre, perm, hist, jend, sv, pref,  \
-                       kt_hi, kt_lo, col_shift, eps)
+    hipLaunchKernelGGL((k_synch_fused<KS_, 1, 4>), dim3((unsigned)(16 * nbx)), dim3(256), 0, st, nfreq, nlogb,         \
+                       logb_min, inv_dlogb, cre, b_field, perm, hist, ioff, itot, kt_hi, kt_lo, col_shift, eps, fix)
     if (n == 64) EM_SY(2);
     else if (n == 128) EM_SY(4);
     else EM_SY(8);
 #undef EM_SY
     // tracers beyond the contraction's range: the FP32 log-domain kernel
-    hipLaunchKernelGGL(k_synch_check, gw, dim3(256), 0, st, ntr, nfreq, nlogb, logb_min, inv_dlogb, b_field, srow,
-                       pref, col_shift, eps, fix);
-    hipLaunchKernelGGL(k_emis_synch32, gw, dim3(256), 0, st, ntr, n, nfreq, nlogb, logb_min, inv_dlogb, table, lt,
-                       (const int *)(fix + 1), cre, dp, b_field, eps, (const int *)fix);
+    // over the list (waves loop over the listed tracers)
+    int gx = (ntr + 3) / 4;
+    if (gx > 4 * ncu) gx = 4 * ncu;
+    hipLaunchKernelGGL(k_emis_synch32<true>, dim3((unsigned)gx), dim3(256), 0, st, ntr, n, nfreq, nlogb, logb_min,
+                       inv_dlogb, table, lt, (const int *)(fix + 1), cre, dp, b_field, eps, (const int *)fix);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -1288,7 +1473,7 @@ extern "C" int ck_emission_synch(int ntr, int n, int nfreq, int nlogb, double lo
         float *lt = (float *)logt;
         hipLaunchKernelGGL(k_synch_log2t32, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, nlogb, nfreq, n,
                            table, lt);
-        hipLaunchKernelGGL(k_emis_synch32, grid, dim3(256), 0, st, ntr, n, nfreq, nlogb, logb_min, inv_dlogb, table,
+        hipLaunchKernelGGL(k_emis_synch32<false>, grid, dim3(256), 0, st, ntr, n, nfreq, nlogb, logb_min, inv_dlogb, table,
                            (const float *)lt, (const int *)perm, cre, dp, b_field, eps, nullptr);
         return hipGetLastError() == hipSuccess ? 0 : -1;
     }

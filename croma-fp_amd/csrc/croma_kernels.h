@@ -245,20 +245,19 @@ int ck_synch_table(int n, int nfreq, int ntheta, int nlogb, double logb_min, dou
                    double xmin, const double *logx_tab, const double *logfx_tab, const double *fx_tab,
                    const double *logy, const double *pitch_weight, double *table, void *stream);
 /* Synchrotron on MFMA.  Prepare (once per table): Chebyshev coefficients of
- * the log-B interpolation per plane interval (kt_hi / kt_lo [(nlogb - 1) *
- * nfreq * 8][n] bf16, col_shift [(nlogb - 1) * nfreq * 8]) and the float log2
- * table lt [nlogb * nfreq * n] of the FP32 fix-up.  Run (per call): tracers
- * grouped by plane (work: 2 ntr + nlogb ints), per-tracer end bin / s /
- * prefactor / row exponent (jend, sv, pref, srow [ntr]), the split-BF16 x3
- * contraction, then the tracers whose emissivities fall below its range
- * (fix: count + list, ntr + 1 ints) again by the FP32 log-domain kernel.
- * n in {64, 128, 256}. */
+ * the log-B interpolation per plane interval (kt_hi / kt_lo: (nlogb - 1) x
+ * 2 ceil(nfreq / 16) column blocks of 64 columns x n bins, bf16, in the
+ * contraction's LDS order; col_shift [(nlogb - 1) * nfreq]) and the float
+ * log2 table lt [nlogb * nfreq * n] of the FP32 fix-up.  Run (per call):
+ * tracers grouped by plane (work: 2 ntr + 2 nlogb + 8 ints), the fused
+ * split-BF16 x3 contraction with its range check, then the tracers whose
+ * emissivities fall below its range (fix: count + list, ntr + 1 ints) again
+ * by the FP32 log-domain kernel.  n in {64, 128, 256}. */
 int ck_synch_mfma_prepare(int n, int nfreq, int nlogb, const double *table, const double *dp, uint16_t *kt_hi,
                           uint16_t *kt_lo, int *col_shift, float *lt, void *stream);
 int ck_synch_mfma_run(int ntr, int n, int nfreq, int nlogb, double logb_min, double inv_dlogb, const double *table,
                       const uint16_t *kt_hi, const uint16_t *kt_lo, const int *col_shift, const float *lt, int *work,
-                      int *jend, double *sv, double *pref, int *srow, int *fix, const double *cre, const double *dp,
-                      const double *b_field, double *eps, void *stream);
+                      int *fix, const double *cre, const double *dp, const double *b_field, double *eps, void *stream);
 /* Grid-constant tables from the shared physics header (croma_tables.hip):
  * secondary-electron weights fqe_t[jp][je] and np_min[je] from the electron
  * energies, proton momenta, beta_p and proton bin widths; gamma (kind 0),

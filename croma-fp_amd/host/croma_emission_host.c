@@ -255,8 +255,8 @@ struct CromaHipSynchPlan {
     float *d_lt;
     char *d_run;
     long cap;
-    double *d_cre, *d_b, *d_eps, *d_s, *d_pref;
-    int *d_work, *d_jend, *d_srow, *d_fix;
+    double *d_cre, *d_b, *d_eps;
+    int *d_work, *d_fix;
     void *stream, *e0, *e1;
 };
 
@@ -300,8 +300,8 @@ CromaHipSynchPlan *croma_hip_synch_plan_create(const CromaHipContext *ctx, int n
     p->logb_min = logb_min;
     p->inv_dlogb = inv_dlogb;
     ntab = (size_t)nlogb * nfreq * p->n;
-    nkt = (size_t)(nlogb - 1) * nfreq * 8 * p->n;
-    ncs = (size_t)(nlogb - 1) * nfreq * 8;
+    nkt = (size_t)(nlogb - 1) * 2 * ((nfreq + 15) / 16) * 64 * p->n;
+    ncs = (size_t)(nlogb - 1) * nfreq;
     p->d_const = (char *)ck_malloc(SY_R(ntab * 8) + SY_R((size_t)p->n * 8) + 2 * SY_R(nkt * 2) + SY_R(ncs * 4) +
                                    SY_R(ntab * 4));
     p->stream = ck_stream_create();
@@ -348,8 +348,7 @@ int croma_hip_synch_plan_run(CromaHipSynchPlan *p, int ntr, const double *cre, c
         ck_free(p->d_run);
         p->cap = 0;
         p->d_run = (char *)ck_malloc(SY_R(nt * n * 8) + SY_R(nt * 8) + SY_R(nt * p->nfreq * 8) +
-                                     SY_R((2 * nt + p->nlogb) * 4) + SY_R(nt * 4) + 2 * SY_R(nt * 8) + SY_R(nt * 4) +
-                                     SY_R((nt + 1) * 4));
+                                     SY_R((2 * nt + 2 * (size_t)p->nlogb + 8) * 4) + SY_R((nt + 1) * 4));
         if (!p->d_run) {
             ch_set_error("synch plan: device allocation: %s", ck_last_error());
             return CROMA_HIP_EDEVICE;
@@ -362,23 +361,14 @@ int croma_hip_synch_plan_run(CromaHipSynchPlan *p, int ntr, const double *cre, c
         p->d_eps = (double *)q;
         q += SY_R(nt * p->nfreq * 8);
         p->d_work = (int *)q;
-        q += SY_R((2 * nt + p->nlogb) * 4);
-        p->d_jend = (int *)q;
-        q += SY_R(nt * 4);
-        p->d_s = (double *)q;
-        q += SY_R(nt * 8);
-        p->d_pref = (double *)q;
-        q += SY_R(nt * 8);
-        p->d_srow = (int *)q;
-        q += SY_R(nt * 4);
+        q += SY_R((2 * nt + 2 * (size_t)p->nlogb + 8) * 4);
         p->d_fix = (int *)q;
         p->cap = ntr;
     }
     if (ck_h2d(p->d_cre, cre, (size_t)ntr * n * 8, p->stream) ||
         ck_h2d(p->d_b, b_field, (size_t)ntr * 8, p->stream) || ck_event_record(p->e0, p->stream) ||
         ck_synch_mfma_run(ntr, n, p->nfreq, p->nlogb, p->logb_min, p->inv_dlogb, p->d_tab, p->d_hi, p->d_lo, p->d_cs,
-                          p->d_lt, p->d_work, p->d_jend, p->d_s, p->d_pref, p->d_srow, p->d_fix, p->d_cre, p->d_dp,
-                          p->d_b, p->d_eps, p->stream) ||
+                          p->d_lt, p->d_work, p->d_fix, p->d_cre, p->d_dp, p->d_b, p->d_eps, p->stream) ||
         ck_event_record(p->e1, p->stream) || ck_d2h(eps, p->d_eps, (size_t)ntr * p->nfreq * 8, p->stream) ||
         ck_stream_sync(p->stream)) {
         ch_set_error("synch MFMA: %s", ck_last_error());
