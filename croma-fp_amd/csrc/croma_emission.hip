@@ -268,13 +268,14 @@ __global__ __launch_bounds__(256) void k_emis_stream(int ntr, int nbins, int ld_
                 bf16x8 ah, al;
 #pragma unroll
                 for (int m = 0; m < 4; ++m) {
+                    // (hardware round-to-nearest-even float -> bf16)
                     const float f0 = (float)ldexp(a[kk][m].x, -sh), f1 = (float)ldexp(a[kk][m].y, -sh);
-                    const uint16_t h0 = em_bf16_bits(f0), h1 = em_bf16_bits(f1);
-                    ah[2 * m] = __builtin_bit_cast(__bf16, h0);
-                    ah[2 * m + 1] = __builtin_bit_cast(__bf16, h1);
+                    const __bf16 h0 = (__bf16)f0, h1 = (__bf16)f1;
+                    ah[2 * m] = h0;
+                    ah[2 * m + 1] = h1;
                     if (SPLIT) {
-                        al[2 * m] = __builtin_bit_cast(__bf16, em_bf16_bits(f0 - em_bf16_val(h0)));
-                        al[2 * m + 1] = __builtin_bit_cast(__bf16, em_bf16_bits(f1 - em_bf16_val(h1)));
+                        al[2 * m] = (__bf16)(f0 - (float)h0);
+                        al[2 * m + 1] = (__bf16)(f1 - (float)h1);
                     }
                 }
 #pragma unroll
@@ -1121,44 +1122,28 @@ __global__ __launch_bounds__(64 * NW, 2) void k_synch_fused(int nfreq, int nlogb
             // bounds) and masks the whole row
             const bool live = t0 + loc < pend;
             const int it = perm[min(t0 + loc, pend - 1)];
-            // lane (r, q) holds bins 32 ks + 8 m + 2 q (+1) of row r.  Three
-            // reads of the row (the later ones from the cache), so the raw
-            // row never has to sit in registers whole:
-            // (a) the end of the sum and each 32-bin block's largest exponent
+            // lane (r, q) holds bins 32 ks + 8 m + 2 q (+1) of row r, all read
+            // at once (one memory round trip per tile)
             const double *row = cre + (size_t)it * K + 2 * q;
-            // (loads in groups of SG blocks: the scheduling barriers keep the
-            // compiler from hoisting every load of the row at once)
-            constexpr int SG = KS < 4 ? KS : 4;
+            double2 v[KS][4];
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+                for (int m = 0; m < 4; ++m) v[ks][m] = *reinterpret_cast<const double2 *>(row + 32 * ks + 8 * m);
             // bins as compile-time offsets o from the lane's first bin 2 q
             // (k = o + 2 q), so no per-bin index occupies a register
             const int q2 = 2 * q;
-            int kfo = K, eblk[KS];
+            // the end of the sum: the first non-positive bin from 3 on
+            // (offsets 0, 1 are bins 2 q, 2 q + 1, from 3 on only for q >= 1)
+            int kfo = K;
 #pragma unroll
-            for (int g0 = 0; g0 < KS; g0 += SG) {
-                double2 v[SG][4];
+            for (int ks = 0; ks < KS; ++ks)
 #pragma unroll
-                for (int g = 0; g < SG; ++g)
-#pragma unroll
-                    for (int m = 0; m < 4; ++m)
-                        v[g][m] = *reinterpret_cast<const double2 *>(row + 32 * (g0 + g) + 8 * m);
-#pragma unroll
-                for (int g = 0; g < SG; ++g) {
-                    int e = NONE;
-#pragma unroll
-                    for (int m = 0; m < 4; ++m) {
-                        const int o = 32 * (g0 + g) + 8 * m;
-                        // the first non-positive bin from 3 on (offsets 0, 1
-                        // are bins 2 q, 2 q + 1, from 3 on only for q >= 1)
-                        if ((o >= 3 || o + q2 >= 3) && !(v[g][m].x > 0.0)) kfo = min(kfo, o);
-                        if ((o + 1 >= 3 || o + 1 + q2 >= 3) && !(v[g][m].y > 0.0)) kfo = min(kfo, o + 1);
-                        const double v0 = fabs(v[g][m].x), v1 = fabs(v[g][m].y);
-                        if (v0 > 0.0 && isfinite(v0)) e = max(e, __builtin_amdgcn_frexp_exp(v0));
-                        if (v1 > 0.0 && isfinite(v1)) e = max(e, __builtin_amdgcn_frexp_exp(v1));
-                    }
-                    eblk[g0 + g] = e;
+                for (int m = 0; m < 4; ++m) {
+                    const int o = 32 * ks + 8 * m;
+                    if ((o >= 3 || o + q2 >= 3) && !(v[ks][m].x > 0.0)) kfo = min(kfo, o);
+                    if ((o + 1 >= 3 || o + 1 + q2 >= 3) && !(v[ks][m].y > 0.0)) kfo = min(kfo, o + 1);
                 }
-                __builtin_amdgcn_sched_barrier(0);
-            }
             int kf = kfo + q2;
             if (!live) kf = 0;
             kf = min(kf, __shfl_xor(kf, 16, 64));
@@ -1166,41 +1151,36 @@ __global__ __launch_bounds__(64 * NW, 2) void k_synch_fused(int nfreq, int nlogb
             const int jend = kf - 1;
             // a bin of the lane enters the sum if its offset is below jr
             const int jr = jend - q2;
-            // (b) the largest exponent of the bins before the end: whole
-            // blocks before block jend / 32, that block bin by bin
-            const int kb = max(jend, 0) >> 5;
+            // the largest exponent of the bins before the end
             int e = NONE;
 #pragma unroll
             for (int ks = 0; ks < KS; ++ks)
-                if (ks < kb) e = max(e, eblk[ks]);
-#pragma unroll
-            for (int m = 0; m < 4; ++m) {
-                const int ob = 32 * kb + 8 * m;
-                const double2 v = *reinterpret_cast<const double2 *>(row + ob);
-                const double v0 = fabs(v.x), v1 = fabs(v.y);
-                if (ob < jr && v0 > 0.0 && isfinite(v0)) e = max(e, __builtin_amdgcn_frexp_exp(v0));
-                if (ob + 1 < jr && v1 > 0.0 && isfinite(v1)) e = max(e, __builtin_amdgcn_frexp_exp(v1));
-            }
-            e = max(e, __shfl_xor(e, 16, 64));
-            e = max(e, __shfl_xor(e, 32, 64));
-            // (c) the split BF16 operands, masked past the end
-            const int sh = e == NONE ? 0 : e - EM_SY_OFF;
-            __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-            for (int ks = 0; ks < KS; ++ks) {
-                if (ks % SG == 0 && ks > 0) __builtin_amdgcn_sched_barrier(0);
-                bf16x8 h8, l8;
 #pragma unroll
                 for (int m = 0; m < 4; ++m) {
                     const int o = 32 * ks + 8 * m;
-                    const double2 v = *reinterpret_cast<const double2 *>(row + o);
-                    const float f0 = o < jr ? (float)ldexp(v.x, -sh) : 0.0f;
-                    const float f1 = o + 1 < jr ? (float)ldexp(v.y, -sh) : 0.0f;
-                    const uint16_t h0 = em_bf16_bits(f0), h1 = em_bf16_bits(f1);
-                    h8[2 * m] = __builtin_bit_cast(__bf16, h0);
-                    h8[2 * m + 1] = __builtin_bit_cast(__bf16, h1);
-                    l8[2 * m] = __builtin_bit_cast(__bf16, em_bf16_bits(f0 - em_bf16_val(h0)));
-                    l8[2 * m + 1] = __builtin_bit_cast(__bf16, em_bf16_bits(f1 - em_bf16_val(h1)));
+                    const double v0 = fabs(v[ks][m].x), v1 = fabs(v[ks][m].y);
+                    if (o < jr && v0 > 0.0 && isfinite(v0)) e = max(e, __builtin_amdgcn_frexp_exp(v0));
+                    if (o + 1 < jr && v1 > 0.0 && isfinite(v1)) e = max(e, __builtin_amdgcn_frexp_exp(v1));
+                }
+            e = max(e, __shfl_xor(e, 16, 64));
+            e = max(e, __shfl_xor(e, 32, 64));
+            // the split BF16 operands, masked past the end
+            const int sh = e == NONE ? 0 : e - EM_SY_OFF;
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) {
+                bf16x8 h8, l8;
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    // (the float -> bf16 conversions round to nearest even in
+                    // hardware, v_cvt_pk_bf16_f32)
+                    const int o = 32 * ks + 8 * m;
+                    const float f0 = o < jr ? (float)ldexp(v[ks][m].x, -sh) : 0.0f;
+                    const float f1 = o + 1 < jr ? (float)ldexp(v[ks][m].y, -sh) : 0.0f;
+                    const __bf16 h0 = (__bf16)f0, h1 = (__bf16)f1;
+                    h8[2 * m] = h0;
+                    h8[2 * m + 1] = h1;
+                    l8[2 * m] = (__bf16)(f0 - (float)h0);
+                    l8[2 * m + 1] = (__bf16)(f1 - (float)h1);
                 }
                 ah[tt][ks] = h8;
                 al[tt][ks] = l8;
