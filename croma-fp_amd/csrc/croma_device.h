@@ -617,7 +617,7 @@ __device__ __forceinline__ void pm_row_scan(int gl, int rl, double &m00, double 
 // entering each lane comes from a scan of 2x2 projective maps across the
 // group's lanes (renormalised by powers of two, so exact up to rounding of
 // the products); inside the lane the reference recurrence is applied.
-template <int R, int G>
+template <int R, int G, bool NOSRC = false>
 __device__ __forceinline__ void factorize(int gl, int n, const double (&sv)[R], Factor<R, G> &F, int &bad)
 {
     double m00 = 1.0, m01 = 0.0, m10 = 0.0, m11 = 1.0;
@@ -683,7 +683,7 @@ __device__ __forceinline__ void factorize(int gl, int n, const double (&sv)[R], 
         // the value a substep shifts in from the lane before (another
         // tracer's lanes, or 0) drops out without a select
         F.al[k] = (k == 0 && gl == 0) ? 0.0 : a * inv;
-        F.vs[k] = inv * sv[k];
+        if constexpr (!NOSRC) F.vs[k] = inv * sv[k];
         F.g[k] = -cp;
         A *= F.al[k];
         Bq *= F.g[k];
@@ -730,17 +730,18 @@ __device__ __forceinline__ void scan_bwd(const double *Qb, double &B)
 }
 
 // One implicit substep: x <- M^{-1} (x + src) for the group, in place.
-template <int R, int G>
+// NOSRC: a run without sources (src = 0; vs is not formed).
+template <int R, int G, bool NOSRC = false>
 __device__ __forceinline__ void apply_solve(int gl, const Factor<R, G> &F, double (&x)[R])
 {
     double t[R];
     // the lane-local prefixes start from the first row itself (al * 0 + t is
     // t exactly for the finite multipliers of a factorised run)
-    t[0] = fma(F.iv[0], x[0], F.vs[0]);
+    t[0] = NOSRC ? F.iv[0] * x[0] : fma(F.iv[0], x[0], F.vs[0]);
     double L = t[0];
 #pragma unroll
     for (int k = 1; k < R; ++k) {
-        t[k] = fma(F.iv[k], x[k], F.vs[k]);
+        t[k] = NOSRC ? F.iv[k] * x[k] : fma(F.iv[k], x[k], F.vs[k]);
         L = fma(F.al[k], L, t[k]);
     }
     scan_fwd<G, 0, Factor<R, G>::LV>(F.Af, L);
@@ -1004,18 +1005,13 @@ __device__ __forceinline__ T wave_up1_t(T v)
 // Builds the rows of the lane into F (al = a, iv = b, g = c) and factorises.
 // The rows' coefficients (B, C and the face inputs) are of type T; the
 // face weights, the rows a, b, c and the factorisation are FP64.
-template <bool ELEC, int R, int G, typename T>
+template <bool ELEC, int R, int G, typename T, bool NOSRC = false>
 __device__ __forceinline__ void build_factor(int gl, int n, const SpGridT<T> &sg, const RunCoef &rc,
                                              const T *radp, const SrcRows &src, Factor<R, G> &F, int &bad)
 {
 #pragma clang fp contract(off)
     const int base = gl * R;
     const int o = run_opaque(0);
-    // the run's sources first, so their loads overlap the row build (absent
-    // rows point at the grid's zero row: no branches)
-    double sv[R];
-#pragma unroll
-    for (int k = 0; k < R; ++k) sv[k] = src.scale * (src.inj[o + k] * src.ka * src.kb + src.q[o + k]);
     T B0, C0, BL, CL;
     row_BC<ELEC, T>(sg, rc, base, radp[o], B0, C0);
     row_BC<ELEC, T>(sg, rc, base + R - 1, radp[o + R - 1], BL, CL);
@@ -1051,7 +1047,13 @@ __device__ __forceinline__ void build_factor(int gl, int n, const SpGridT<T> &sg
         Bk = Br;
         Ck = Cr;
     }
-    factorize<R, G>(gl, n, sv, F, bad);
+    // the run's sources after the rows (absent rows point at the grid's zero
+    // row: no branches), so they hold no registers during the row build;
+    // their loads overlap the factorisation's pivot scan
+    double sv[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) sv[k] = NOSRC ? 0.0 : src.scale * (src.inj[o + k] * src.ka * src.kb + src.q[o + k]);
+    factorize<R, G, NOSRC>(gl, n, sv, F, bad);
 }
 
 // ---------------------------------------------------------------------------
@@ -1145,7 +1147,7 @@ __device__ __forceinline__ void unpark_rows(const double *own, double (&x)[R])
 // All runs of one species over one segment: up to three runs of constant
 // coefficients (before / inside / after the on-window, tracer_fp_cpu.c:150-157).
 // On entry the spectrum is parked (park_rows); on exit it is in x.
-template <bool ELEC, int N, int R, int G, typename T>
+template <bool ELEC, int N, int R, int G, typename T, bool NOSRC = false>
 __device__ __forceinline__ void species_segment(int gl, const SpGridT<T> &sg, int steps, bool windowed, int ls, int le,
                                                 RunCoef rc, double dfac_on, double dfac_off, bool mode_on,
                                                 const T *radp, const SrcRows &src, double *xpark,
@@ -1165,9 +1167,9 @@ __device__ __forceinline__ void species_segment(int gl, const SpGridT<T> &sg, in
         rc.dfac = on ? dfac_on : dfac_off;
         if (s0 > 0) park_rows<R>(xpark, x);
         Factor<R, G> F;
-        build_factor<ELEC, R, G, T>(gl, N, sg, rc, radp, src, F, bad);
+        build_factor<ELEC, R, G, T, NOSRC>(gl, N, sg, rc, radp, src, F, bad);
         unpark_rows<R>(xpark, x);
-        for (int s = s0; s < s1; ++s) apply_solve<R, G>(gl, F, x);
+        for (int s = s0; s < s1; ++s) apply_solve<R, G, NOSRC>(gl, F, x);
         s0 = s1;
     }
 }
@@ -1182,7 +1184,8 @@ enum { CK_MODE_E = 0,      // electron_only: protons are exactly zero
        CK_MODE_EP = 1,     // both species, secondaries contracted in the kernel
        CK_MODE_P_REC = 2,  // protons only; records the segment-start proton rows
        CK_MODE_E_SEC = 3,  // electrons only; secondaries from the contracted rows
-       CK_MODE_F32 = CK_LAUNCH_F32  // added to a mode: FP32 coefficient evaluation (losses, D_pp, B, C, face inputs)
+       CK_MODE_F32 = CK_LAUNCH_F32,  // added to a mode: FP32 coefficient evaluation (losses, D_pp, B, C, face inputs)
+       CK_MODE_E_NOSRC = CK_LAUNCH_NOSRC  // electron_only with no source rows in the launch (src = 0)
 };
 
 // LDS doubles per lane group of the fused kernel: loss rates (padded), the
@@ -1388,8 +1391,8 @@ __device__ __forceinline__ void snapshot_update(const CkGrid &g, const SpGridT<T
             park_rows<R>(own_x, xe);
             seg_radp_e<R, T>(sge, S, base, nseg, alpha, q, (T *)s_grp + gl * (R + 1));
             const SrcRows se = {S.qepri ? S.qepri + roff : g.zeros + base, inj, K_GYR, q.n_gas, dt};
-            species_segment<true, N, R, G, T>(gl, sge, steps, windowed, ls, le, rc, dfac_on, dfac_off, mode_on, own_rad,
-                                           se, own_x, xe, bad);
+            species_segment<true, N, R, G, T, MODE == CK_MODE_E_NOSRC>(gl, sge, steps, windowed, ls, le, rc, dfac_on,
+                                                                      dfac_off, mode_on, own_rad, se, own_x, xe, bad);
         }
     }
 }

@@ -33,6 +33,7 @@ static int ck_launch_r(int occ, int mode, int nblocks, const CkGrid &grid, const
             CK_MODE_CASE(1, O)                                                                                         \
             CK_MODE_CASE(2, O)                                                                                         \
             CK_MODE_CASE(3, O)                                                                                         \
+            CK_MODE_CASE(CK_LAUNCH_NOSRC, O)                                                                           \
         default: return -1;                                                                                            \
         }
     switch (occ) {

@@ -16,6 +16,9 @@ extern "C" {
 #define CK_MAX_N 512
 /* added to a fused launch mode: the FP32-coefficient instance */
 #define CK_LAUNCH_F32 4
+/* added to launch mode 0 (electron_only): no slot of the launch has source
+ * rows (qepri absent, no secondaries), so the instance forms no source terms */
+#define CK_LAUNCH_NOSRC 8
 #define CK_NBG 8 /* n_gas, kbt, b_field, divv, l_turb, dv, cs, beta_pl */
 
 enum { CK_BG_NGAS = 0, CK_BG_KBT, CK_BG_B, CK_BG_DIVV, CK_BG_LTURB, CK_BG_DV, CK_BG_CS, CK_BG_BETA };

@@ -96,6 +96,7 @@ extern "C" int ck_fused_lds_check(int n, int r, int occ, int mode, int tb, long 
 // float grid tables must fit the LDS beside the lane groups' regions.
 extern "C" int ck_fused_f32_supported(int n, int r, int occ, int mode)
 {
+    if (mode == CK_LAUNCH_NOSRC) mode = 0;
     if (!ck_fused_supported(n, r) || occ < 1 || occ > 2 || mode < 0 || mode > 3) return 0;
     return fused_use_lg(n, r, 4, mode, occ, 4) ? 1 : 0;
 }
@@ -105,7 +106,7 @@ extern "C" int ck_launch_fused(int n, int r, int occ, int nblocks, int mode, CkG
 {
     int rc = -1;
     if (!ck_fused_supported(n, r) || ntasks < 0 || nblocks <= 0) return -1;
-    if (occ < 1 || occ > 2 || mode < 0 || mode > 7) return -1;
+    if (occ < 1 || occ > 2 || mode < 0 || (mode > 7 && (mode & ~CK_LAUNCH_F32) != CK_LAUNCH_NOSRC)) return -1;
     if (ntasks == 0) return 0;
     switch (n) {
     case 64: rc = ck_launch_fused_n64(r, occ, nblocks, mode, grid, slots, tasks, ntasks, counter, err, stream); break;

@@ -837,6 +837,7 @@ typedef struct {
     void *ev_g, *ev_e;       /* contraction done, electron pass done (electron stream) */
     int dev_nsub;            /* nsub, n_on, order and segment rows are built on the device */
     int *d_key, *d_segloc, *d_blk, *d_hist; /* their work buffers */
+    int nosrc;               /* electron_only window whose snapshots have no source rows */
 } WinBuf;
 
 struct CromaHipBatch {
@@ -1154,6 +1155,7 @@ static int prepare_window(CromaHipBatch *b, WinBuf *w, int s_first, int count, i
     w->split = split;
     w->dev_nsub = dev_nsub;
     w->segrows = 0;
+    w->nosrc = !b->protons;
     /* the IC rows come from the run's table (run_ic_rows); the host only
      * evaluates the two end bins nsub needs (same arithmetic as the
      * reference row) */
@@ -1194,6 +1196,7 @@ static int prepare_window(CromaHipBatch *b, WinBuf *w, int s_first, int count, i
             S->qpi = (b->protons && b->has_qpi) ? b->d_qpi : 0;
             S->qepri = b->has_qepri ? b->d_qepri : 0;
         }
+        if (S->qepri) w->nosrc = 0;
         S->mass = b->has_mass ? b->d_mass : 0;
         S->dis = b->has_dis ? b->d_dis : 0;
         for (f = 0; f < CK_NBG; f++) {
@@ -1323,9 +1326,13 @@ static int launch_mode(CromaHipContext *c, int protons, int mode, const CkSlot *
     return 0;
 }
 
-static int launch_tasks(CromaHipContext *c, int protons, const CkSlot *d_slot, const CkTask *d_tasks, int ntasks)
+/* nosrc: an electron_only launch whose slots have no source rows (the
+ * instance without source terms; the same results) */
+static int launch_tasks(CromaHipContext *c, int protons, int nosrc, const CkSlot *d_slot, const CkTask *d_tasks,
+                        int ntasks)
 {
-    return launch_mode(c, protons, protons ? 1 : 0, d_slot, d_tasks, ntasks, 0, c->d_counter, c->stream);
+    return launch_mode(c, protons, protons ? 1 : (nosrc ? CK_LAUNCH_NOSRC : 0), d_slot, d_tasks, ntasks, 0,
+                       c->d_counter, c->stream);
 }
 
 /* queue counter of the electron pass of a pipelined split window (its own
@@ -1378,7 +1385,7 @@ static int issue_window(void *arg)
         return dev_fail("device nsub");
     if (wi->pipe) return launch_mode(c, 1, 2, w->d_slot, w->d_tasks, b->ntr, wi->shared, c->d_counter, c->stream);
     if (w->split) return launch_split(b, w);
-    return launch_tasks(c, b->protons, w->d_slot, w->d_tasks, b->ntr);
+    return launch_tasks(c, b->protons, w->nosrc, w->d_slot, w->d_tasks, b->ntr);
 }
 
 static int issue_gemm(void *arg)
@@ -1409,7 +1416,7 @@ static int launch_window(CromaHipBatch *b, WinBuf *w, WinBuf *prev, int last, Gr
     key[2] = t->rows_per_lane;
     key[3] = t->waves_per_simd;
     key[4] = (long)(size_t)b->d_segrec;
-    key[5] = b->protons | ((long)c->coeff_fp32 << 1);
+    key[5] = b->protons | ((long)c->coeff_fp32 << 1) | ((long)w->nosrc << 2);
     key[6] = c->jp0 | ((long)w->dev_nsub << 20) | ((long)w->count << 21);
     key[7] = b->ntr;
     wi.b = b;
@@ -1799,7 +1806,7 @@ typedef struct {
 static int issue_tasks(void *arg)
 {
     XIssue *x = (XIssue *)arg;
-    return launch_tasks(x->c, x->protons, x->d_slot, x->d_tasks, x->ntr);
+    return launch_tasks(x->c, x->protons, 0, x->d_slot, x->d_tasks, x->ntr);
 }
 
 /* ------------------------------------------------------------------ */

@@ -55,3 +55,22 @@ def test_spectra_match_oracle(ctxs, case, path):
         assert p_ok, f"proton spectra differ: max rel err {p_err:.3e}"
     else:
         assert np.all(crp == 0.0)
+
+
+def test_sourceless_instance_matches_zero_sources(ctxs):
+    """An electron_only window with no source rows runs the instance without
+    source terms; explicit all-zero source rows run the general one.  Both
+    give the same spectra (src = 0 either way), and both match the oracle."""
+    n, nt, ns = 512, 48, 2
+    w = small_workload(n, nt, ns, 1, 2, dsa_frac=0.0)
+    print(f"[parity] sourceless vs zero sources n={n} ntracer={nt} nsnap={ns}", flush=True)
+    ctx = _ctx(ctxs, n, hip_params(w))
+    _, cre_none, _ = run_hip(w, ctx)
+    w.qepri = {s: np.zeros(nt * n) for s in range(ns)}
+    _, cre_zero, _ = run_hip(w, ctx)
+    assert np.array_equal(cre_none, cre_zero)
+    w.qepri = {}
+    _, cre_ref = run_oracle.run_workload(w, run_oracle.oracle_params(w))
+    err, ok = rel_err(cre_none, cre_ref)
+    print(f"  max rel err vs oracle {err:.3e}", flush=True)
+    assert ok
