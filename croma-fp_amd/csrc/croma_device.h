@@ -459,12 +459,12 @@ template <int S>
 __device__ __forceinline__ double row_up(double v) { return dpp64<0x110 + S>(v); }
 template <int S>
 __device__ __forceinline__ double row_dn(double v) { return dpp64<0x100 + S>(v); }
-// odd rows (1, 3): the value at lane 15 of the row before (row_bcast:15);
-// even rows: 0
-__device__ __forceinline__ double bcast15_odd(double v)
+// the value at lane 15 of the row before (row_bcast:15; row 0: 0).  Rows 1
+// and 3 use it; rows 0 and 2 multiply it by a zero multiplier (it is finite).
+__device__ __forceinline__ double bcast15(double v)
 {
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x142, 0xA, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x142, 0xA, 0xF, false);
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x142, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x142, 0xF, 0xF, true);
     return __hiloint2double(hi, lo);
 }
 // lane l - 1 / l + 1 across the whole wave (wave_shr:1 / wave_shl:1)
@@ -693,7 +693,8 @@ __device__ __forceinline__ void factorize(int gl, int n, const double (&sv)[R], 
     // inside 16-lane rows (DPP); a group spanning several rows completes the
     // scan with per-row carries (see apply_solve).
     scan_coeffs<G, 0>(gl, rl, A, Bq, F);
-    F.PA = A;
+    // two-row groups: the carry multiplier is used in rows 1 and 3 only
+    F.PA = (G == 32 && !((threadIdx.x >> 4) & 1)) ? 0.0 : A;
     F.PQ = Bq;
     if constexpr (G == 32) {
         const int row = (int)((threadIdx.x & 63) >> 4);
@@ -747,8 +748,9 @@ __device__ __forceinline__ void apply_solve(int gl, const Factor<R, G> &F, doubl
     scan_fwd<G, 0, Factor<R, G>::LV>(F.Af, L);
     if constexpr (G == 32) {
         // rows 1 and 3 continue the groups begun in rows 0 and 2: they take
-        // the inclusive value at the last lane of the row before
-        L = fma(F.PA, bcast15_odd(L), L);
+        // the inclusive value at the last lane of the row before (PA is 0 in
+        // rows 0 and 2)
+        L = fma(F.PA, bcast15(L), L);
     } else if constexpr (G == 64) {
         // carries through the four rows of the group: the running value
         // entering row r is T_{r-1} + Tf_{r-1} C_{r-1} (T: inclusive in-row
@@ -1169,7 +1171,13 @@ __device__ __forceinline__ void species_segment(int gl, const SpGridT<T> &sg, in
         Factor<R, G> F;
         build_factor<ELEC, R, G, T, NOSRC>(gl, N, sg, rc, radp, src, F, bad);
         unpark_rows<R>(xpark, x);
-        for (int s = s0; s < s1; ++s) apply_solve<R, G, NOSRC>(gl, F, x);
+        // substeps two per loop trip (one loop branch per pair)
+        int s = s0;
+        for (; s + 1 < s1; s += 2) {
+            apply_solve<R, G, NOSRC>(gl, F, x);
+            apply_solve<R, G, NOSRC>(gl, F, x);
+        }
+        if (s < s1) apply_solve<R, G, NOSRC>(gl, F, x);
         s0 = s1;
     }
 }
