@@ -598,9 +598,16 @@ static int sv_launch_class(CromaHipServer *s, int ci)
                  ck_h2d(k->d_tasks, k->h_tasks, (size_t)ntasks * sizeof(CkTask), k->stream) ||
                  (split && ck_h2d(k->d_segoff, k->h_segoff, (size_t)nsegoff * sizeof(int), k->stream)) ||
                  ck_fill_i32(k->d_ctrl, 0, 1, k->stream);
-        if (!rc && !split)
-            rc = ck_launch_fused(n, tile.rows_per_lane, tile.waves_per_simd, nblocks, protons, *dg, k->d_slots,
+        if (!rc && !split) {
+            /* electron_only launches whose chunks carry no source rows run
+             * the instance without source terms (the same results) */
+            int mode = protons, nosrc = !protons;
+            for (i = 0; i < nl && nosrc; i++)
+                if (k->launched[i]->has_qepri) nosrc = 0;
+            if (nosrc) mode = CK_LAUNCH_NOSRC;
+            rc = ck_launch_fused(n, tile.rows_per_lane, tile.waves_per_simd, nblocks, mode, *dg, k->d_slots,
                                  k->d_tasks, ntasks, k->d_ctrl, k->d_ctrl + 1, k->stream);
+        }
         if (!rc && split) {
             rc = ck_launch_fused(n, tile.rows_per_lane, tile.waves_per_simd, nblocks, 2, *dg, k->d_slots, k->d_tasks,
                                  ntasks, k->d_ctrl, k->d_ctrl + 1, k->stream);
