@@ -53,11 +53,6 @@ __device__ __forceinline__ double em_dpp64(double v)
     const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
     return __hiloint2double(hi, lo);
 }
-template <int CTRL>
-__device__ __forceinline__ int em_dpp32(int v)
-{
-    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
-}
 
 // Largest binary exponent of row i of spectra (frexp convention), 0 for an
 // all-zero row.  4 threads per row.
