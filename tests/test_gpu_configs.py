@@ -189,3 +189,44 @@ def test_c4_stream_chunks_match_oracle():
             ok = ok and pok
         print(msg + f", latency {1e3 * st['latency_s']:.1f} ms", flush=True)
         assert ok
+
+
+@pytest.mark.parametrize("mode", ["drury83", "positive_delta"])
+def test_dsa_reacceleration_feeds_the_solve(mode):
+    """DSA injection with reacceleration of the pre-shock spectra (state
+    mode: the fresh spectra and the reacceleration deltas added to crp / cre;
+    Drury83 also sets each shocked tracer's disable-adiabatic flag, the
+    reference driver's dis input, include/tracer_fp_solve.inc:80-94) feeding
+    a two-species batch: HIP against the oracle on the same injected state
+    and flags, at the reference comparator's bar."""
+    n = 128
+    w = small_workload(n, 96, 2, 3, 1, seed=47)
+    rng = np.random.default_rng(48)
+    nt = w.ntracer
+    mach = np.where(rng.random(nt) < 0.5, 10 ** rng.uniform(np.log10(2.0), 1.0, nt), np.nan)
+    temp = 10 ** rng.uniform(6.0, 8.0, nt)
+    rho = 10 ** rng.uniform(-29.0, -26.0, nt)
+    if mode == "drury83":
+        prm = ch.dsa_default_params(reacc_mode=ch.DSA_REACC_DRURY83, reacc_eta_cap=1e-3)
+    else:
+        prm = ch.dsa_default_params(reacc_mode=ch.DSA_REACC_POSITIVE_DELTA)
+    crp, cre = w.crp0.copy(), w.cre0.copy()
+    # the state carries a CR population before the shock (the positive
+    # delta's budget grows with the proton pressure)
+    crp *= 1e3
+    dis = np.zeros(nt, np.uint8)
+    ninj = ch.dsa_inject(n, prm, ch.DSA_STATE, mach, rho, w.mass, temp1=temp, crp=crp, cre=cre, dis=dis)
+    print(f"[dsa reacc] mode={mode} injected {ninj} of {nt}, flags {int(dis.sum())}", flush=True)
+    assert ninj > 0
+    assert (int(dis.sum()) > 0) == (mode == "drury83")
+    w.crp0, w.cre0 = crp, cre
+    crp_ref, cre_ref = run_oracle.run_workload(w, run_oracle.oracle_params(w), dis=dis)
+    ctx = ch.Context(n, hip_params(w))
+    try:
+        crp_h, cre_h, _ = run_hip(w, ctx, dis=dis)
+    finally:
+        ctx.close()
+    err, ok = rel_err(cre_h, cre_ref)
+    perr, pok = rel_err(crp_h, crp_ref)
+    print(f"  electron max rel err {err:.3e}, proton {perr:.3e}", flush=True)
+    assert ok and pok
