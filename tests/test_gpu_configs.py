@@ -15,6 +15,8 @@
   tracers, mixed grids, species and D_pp, 10 snapshots each; every chunk
   checked on a tracer sample against the oracle, and the largest chunk
   bit-identical alone and in the mix.
+- DSA injection with reacceleration (Drury83 with its disable-adiabatic
+  flags, positive delta) feeding a two-species batch.
 """
 import os
 import time
