@@ -594,19 +594,22 @@ __device__ __forceinline__ int em_synch_plane(double bval, double logb_min, doub
 // Up to EM_LDS_HIST planes the per-tracer kernels count in a block-local LDS
 // histogram and add to the global one once per non-empty plane of the block
 // (one global atomic per tracer serialises on the planes many tracers share).
+// A block takes a chunk of up to EM_SORT_J x 256 consecutive tracers, so that
+// its flush of the histogram (one pass over the planes) serves many tracers.
 #define EM_LDS_HIST 2048
-__global__ __launch_bounds__(256) void k_synch_plane_count(int ntr, const double *__restrict__ b_field,
+#define EM_SORT_J 8
+__global__ __launch_bounds__(256) void k_synch_plane_count(int ntr, int chunk, const double *__restrict__ b_field,
                                                            double logb_min, double inv_dlogb, int nlogb,
                                                            int *__restrict__ key, int *__restrict__ hist)
 {
     __shared__ int s_hist[EM_LDS_HIST];
     const bool local = nlogb <= EM_LDS_HIST;
-    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int i0 = blockIdx.x * chunk, i1 = min(ntr, i0 + chunk);
     if (local) {
         for (int b = threadIdx.x; b < nlogb; b += 256) s_hist[b] = 0;
         __syncthreads();
     }
-    if (i < ntr) {
+    for (int i = i0 + (int)threadIdx.x; i < i1; i += 256) {
         const int ib = em_synch_plane(b_field[i], logb_min, inv_dlogb, nlogb);
         key[i] = ib;
         atomicAdd(local ? &s_hist[ib] : &hist[ib], 1);
@@ -618,6 +621,23 @@ __global__ __launch_bounds__(256) void k_synch_plane_count(int ntr, const double
             if (c) atomicAdd(&hist[b], c);
         }
     }
+}
+
+// Tracers per block of the two per-tracer sort kernels: about two blocks per
+// CU, at most EM_SORT_J x 256.
+static int em_sort_chunk(int ntr)
+{
+    static int ncu = 0;
+    if (!ncu) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0)
+            ncu = 256;
+    }
+    int j = (ntr + 512 * ncu - 1) / (512 * ncu);
+    if (j < 1) j = 1;
+    if (j > EM_SORT_J) j = EM_SORT_J;
+    return 256 * j;
 }
 
 __global__ __launch_bounds__(256) void k_synch_plane_scan(int nh, int *__restrict__ hist)
@@ -645,26 +665,38 @@ __global__ __launch_bounds__(256) void k_synch_plane_scan(int nh, int *__restric
     }
 }
 
-__global__ __launch_bounds__(256) void k_synch_plane_place(int ntr, int nlogb, const int *__restrict__ key,
+__global__ __launch_bounds__(256) void k_synch_plane_place(int ntr, int chunk, int nlogb, const int *__restrict__ key,
                                                            int *__restrict__ hist, int *__restrict__ perm)
 {
     __shared__ int s_hist[EM_LDS_HIST];
-    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int i0 = blockIdx.x * chunk, i1 = min(ntr, i0 + chunk);
     if (nlogb > EM_LDS_HIST) {
-        if (i < ntr) perm[atomicAdd(&hist[key[i]], 1)] = i;
+        for (int i = i0 + (int)threadIdx.x; i < i1; i += 256) perm[atomicAdd(&hist[key[i]], 1)] = i;
         return;
     }
     for (int b = threadIdx.x; b < nlogb; b += 256) s_hist[b] = 0;
     __syncthreads();
-    const int k = i < ntr ? key[i] : 0;
-    const int rank = i < ntr ? atomicAdd(&s_hist[k], 1) : 0;
+    // the thread's tracers i0 + threadIdx.x + 256 u: plane and rank in the
+    // block's part of the plane
+    int k[EM_SORT_J], rank[EM_SORT_J];
+#pragma unroll
+    for (int u = 0; u < EM_SORT_J; ++u) {
+        const int i = i0 + (int)threadIdx.x + 256 * u;
+        const bool in = 256 * u < chunk && i < i1;
+        k[u] = in ? key[i] : 0;
+        rank[u] = in ? atomicAdd(&s_hist[k[u]], 1) : 0;
+    }
     __syncthreads();
     for (int b = threadIdx.x; b < nlogb; b += 256) {
         const int c = s_hist[b];
         if (c) s_hist[b] = atomicAdd(&hist[b], c);
     }
     __syncthreads();
-    if (i < ntr) perm[s_hist[k] + rank] = i;
+#pragma unroll
+    for (int u = 0; u < EM_SORT_J; ++u) {
+        const int i = i0 + (int)threadIdx.x + 256 * u;
+        if (256 * u < chunk && i < i1) perm[s_hist[k[u]] + rank[u]] = i;
+    }
 }
 
 template <typename T>
@@ -1392,13 +1424,15 @@ extern "C" int ck_synch_mfma_run(int ntr, int n, int nfreq, int nlogb, double lo
             hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0)
             ncu = 256;
     }
-    const dim3 g1((ntr + 255) / 256);
+    const int chunk = em_sort_chunk(ntr);
+    const dim3 g1((ntr + chunk - 1) / chunk);
     int *key = work, *hist = work + ntr, *perm = work + ntr + nlogb;
     int *ioff = perm + ntr, *itot = ioff + nlogb;
     if (ck_fill_i32(hist, 0, (size_t)nlogb, stream) || ck_fill_i32(fix, 0, 1, stream)) return -1;
-    hipLaunchKernelGGL(k_synch_plane_count, g1, dim3(256), 0, st, ntr, b_field, logb_min, inv_dlogb, nlogb, key, hist);
+    hipLaunchKernelGGL(k_synch_plane_count, g1, dim3(256), 0, st, ntr, chunk, b_field, logb_min, inv_dlogb, nlogb, key,
+                       hist);
     hipLaunchKernelGGL(k_synch_plane_scan, dim3(1), dim3(256), 0, st, nlogb, hist);
-    hipLaunchKernelGGL(k_synch_plane_place, g1, dim3(256), 0, st, ntr, nlogb, key, hist, perm);
+    hipLaunchKernelGGL(k_synch_plane_place, g1, dim3(256), 0, st, ntr, chunk, nlogb, key, hist, perm);
     // after the placement hist[p] is the end of plane interval p in perm
     const int nbx = ncu / 8 > 0 ? ncu / 8 : 1; // CUs per XCD
     // persistent: two blocks of four waves per CU (the same number per XCD),
@@ -1428,16 +1462,17 @@ extern "C" int ck_emission_synch(int ntr, int n, int nfreq, int nlogb, double lo
 {
     if (ntr <= 0 || n <= 3 || n > 512 || nfreq <= 0 || nlogb < 2) return -1;
     hipStream_t st = (hipStream_t)stream;
-    const dim3 grid((ntr + 3) / 4), g1((ntr + 255) / 256);
+    const int chunk = em_sort_chunk(ntr);
+    const dim3 grid((ntr + 3) / 4), g1((ntr + chunk - 1) / chunk);
     int *perm = nullptr;
     if (work) {
         int *key = work, *hist = work + ntr;
         perm = work + ntr + nlogb;
         if (ck_fill_i32(hist, 0, (size_t)nlogb, stream)) return -1;
-        hipLaunchKernelGGL(k_synch_plane_count, g1, dim3(256), 0, st, ntr, b_field, logb_min, inv_dlogb, nlogb, key,
-                           hist);
+        hipLaunchKernelGGL(k_synch_plane_count, g1, dim3(256), 0, st, ntr, chunk, b_field, logb_min, inv_dlogb, nlogb,
+                           key, hist);
         hipLaunchKernelGGL(k_synch_plane_scan, dim3(1), dim3(256), 0, st, nlogb, hist);
-        hipLaunchKernelGGL(k_synch_plane_place, g1, dim3(256), 0, st, ntr, nlogb, key, hist, perm);
+        hipLaunchKernelGGL(k_synch_plane_place, g1, dim3(256), 0, st, ntr, chunk, nlogb, key, hist, perm);
     }
     const size_t tot = (size_t)nlogb * nfreq * n;
     if (fp64 && logt)
