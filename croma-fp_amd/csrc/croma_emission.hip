@@ -1116,6 +1116,7 @@ __global__ __launch_bounds__(64 * NW, 2) void k_synch_fused(int nfreq, int nlogb
     constexpr int NT = 64 * NW;            // threads
     constexpr int TPB = 16 * TPW * NW;     // tracers per item
     constexpr int HC = CH / 2; // bf16 per half of one half block (32 columns)
+    static_assert(HC / 8 >= NT && (HC / 8) % NT == 0, "every thread issues whole 16-byte loads of a half block");
     // two half-block buffers: one is filled while the other multiplies
     __shared__ __attribute__((aligned(16))) uint16_t b_hi[2][HC];
     __shared__ __attribute__((aligned(16))) uint16_t b_lo[2][HC];
