@@ -264,6 +264,47 @@ def test_synch_plan_reuse_matches_one_shot():
     assert np.array_equal(small, one[:300]) and np.array_equal(again, one[:300])
 
 
+def test_synch_large_batch_equals_its_subsets():
+    """300,000 tracers: the plane sort then gives each block of its count and
+    place kernels a chunk of several hundred tracers.  Every tracer's
+    emissivities depend on its own spectrum and field only, so the large
+    batch equals a 1,500-tracer subset run alone bit for bit, in the FP64
+    and the MFMA modes (each tracer listed once, none lost), and the FP64
+    subset equals the oracle."""
+    fx = np.load(os.path.join(GOLD, "fx_table.npz"))
+    n = 64
+    o = po.Oracle(n)
+    nfreq, ntheta, nlogb, logb_min, inv = 32, 8, 512, -3.0, 512 / 6.0
+    logy, pw = po.synch_logy(o, nfreq, 7.0, 11.0, ntheta)
+    rng = np.random.default_rng(53)
+    nt = 300_000
+    q = rng.uniform(2.0, 3.5, nt)
+    cre = (o.ep[None, :] ** -q[:, None]) * np.exp(-o.ep[None, :] / 10 ** rng.uniform(2, 6, nt)[:, None])
+    b = 10 ** rng.uniform(-8.0, -5.0, nt)
+    idx = np.sort(rng.choice(nt, 1500, replace=False))
+    ctx = ch.Context(n, ch.default_params())
+    try:
+        tab, _ = ctx.synch_kernel_table(nfreq, ntheta, nlogb, logb_min, inv, fx["logx"], fx["logfx"], fx["fx"],
+                                        float(fx["xmin"][0]), logy, pw)
+        big64, _ = ctx.emission_synch(cre, b, tab, nfreq, nlogb, logb_min, inv, mode=ch.EMIS_FP64)
+        sub64, _ = ctx.emission_synch(cre[idx], b[idx], tab, nfreq, nlogb, logb_min, inv, mode=ch.EMIS_FP64)
+        plan = ch.SynchPlan(ctx, tab, nfreq, nlogb, logb_min, inv)
+        try:
+            bigm, ms = plan.run(cre, b)
+            subm, _ = plan.run(cre[idx], b[idx])
+        finally:
+            plan.close()
+    finally:
+        ctx.close()
+    print(f"[emission] synch MFMA run on {nt} tracers x {n} bins: {ms:.3f} ms", flush=True)
+    assert np.array_equal(big64[idx], sub64)
+    assert np.array_equal(bigm[idx], subm)
+    assert np.all(np.isfinite(bigm))
+    ref = po.emission_synch(n, nfreq, nlogb, logb_min, inv, tab, cre[idx[:40]].ravel(), o.edp,
+                            b[idx[:40]]).reshape(40, nfreq)
+    assert rel(sub64[:40], ref) < 1e-12
+
+
 def test_synch_mfma_refuses_coarse_planes():
     """Planes coarser than 0.02 in log10 B leave the Chebyshev truncation
     unbounded: the MFMA mode refuses them (the FP32 VALU mode serves them)."""
