@@ -615,7 +615,11 @@ def main():
 
     if not args.no_serve:
         progress("served stream")
-        out["served"] = served_latency(ch, chunks=args.serve_chunks, seconds=args.serve_seconds)
+        try:
+            out["served"] = served_latency(ch, chunks=args.serve_chunks, seconds=args.serve_seconds)
+        except ch.CromaHipError as exc:
+            # a failed request is reported in the line, not in place of it
+            out["served"] = {"error": str(exc)}
 
     if not args.no_cpu:
         progress("cpu baseline")

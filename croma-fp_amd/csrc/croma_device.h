@@ -1449,8 +1449,14 @@ __device__ __forceinline__ void k_fused_body(const CkGrid &g, const CkSlot *__re
         tbase = __shfl(tbase, 0, 64);
         if (tbase >= ntasks) break;
         const int tid = tbase + grp;
-        if (tid < ntasks) {
-            const CkTask task = tasks[tid];
+        // groups past the end of the list repeat the wave's first task (the
+        // same data through the same instructions) and store nothing: the
+        // solve's cross-lane reads multiply a neighbouring group's values by
+        // zero, so every lane must hold finite values of a real tracer, not
+        // registers that were never written in this launch
+        const bool idle = tid >= ntasks;
+        {
+            const CkTask task = tasks[idle ? tbase : tid];
             const CkSlot &S0 = slots[task.slot];
             const int t = task.tracer;
             const size_t roff = (size_t)t * N + base;
@@ -1464,20 +1470,22 @@ __device__ __forceinline__ void k_fused_body(const CkGrid &g, const CkSlot *__re
             for (int sl = task.slot; sl < task.slot + task.nslot; ++sl) {
                 const CkSlot &S = slots[sl];
                 snapshot_update<N, R, MODE, PK, T>(g, sge, sgp, S, t, gl, s_grp, xe, xp, bad);
-                if (ELEC && S.out_e) {
+                if (ELEC && S.out_e && !idle) {
 #pragma unroll
                     for (int k = 0; k < R; ++k) S.out_e[roff + k] = xe[k];
                 }
-                if (PROT && S.out_p) {
+                if (PROT && S.out_p && !idle) {
 #pragma unroll
                     for (int k = 0; k < R; ++k) S.out_p[roff + k] = xp[k];
                 }
             }
+            if (!idle) {
 #pragma unroll
-            for (int k = 0; k < R; ++k) {
-                if (!isfinite(xe[k]) || !isfinite(xp[k])) bad |= ERR_NONFINITE;
-                if (ELEC) S0.cre[roff + k] = xe[k];
-                if (PROT) S0.crp[roff + k] = xp[k];
+                for (int k = 0; k < R; ++k) {
+                    if (!isfinite(xe[k]) || !isfinite(xp[k])) bad |= ERR_NONFINITE;
+                    if (ELEC) S0.cre[roff + k] = xe[k];
+                    if (PROT) S0.crp[roff + k] = xp[k];
+                }
             }
         }
     }
