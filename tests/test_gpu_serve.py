@@ -103,3 +103,43 @@ def test_cancel_queued():
         assert s.stats()["cancelled"] == 1
     finally:
         s.close()
+
+
+def test_many_small_odd_chunks():
+    """120 concurrent chunks with odd tracer counts (33..301) over all grid
+    sizes: the last wave of most launches holds lane groups past the end of
+    its task list, which must not disturb the live groups beside them (the
+    solve reads across groups and multiplies by zero).  Every chunk
+    finishes with finite spectra; chunks run alone equal their served
+    results, and two of them equal the oracle."""
+    rng = np.random.default_rng(77)
+    specs = [
+        dict(ntracer=33, n=64, species=ch.SPECIES_ELECTRON, dpp=ch.DPP_TTD, nsnap=2, seed=901, z0=0.5),
+        dict(ntracer=65, n=128, species=ch.SPECIES_ELECTRON_PROTON, dpp=ch.DPP_ASA, nsnap=2, seed=902, z0=0.7),
+    ]
+    for i in range(118):
+        specs.append(dict(ntracer=int(2 * rng.integers(16, 151) + 1), n=int(rng.choice([64, 128, 256, 512])),
+                          species=int(rng.choice([ch.SPECIES_ELECTRON, ch.SPECIES_ELECTRON_PROTON])),
+                          dpp=int(rng.choice([ch.DPP_TTD, ch.DPP_ASA, ch.DPP_OFF])), nsnap=2, seed=1000 + i,
+                          z0=float(rng.uniform(0.2, 0.8))))
+    s = ch.Server(2 * 2 ** 30, serve_stream.server_params())
+    try:
+        print(f"[serve] {len(specs)} odd-sized chunks", flush=True)
+        ids = [s.submit(**serve_stream.make_chunk(sp)[1]) for sp in specs]
+        served = [s.wait(i)[:2] for i in ids]
+        assert s.stats()["failed"] == 0
+        for crp, cre in served:
+            assert np.all(np.isfinite(cre)) and np.all(np.isfinite(crp))
+        for k in (0, 1, 2, 3):
+            crp, cre, _ = s.wait(s.submit(**serve_stream.make_chunk(specs[k])[1]))
+            assert np.array_equal(cre, served[k][1]) and np.array_equal(crp, served[k][0]), specs[k]
+    finally:
+        s.close()
+    for k in (0, 1):
+        crp_ref, cre_ref = oracle_result(specs[k])
+        e_err, e_ok = rel_err(served[k][1], cre_ref)
+        print(f"  chunk {k}: electron max rel err {e_err:.3e}", flush=True)
+        assert e_ok
+        if specs[k]["species"] != ch.SPECIES_ELECTRON:
+            p_err, p_ok = rel_err(served[k][0], crp_ref)
+            assert p_ok
